@@ -1,0 +1,174 @@
+/*
+ * racon_hip.h - C ABI of the MI355X (gfx950) polishing hot path.
+ *
+ * Plain pointers and sizes only. Every entry point returns 0 on success or a
+ * negative error code; rh_last_error() describes the last failure of the
+ * calling thread. A batch object is used by one host thread at a time; many
+ * batch objects may share the device.
+ *
+ * Reference interfaces these entry points replace (file:line in isovic/racon):
+ *   - POA consensus of windows: Window::generate_consensus(engine, trim)
+ *     (src/window.cpp:65-142), called per window from Polisher::polish
+ *     (src/polisher.cpp:486-504); the reference's device variant is
+ *     CUDABatchProcessor::addWindow / generateConsensus / getConsensus
+ *     (src/cuda/cudabatch.hpp:39-64, src/cuda/cudabatch.cpp:77-261).
+ *   - Overlap alignment: Overlap::align_overlaps -> edlibAlign(NW, PATH)
+ *     (src/overlap.cpp:205-224), called per overlap from
+ *     Polisher::find_overlap_breaking_points (src/polisher.cpp:462-484); the
+ *     reference's device variant is CUDABatchAligner::addOverlap / alignAll /
+ *     generate_cigar_strings (src/cuda/cudaaligner.hpp:34-66).
+ */
+#ifndef RACON_HIP_H_
+#define RACON_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors ---- */
+enum {
+  RH_OK = 0,
+  RH_ERR_ARGUMENT = -1,
+  RH_ERR_DEVICE = -2,
+  RH_ERR_MEMORY = -3,
+  RH_ERR_STATE = -4,
+  RH_ERR_KERNEL = -5
+};
+
+/* Message of the calling thread's last error ("" if none). */
+const char* rh_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int32_t rh_version(void);
+
+/* Number of visible HIP devices (0 if none or no runtime). */
+int32_t rh_device_count(void);
+
+/* ---- (b)+(c) window consensus ---- */
+
+/* One read layer of a window, in the window's insertion order
+ * (Window::add_layer, src/window.cpp:42-63). quality may be NULL (weight 1
+ * per base); begin/end are backbone positions, end inclusive. */
+typedef struct {
+  const char* sequence;
+  const char* quality;
+  uint32_t length;
+  uint32_t begin;
+  uint32_t end;
+} rh_layer;
+
+/* One window (createWindow, src/window.cpp:15-27). backbone_quality is never
+ * NULL (racon passes '!' x length when the target has no quality). tgs != 0
+ * for third-generation windows (trimming applies). */
+typedef struct {
+  const char* backbone;
+  const char* backbone_quality;
+  uint32_t backbone_length;
+  uint32_t n_layers;
+  const rh_layer* layers;
+  uint32_t tgs;
+} rh_window;
+
+/* Per-window outcome codes. */
+enum {
+  RH_WINDOW_UNPOLISHED = 0,         /* < 3 sequences: consensus = backbone */
+  RH_WINDOW_POLISHED = 1,
+  RH_WINDOW_POLISHED_UNTRIMMED = 2  /* racon's "might be chimeric" case   */
+};
+
+/* Kernel launch modes. */
+enum {
+  RH_POA_BATCHED = 0,  /* persistent kernel over a size-ordered work queue */
+  RH_POA_PLAIN = 1     /* one kernel launch per window                     */
+};
+
+typedef struct rh_poa_batch rh_poa_batch;
+
+typedef struct {
+  double pack_ms;       /* host packing of the batch                    */
+  double h2d_ms;        /* host -> device copy                          */
+  double kernel_ms;     /* device time of the consensus kernels (events)*/
+  double d2h_ms;        /* device -> host copy                          */
+  uint64_t cells;       /* POA DP cells: sum |V_aligned| * layer length */
+  uint64_t windows;     /* windows in the batch                         */
+  uint64_t gpu_windows; /* windows that needed the kernel (>= 3 seqs)   */
+  uint64_t retried;     /* windows rerun with larger scratch slots      */
+  uint64_t input_bytes; /* packed input bytes                           */
+  uint32_t slots;       /* resident waves (slots) of the last launch    */
+  uint32_t launches;    /* kernel launches of the last run              */
+} rh_poa_stats;
+
+/* Creates a batch on `device` with racon's scores (match > 0, mismatch and
+ * gap < 0; src/polisher.cpp:181-184) and the trim switch (-T). */
+int32_t rh_poa_create(int32_t device, int8_t match, int8_t mismatch, int8_t gap,
+                      int32_t trim, rh_poa_batch** batch);
+void rh_poa_destroy(rh_poa_batch* batch);
+
+/* Packs and copies `n` windows to the device. Layers are ordered inside the
+ * library exactly as Window::generate_consensus orders them (std::sort by
+ * begin, src/window.cpp:78-85). The input bytes are read during this call
+ * only. */
+int32_t rh_poa_upload(rh_poa_batch* batch, uint32_t n, const rh_window* windows);
+
+/* Runs the consensus of every uploaded window (blocking). Windows whose
+ * graph outgrows the default scratch slot are rerun in larger slots. */
+int32_t rh_poa_run(rh_poa_batch* batch, int32_t mode);
+
+/* Upper bound of the total consensus bytes of the uploaded batch. */
+uint64_t rh_poa_output_bound(const rh_poa_batch* batch);
+
+/* Copies results back: consensus of window i at out + offsets[i] with
+ * lengths[i] bytes, status[i] one of RH_WINDOW_*. */
+int32_t rh_poa_download(rh_poa_batch* batch, char* out, uint64_t capacity,
+                        uint64_t* offsets, uint32_t* lengths, uint8_t* status);
+
+int32_t rh_poa_get_stats(const rh_poa_batch* batch, rh_poa_stats* stats);
+
+/* Autotuning knobs (0 = default): resident slots of the persistent kernel
+ * and the DP row capacity of a default slot. */
+int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows);
+
+/* ---- (a) overlap alignment ---- */
+
+/* One pair in edlib's orientation: query = read span (reverse-complemented
+ * for '-' overlaps), target = contig span (src/overlap.cpp:193-197). */
+typedef struct {
+  const char* query;
+  uint32_t query_length;
+  const char* target;
+  uint32_t target_length;
+} rh_pair;
+
+typedef struct rh_align_batch rh_align_batch;
+
+typedef struct {
+  double pack_ms;
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+  uint64_t pairs;
+  uint64_t cells;       /* sum |q| * |t| (full-matrix equivalent) */
+  uint64_t ops;         /* total alignment operations             */
+  uint32_t launches;
+} rh_align_stats;
+
+int32_t rh_align_create(int32_t device, rh_align_batch** batch);
+void rh_align_destroy(rh_align_batch* batch);
+int32_t rh_align_upload(rh_align_batch* batch, uint32_t n, const rh_pair* pairs);
+int32_t rh_align_run(rh_align_batch* batch);
+/* Bound on the total number of ops (sum |q| + |t|). */
+uint64_t rh_align_output_bound(const rh_align_batch* batch);
+/* Edit operations of pair i at ops + offsets[i], n_ops[i] of them, encoded
+ * 0 match, 1 insertion, 2 deletion, 3 mismatch (edlib's alignment codes);
+ * edit_distance[i] is the NW distance. */
+int32_t rh_align_download(rh_align_batch* batch, uint8_t* ops, uint64_t capacity,
+                          uint64_t* offsets, uint32_t* n_ops, int32_t* edit_distance);
+int32_t rh_align_get_stats(const rh_align_batch* batch, rh_align_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RACON_HIP_H_ */

@@ -1,0 +1,105 @@
+"""ORACLE (test infrastructure only): ctypes access to oracle/_build/liboracle.so.
+
+Imported from tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+only, always as the checker, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+RACON_CPU = os.path.join(ORACLE_DIR, "_build", "racon_cpu")
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing at {LIB_PATH}: run make -C oracle")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.oracle_edlib_align.restype = ctypes.c_int32
+    lib.oracle_edlib_align.argtypes = [
+        ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+        ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+        ctypes.POINTER(ctypes.c_int32)]
+    lib.oracle_edit_distance.restype = ctypes.c_int32
+    lib.oracle_edit_distance.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p,
+                                         ctypes.c_int32]
+    lib.oracle_window_consensus.restype = ctypes.c_int32
+    lib.oracle_window_consensus.argtypes = [
+        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32,
+        ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+        ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+        ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, ctypes.c_uint64,
+        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+        ctypes.POINTER(ctypes.c_uint64)]
+    lib.oracle_sorted_layer_order.restype = None
+    lib.oracle_sorted_layer_order.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                              ctypes.POINTER(ctypes.c_uint32)]
+    _lib = lib
+    return lib
+
+
+def edlib_align(query: bytes, target: bytes, exact: bool = False) -> Tuple[int, bytes, int]:
+    """(edit distance, ops, status) of the edlib NW path restatement."""
+    lib = load()
+    cap = len(query) + len(target) + 1
+    ops = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64()
+    st = ctypes.c_int32()
+    d = lib.oracle_edlib_align(query, len(query), target, len(target), int(exact), ops, cap,
+                               ctypes.byref(n), ctypes.byref(st))
+    return d, ops.raw[:n.value], st.value
+
+
+def edit_distance(query: bytes, target: bytes) -> int:
+    return load().oracle_edit_distance(query, len(query), target, len(target))
+
+
+def layer_order(begins: Sequence[int]) -> List[int]:
+    """racon's layer order (std::sort by begin) as 0-based layer indices."""
+    lib = load()
+    n = len(begins)
+    arr = (ctypes.c_uint32 * max(1, n))(*begins)
+    out = (ctypes.c_uint32 * max(1, n))()
+    lib.oracle_sorted_layer_order(n, arr, out)
+    return [out[i] for i in range(n)]
+
+
+def window_consensus(backbone: bytes, backbone_quality: bytes, layers, tgs: bool,
+                     trim: bool, match: int, mismatch: int, gap: int,
+                     presorted: bool = False):
+    """(consensus, status, cells, graph nodes) of one window.
+
+    layers: sequence of (sequence, quality or None, begin, end) in insertion
+    order; they are put in racon's order first unless presorted.
+    """
+    lib = load()
+    if not presorted:
+        order = layer_order([l[2] for l in layers])
+        layers = [layers[i] for i in order]
+    n = len(layers)
+    seqs = (ctypes.c_char_p * max(1, n))(*[l[0] for l in layers])
+    quals = (ctypes.c_char_p * max(1, n))(*[l[1] for l in layers])
+    lens = (ctypes.c_uint32 * max(1, n))(*[len(l[0]) for l in layers])
+    begins = (ctypes.c_uint32 * max(1, n))(*[l[2] for l in layers])
+    ends = (ctypes.c_uint32 * max(1, n))(*[l[3] for l in layers])
+    cap = len(backbone) + sum(len(l[0]) for l in layers) + 16
+    out = ctypes.create_string_buffer(cap)
+    out_len = ctypes.c_uint64()
+    cells = ctypes.c_uint64()
+    nodes = ctypes.c_uint64()
+    st = lib.oracle_window_consensus(backbone, backbone_quality, len(backbone), n, seqs, quals,
+                                     lens, begins, ends, int(tgs), int(trim), match, mismatch,
+                                     gap, out, cap, ctypes.byref(out_len), ctypes.byref(cells),
+                                     ctypes.byref(nodes))
+    if st < 0:
+        raise RuntimeError("oracle window output overflow")
+    return out.raw[:out_len.value], st, cells.value, nodes.value

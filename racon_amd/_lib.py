@@ -1,0 +1,281 @@
+"""ctypes binding of libracon_hip.so (include/racon_hip.h).
+
+The product is the C ABI; this module only lets Python drive it (tests,
+benchmark, smoke run). Loading fails loudly when the library has not been
+built: there is no fallback implementation behind it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libracon_hip.so")
+RACON_BIN = os.path.join(PKG_DIR, "bin", "racon")
+
+RH_OK = 0
+RH_POA_BATCHED = 0
+RH_POA_PLAIN = 1
+RH_WINDOW_UNPOLISHED = 0
+RH_WINDOW_POLISHED = 1
+RH_WINDOW_POLISHED_UNTRIMMED = 2
+
+
+class RhLayer(ctypes.Structure):
+    _fields_ = [
+        ("sequence", ctypes.c_char_p),
+        ("quality", ctypes.c_char_p),
+        ("length", ctypes.c_uint32),
+        ("begin", ctypes.c_uint32),
+        ("end", ctypes.c_uint32),
+    ]
+
+
+class RhWindow(ctypes.Structure):
+    _fields_ = [
+        ("backbone", ctypes.c_char_p),
+        ("backbone_quality", ctypes.c_char_p),
+        ("backbone_length", ctypes.c_uint32),
+        ("n_layers", ctypes.c_uint32),
+        ("layers", ctypes.POINTER(RhLayer)),
+        ("tgs", ctypes.c_uint32),
+    ]
+
+
+class RhPoaStats(ctypes.Structure):
+    _fields_ = [
+        ("pack_ms", ctypes.c_double),
+        ("h2d_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("d2h_ms", ctypes.c_double),
+        ("cells", ctypes.c_uint64),
+        ("windows", ctypes.c_uint64),
+        ("gpu_windows", ctypes.c_uint64),
+        ("retried", ctypes.c_uint64),
+        ("input_bytes", ctypes.c_uint64),
+        ("slots", ctypes.c_uint32),
+        ("launches", ctypes.c_uint32),
+    ]
+
+
+class RhPair(ctypes.Structure):
+    _fields_ = [
+        ("query", ctypes.c_char_p),
+        ("query_length", ctypes.c_uint32),
+        ("target", ctypes.c_char_p),
+        ("target_length", ctypes.c_uint32),
+    ]
+
+
+class RhAlignStats(ctypes.Structure):
+    _fields_ = [
+        ("pack_ms", ctypes.c_double),
+        ("h2d_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("d2h_ms", ctypes.c_double),
+        ("pairs", ctypes.c_uint64),
+        ("cells", ctypes.c_uint64),
+        ("ops", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+    ]
+
+
+# Every entry point of include/racon_hip.h with its ctypes signature.
+_SIGNATURES = {
+    "rh_last_error": (ctypes.c_char_p, []),
+    "rh_version": (ctypes.c_int32, []),
+    "rh_device_count": (ctypes.c_int32, []),
+    "rh_poa_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int8, ctypes.c_int8,
+                                       ctypes.c_int8, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "rh_poa_destroy": (None, [ctypes.c_void_p]),
+    "rh_poa_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                       ctypes.POINTER(RhWindow)]),
+    "rh_poa_run": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "rh_poa_output_bound": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "rh_poa_download": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_uint8)]),
+    "rh_poa_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaStats)]),
+    "rh_poa_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                           ctypes.c_uint32]),
+    "rh_align_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "rh_align_destroy": (None, [ctypes.c_void_p]),
+    "rh_align_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                         ctypes.POINTER(RhPair)]),
+    "rh_align_run": (ctypes.c_int32, [ctypes.c_void_p]),
+    "rh_align_output_bound": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "rh_align_download": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64,
+                                           ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.POINTER(ctypes.c_uint32),
+                                           ctypes.POINTER(ctypes.c_int32)]),
+    "rh_align_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhAlignStats)]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libracon_hip.so; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"libracon_hip.so not found at {path}: build it first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    for name, (restype, argtypes) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != RH_OK:
+        msg = load_library().rh_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+@dataclass
+class Layer:
+    sequence: bytes
+    quality: Optional[bytes]
+    begin: int
+    end: int
+
+
+@dataclass
+class Window:
+    backbone: bytes
+    backbone_quality: bytes
+    layers: List[Layer]
+    tgs: bool = True
+
+
+class PoaBatch:
+    """A device batch of windows (rh_poa_*)."""
+
+    def __init__(self, match: int = 3, mismatch: int = -5, gap: int = -4,
+                 trim: bool = True, device: int = 0):
+        self.lib = load_library()
+        self.handle = ctypes.c_void_p()
+        _check(self.lib.rh_poa_create(device, match, mismatch, gap, int(trim),
+                                      ctypes.byref(self.handle)), "rh_poa_create")
+        self._keep = []
+        self.n = 0
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.rh_poa_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, slots: int = 0, rows: int = 0) -> None:
+        _check(self.lib.rh_poa_set_tuning(self.handle, slots, rows), "rh_poa_set_tuning")
+
+    def upload(self, windows: Sequence[Window]) -> None:
+        arr = (RhWindow * len(windows))()
+        keep = []
+        for i, w in enumerate(windows):
+            layers = (RhLayer * max(1, len(w.layers)))()
+            for k, l in enumerate(w.layers):
+                layers[k].sequence = l.sequence
+                layers[k].quality = l.quality
+                layers[k].length = len(l.sequence)
+                layers[k].begin = l.begin
+                layers[k].end = l.end
+            keep.append(layers)
+            arr[i].backbone = w.backbone
+            arr[i].backbone_quality = w.backbone_quality
+            arr[i].backbone_length = len(w.backbone)
+            arr[i].n_layers = len(w.layers)
+            arr[i].layers = layers
+            arr[i].tgs = 1 if w.tgs else 0
+        _check(self.lib.rh_poa_upload(self.handle, len(windows), arr), "rh_poa_upload")
+        self._keep = keep
+        self.n = len(windows)
+
+    def run(self, mode: int = RH_POA_BATCHED) -> None:
+        _check(self.lib.rh_poa_run(self.handle, mode), "rh_poa_run")
+
+    def download(self):
+        cap = self.lib.rh_poa_output_bound(self.handle)
+        out = ctypes.create_string_buffer(max(1, cap))
+        offsets = (ctypes.c_uint64 * max(1, self.n))()
+        lengths = (ctypes.c_uint32 * max(1, self.n))()
+        status = (ctypes.c_uint8 * max(1, self.n))()
+        _check(self.lib.rh_poa_download(self.handle, out, cap, offsets, lengths, status),
+               "rh_poa_download")
+        raw = out.raw
+        return ([raw[offsets[i]:offsets[i] + lengths[i]] for i in range(self.n)],
+                [status[i] for i in range(self.n)])
+
+    def stats(self) -> RhPoaStats:
+        st = RhPoaStats()
+        _check(self.lib.rh_poa_get_stats(self.handle, ctypes.byref(st)), "rh_poa_get_stats")
+        return st
+
+
+class AlignBatch:
+    """A device batch of pairwise NW alignments (rh_align_*)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        self.handle = ctypes.c_void_p()
+        _check(self.lib.rh_align_create(device, ctypes.byref(self.handle)), "rh_align_create")
+        self.n = 0
+        self._keep = None
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.rh_align_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, pairs: Sequence[tuple]) -> None:
+        arr = (RhPair * max(1, len(pairs)))()
+        for i, (q, t) in enumerate(pairs):
+            arr[i].query = q
+            arr[i].query_length = len(q)
+            arr[i].target = t
+            arr[i].target_length = len(t)
+        _check(self.lib.rh_align_upload(self.handle, len(pairs), arr), "rh_align_upload")
+        self._keep = (arr, list(pairs))
+        self.n = len(pairs)
+
+    def run(self) -> None:
+        _check(self.lib.rh_align_run(self.handle), "rh_align_run")
+
+    def download(self):
+        cap = self.lib.rh_align_output_bound(self.handle)
+        ops = ctypes.create_string_buffer(max(1, cap))
+        offsets = (ctypes.c_uint64 * max(1, self.n))()
+        n_ops = (ctypes.c_uint32 * max(1, self.n))()
+        dist = (ctypes.c_int32 * max(1, self.n))()
+        _check(self.lib.rh_align_download(self.handle, ops, cap, offsets, n_ops, dist),
+               "rh_align_download")
+        raw = ops.raw
+        return ([raw[offsets[i]:offsets[i] + n_ops[i]] for i in range(self.n)],
+                [dist[i] for i in range(self.n)])
+
+    def stats(self) -> RhAlignStats:
+        st = RhAlignStats()
+        _check(self.lib.rh_align_get_stats(self.handle, ctypes.byref(st)),
+               "rh_align_get_stats")
+        return st

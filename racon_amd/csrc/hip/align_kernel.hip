@@ -1,0 +1,452 @@
+// Global (NW) edit-distance alignment of racon overlaps on gfx950.
+//
+// Reproduces the alignment path racon gets from edlib (EDLIB_MODE_NW,
+// EDLIB_TASK_PATH; src/overlap.cpp:205-224). The path is defined by a
+// recursion on the target: a subproblem (query span, target span, score)
+// whose stored columns would take (2*8+4) * ceil(|q|/64) * |t| < 1 MiB is
+// traced back directly from its last cell, preferring up (insertion), then
+// left (deletion), then the diagonal; a larger one is split at column
+// |t|/2 at the first query row x where the forward score D(q[..x], t[..|t|/2))
+// plus the reverse score of the rest equals the subproblem score (then x = -1,
+// then x = |q|-1), and the two halves are solved in order.
+//
+// Every score here is exact (no band), which gives the same path as edlib's
+// banded computation: banding only drops cells whose score cannot be on an
+// optimal path, and every choice above reads cells on optimal paths.
+//
+// Myers' bit-vector recurrence runs on 64-row blocks, one block per lane, in
+// an anti-diagonal pipeline: at step s lane b advances column s - b and hands
+// its horizontal carry to lane b + 1 through a lane shuffle. Queries longer
+// than 64 blocks are processed in strips of 64 blocks whose bottom carries
+// pass through the slot. One wave owns one overlap; a persistent grid takes
+// overlaps from a work queue ordered by size.
+#include <hip/hip_runtime.h>
+
+#include "align_types.hpp"
+
+namespace rh {
+
+namespace {
+
+constexpr int kWave = 64;
+typedef uint64_t Word;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// One Myers/Hyyro block step with horizontal input hin in {-1, 0, +1}.
+__device__ __forceinline__ int block_step(Word& P, Word& M, Word eq, int hin) {
+  const Word hin_neg = hin < 0 ? Word(1) : Word(0);
+  const Word xv = eq | M;
+  eq |= hin_neg;
+  const Word xh = (((eq & P) + P) ^ P) | eq;
+  Word ph = M | ~(xh | P);
+  Word mh = P & xh;
+  int hout = 0;
+  if (ph >> 63) hout = 1;
+  if (mh >> 63) hout = -1;
+  ph <<= 1;
+  mh <<= 1;
+  mh |= hin_neg;
+  ph |= hin > 0 ? Word(1) : Word(0);
+  P = mh | ~(xv | ph);
+  M = ph & xv;
+  return hout;
+}
+
+struct Seg {
+  const uint8_t* q;  // query span
+  const uint8_t* t;  // target span
+  int ql;
+  int tl;
+  bool rev;  // read both spans from the end
+  __device__ uint8_t qc(int i) const { return rev ? q[ql - 1 - i] : q[i]; }
+  __device__ uint8_t tc(int j) const { return rev ? t[tl - 1 - j] : t[j]; }
+};
+
+// Match words of one block for A, C, G, T and N; rows past the query end
+// match everything.
+struct Peq {
+  Word w[5];
+  Word pad;  // rows beyond the query
+};
+
+__device__ __forceinline__ int code_of(uint8_t c) {
+  switch (c) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    case 'N': return 4;
+    default: return 5;
+  }
+}
+
+__device__ Peq build_peq(const Seg& g, int block) {
+  Peq p;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) p.w[k] = 0;
+  p.pad = 0;
+  const int r0 = block * 64;
+  for (int i = 0; i < 64; ++i) {
+    const int r = r0 + i;
+    const Word bit = Word(1) << i;
+    if (r >= g.ql) {
+      p.pad |= bit;
+      continue;
+    }
+    const int c = code_of(g.qc(r));
+    if (c < 5) p.w[c] |= bit;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) p.w[k] |= p.pad;
+  return p;
+}
+
+// Match word for a target byte outside A/C/G/T/N (exact byte compare).
+__device__ Word rare_eq(const Seg& g, int block, uint8_t c, Word pad) {
+  Word w = pad;
+  const int r0 = block * 64;
+  for (int i = 0; i < 64; ++i) {
+    const int r = r0 + i;
+    if (r < g.ql && g.qc(r) == c) w |= Word(1) << i;
+  }
+  return w;
+}
+
+struct SlotA {
+  int32_t* col_left;
+  int32_t* col_right;
+  int8_t* carry;
+  Word* store_p;
+  Word* store_m;
+  int32_t* store_s;
+  uint8_t* ops_tmp;
+};
+
+// Runs the bit-vector DP of seg over all columns. With stop >= 0, writes the
+// exact scores of column `stop` for rows 0..ql-1 into col. With store, keeps
+// (P, M, score) of every (column, block) for the traceback. Returns the score
+// of the last cell (row ql-1, column tl-1) when stop < 0.
+__device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const SlotA& s) {
+  const int lane = lane_id();
+  const int nb = (g.ql + 63) / 64;
+  const int strips = (nb + kWave - 1) / kWave;
+  int last_score = 0;
+  for (int k = 0; k < strips; ++k) {
+    const int b = k * kWave + lane;
+    const int nbs = min(kWave, nb - k * kWave);  // blocks in this strip
+    const bool active_lane = lane < nbs;
+    Peq peq;
+    if (active_lane) {
+      peq = build_peq(g, b);
+    }
+    Word P = ~Word(0), M = 0;
+    int score = (b + 1) * 64;  // D at the block's last row before column 0
+    int hprev = 0;             // carry received from the lane above
+    const int steps = g.tl + nbs - 1;
+    const uint64_t store_base = static_cast<uint64_t>(k) * (g.tl + 63) * 64;
+    for (int st = 0; st < steps; ++st) {
+      const int c = st - lane;
+      const int hup = __shfl_up(hprev, 1, kWave);
+      int hin;
+      if (lane == 0) {
+        hin = k == 0 ? 1 : (c >= 0 && c < g.tl ? static_cast<int>(s.carry[c]) : 1);
+      } else {
+        hin = hup;
+      }
+      int hout = 0;
+      if (active_lane && c >= 0 && c < g.tl) {
+        const uint8_t tc = g.tc(c);
+        const int code = code_of(tc);
+        const Word eq = code < 5 ? peq.w[code] : rare_eq(g, b, tc, peq.pad);
+        hout = block_step(P, M, eq, hin);
+        score += hout;
+        if (store) {
+          const uint64_t idx = store_base + static_cast<uint64_t>(st) * nbs + lane;
+          s.store_p[idx] = P;
+          s.store_m[idx] = M;
+          s.store_s[idx] = score;
+        }
+        if (c == stop) {
+          // exact values of this block's rows at the stop column
+          int v = score;
+          for (int i = 63; i >= 0; --i) {
+            const int r = b * 64 + i;
+            if (r < g.ql) col[r] = v;
+            if ((P >> i) & 1) v -= 1;
+            if ((M >> i) & 1) v += 1;
+          }
+        }
+        if (lane == nbs - 1 && k + 1 < strips) {
+          s.carry[c] = static_cast<int8_t>(hout);
+        }
+      }
+      hprev = hout;
+    }
+    // score of the last row (in the last strip's last block)
+    if (k == strips - 1) {
+      const int last_block = nb - 1;
+      const int last_bit = (g.ql - 1) % 64;
+      int v = score;
+      for (int i = 63; i > last_bit; --i) {
+        if ((P >> i) & 1) v -= 1;
+        if ((M >> i) & 1) v += 1;
+      }
+      last_score = __shfl(v, last_block - k * kWave, kWave);
+    }
+    wave_sync();  // carries and stored columns visible to every lane
+  }
+  return last_score;
+}
+
+// D[r][c] of the stored leaf DP (r, c >= 0).
+__device__ int stored_value(const Seg& g, const SlotA& s, int r, int c) {
+  const int nb = (g.ql + 63) / 64;
+  const int b = r / 64;
+  const int k = b / kWave;
+  const int lane = b % kWave;
+  const int nbs = min(kWave, nb - k * kWave);
+  const uint64_t idx = static_cast<uint64_t>(k) * (g.tl + 63) * 64 +
+                       static_cast<uint64_t>(c + lane) * nbs + lane;
+  const Word P = s.store_p[idx];
+  const Word M = s.store_m[idx];
+  int v = s.store_s[idx];
+  const int bit = r % 64;
+  if (bit < 63) {
+    const Word above = ~Word(0) << (bit + 1);
+    v -= __popcll(P & above);
+    v += __popcll(M & above);
+  }
+  return v;
+}
+
+// Boundary-aware D value: row -1 is c + 1, column -1 is r + 1.
+__device__ int dval(const Seg& g, const SlotA& s, int r, int c) {
+  if (r < 0) return c + 1;
+  if (c < 0) return r + 1;
+  return stored_value(g, s, r, c);
+}
+
+// Traceback of a stored leaf; writes ops (forward order) at out, returns count.
+__device__ int leaf_traceback(const Seg& g, const SlotA& s, uint8_t* out) {
+  int r = g.ql - 1, c = g.tl - 1;
+  int n = 0;
+  uint8_t* tmp = s.ops_tmp;
+  while (r >= 0 && c >= 0) {
+    const int cur = dval(g, s, r, c);
+    const int up = dval(g, s, r - 1, c);
+    if (up + 1 == cur) {
+      tmp[n++] = 1;  // insertion
+      --r;
+      continue;
+    }
+    const int left = dval(g, s, r, c - 1);
+    if (left + 1 == cur) {
+      tmp[n++] = 2;  // deletion
+      --c;
+      continue;
+    }
+    const int diag = dval(g, s, r - 1, c - 1);
+    tmp[n++] = diag == cur ? 0 : 3;
+    --r;
+    --c;
+  }
+  while (c >= 0) {
+    tmp[n++] = 2;
+    --c;
+  }
+  while (r >= 0) {
+    tmp[n++] = 1;
+    --r;
+  }
+  wave_sync();
+  for (int i = lane_id(); i < n; i += kWave) {
+    out[i] = tmp[n - 1 - i];
+  }
+  wave_sync();
+  return n;
+}
+
+struct Task {
+  int q0, ql, t0, tl, score;
+};
+
+__device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResult& R) {
+  const int lane = lane_id();
+  uint8_t* slot = A.scratch + static_cast<uint64_t>(blockIdx.x) * A.slot_bytes;
+  SlotA s;
+  s.col_left = reinterpret_cast<int32_t*>(slot + A.layout.col_left);
+  s.col_right = reinterpret_cast<int32_t*>(slot + A.layout.col_right);
+  s.carry = reinterpret_cast<int8_t*>(slot + A.layout.carry);
+  s.store_p = reinterpret_cast<Word*>(slot + A.layout.store_p);
+  s.store_m = reinterpret_cast<Word*>(slot + A.layout.store_m);
+  s.store_s = reinterpret_cast<int32_t*>(slot + A.layout.store_s);
+  s.ops_tmp = slot + A.layout.ops_tmp;
+
+  const uint8_t* Q = A.arena + pr.q_off;
+  const uint8_t* T = A.arena + pr.t_off;
+  uint8_t* out = A.ops + pr.out_off;
+  const int QL = static_cast<int>(pr.q_len), TL = static_cast<int>(pr.t_len);
+  if (QL > static_cast<int>(A.caps.q) || TL > static_cast<int>(A.caps.t)) {
+    R.status = kAlignScratch;
+    return;
+  }
+
+  auto is_leaf = [](int ql, int tl) {
+    const uint64_t nb = (static_cast<uint64_t>(ql) + 63) / 64;
+    return 20ull * nb * static_cast<uint64_t>(tl) < kLeafBytes;
+  };
+
+  __shared__ Task stack[kAlignStackDepth];
+  int top = 0;
+  int n_ops = 0;
+  int best = -1;
+
+  if (QL == 0 || TL == 0) {
+    for (int i = lane; i < QL + TL; i += kWave) out[i] = QL == 0 ? 2 : 1;
+    R.n_ops = static_cast<uint32_t>(QL + TL);
+    R.edit_distance = QL > TL ? QL : TL;
+    R.status = kAlignOk;
+    return;
+  }
+  if (is_leaf(QL, TL)) {
+    Seg g{Q, T, QL, TL, false};
+    best = forward(g, -1, nullptr, true, s);
+    n_ops = leaf_traceback(g, s, out);
+    R.n_ops = static_cast<uint32_t>(n_ops);
+    R.edit_distance = best;
+    R.status = kAlignOk;
+    return;
+  }
+  // top level: the Hirschberg split also yields the distance
+  if (lane == 0) {
+    stack[0] = Task{0, QL, 0, TL, -1};
+  }
+  top = 1;
+  wave_sync();
+  while (top > 0) {
+    const Task tk = stack[top - 1];
+    wave_sync();
+    --top;
+    const int ql = tk.ql, tl = tk.tl;
+    if (ql == 0 || tl == 0) {
+      for (int i = lane; i < ql + tl; i += kWave) out[n_ops + i] = ql == 0 ? 2 : 1;
+      n_ops += ql + tl;
+      wave_sync();
+      continue;
+    }
+    if (is_leaf(ql, tl)) {
+      Seg g{Q + tk.q0, T + tk.t0, ql, tl, false};
+      forward(g, -1, nullptr, true, s);
+      n_ops += leaf_traceback(g, s, out + n_ops);
+      continue;
+    }
+    const int lw = tl / 2, rw = tl - lw;
+    // forward over the left half: scores of column lw-1
+    Seg gl{Q + tk.q0, T + tk.t0, ql, lw, false};
+    forward(gl, lw - 1, s.col_left, false, s);
+    // reverse over the right half: col_right[r'] = ED of the last r'+1 query
+    // bytes against t[lw..)
+    Seg gr{Q + tk.q0, T + tk.t0 + lw, ql, rw, true};
+    forward(gr, rw - 1, s.col_right, false, s);
+    wave_sync();
+    // right(x) = ED(q[x..], t[lw..]) = col_right[ql-1-x] for x < ql, rw at ql
+    int score = tk.score;
+    if (score < 0) {
+      // distance of the whole pair: min over all split rows
+      int m = 0x7fffffff;
+      for (int x = lane - 1; x <= ql - 1; x += kWave) {
+        const int lv = x < 0 ? lw : s.col_left[x];
+        const int rv = x + 1 >= ql ? rw : s.col_right[ql - 2 - x];
+        m = min(m, lv + rv);
+      }
+      for (int d = 1; d < kWave; d <<= 1) m = min(m, __shfl_xor(m, d, kWave));
+      score = uniform(m);
+      best = score;
+    }
+    // first interior split row x in [0, ql-2]
+    int split = -2;
+    for (int x0 = 0; x0 <= ql - 2 && split == -2; x0 += kWave) {
+      const int x = x0 + lane;
+      bool hit = false;
+      if (x <= ql - 2) {
+        hit = s.col_left[x] + s.col_right[ql - 2 - x] == score;
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (mask != 0) {
+        split = x0 + __ffsll(static_cast<long long>(mask)) - 1;
+      }
+    }
+    int ls = 0, rs = 0;
+    if (split != -2) {
+      ls = s.col_left[split];
+      rs = s.col_right[ql - 2 - split];
+    } else if (lw + s.col_right[ql - 1] == score) {
+      split = -1;
+      ls = lw;
+      rs = s.col_right[ql - 1];
+    } else if (s.col_left[ql - 1] + rw == score) {
+      split = ql - 1;
+      ls = s.col_left[ql - 1];
+      rs = rw;
+    } else {
+      R.status = kAlignNoSplit;
+      return;
+    }
+    split = uniform(split);
+    ls = uniform(ls);
+    rs = uniform(rs);
+    if (top + 2 > static_cast<int>(kAlignStackDepth)) {
+      R.status = kAlignStack;
+      return;
+    }
+    wave_sync();
+    if (lane == 0) {
+      stack[top] = Task{tk.q0 + split + 1, ql - split - 1, tk.t0 + lw, rw, rs};
+      stack[top + 1] = Task{tk.q0, split + 1, tk.t0, lw, ls};
+    }
+    top += 2;
+    wave_sync();
+  }
+  R.n_ops = static_cast<uint32_t>(n_ops);
+  R.edit_distance = best;
+  R.status = kAlignOk;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(64) void align_batch_kernel(AlignParams A) {
+  __shared__ uint32_t next;
+  while (true) {
+    if (threadIdx.x == 0) {
+      next = atomicAdd(A.queue_head, 1u);
+    }
+    __syncthreads();
+    const uint32_t q = next;
+    __syncthreads();
+    if (q >= A.n_order) {
+      break;
+    }
+    const uint32_t i = A.order[q];
+    AlignResult res{0, -1, kAlignNotRun, 0};
+    align_pair(A, A.pairs[i], res);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      A.results[i] = res;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_align_batch(const AlignParams& params, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL(align_batch_kernel, dim3(grid), dim3(64), 0, stream, params);
+  return hipGetLastError();
+}
+
+hipError_t align_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, align_batch_kernel, 64, 0);
+}
+
+}  // namespace rh

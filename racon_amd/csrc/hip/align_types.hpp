@@ -1,0 +1,88 @@
+// Shared host/device layouts of the overlap-alignment batch.
+#pragma once
+
+#include <cstdint>
+
+namespace rh {
+
+struct AlignPair {
+  uint64_t q_off;   // arena offset of the query bytes
+  uint64_t t_off;   // arena offset of the target bytes
+  uint64_t out_off; // output offset of the ops (capacity q_len + t_len)
+  uint32_t q_len;
+  uint32_t t_len;
+};
+
+struct AlignResult {
+  uint32_t n_ops;
+  int32_t edit_distance;
+  int32_t status;  // 0 ok, < 0 error
+  uint32_t pad;
+};
+
+enum : int32_t {
+  kAlignOk = 0,
+  kAlignNoSplit = -1,    // no optimal split found (inconsistent score)
+  kAlignStack = -2,      // subproblem stack exhausted
+  kAlignScratch = -3,    // scratch slot too small
+  kAlignNotRun = -100,
+};
+
+// Leaf size rule of the path search: a subproblem is traced back directly
+// when (2 * 8 + 4) * ceil(|q| / 64) * |t| < 1 MiB, else split at |t| / 2.
+constexpr uint64_t kLeafBytes = 1024 * 1024;
+constexpr uint32_t kAlignStackDepth = 128;
+
+struct AlignSlotLayout {
+  uint64_t col_left;   // int32 [q cap]
+  uint64_t col_right;  // int32 [q cap]
+  uint64_t carry;      // int8  [t cap + 64]
+  uint64_t store_p;    // uint64 [store words]
+  uint64_t store_m;    // uint64 [store words]
+  uint64_t store_s;    // int32  [store words]
+  uint64_t ops_tmp;    // uint8 [q cap + t cap]
+  uint64_t total;
+};
+
+struct AlignSlotCaps {
+  uint32_t q;            // longest query
+  uint32_t t;            // longest target
+  uint64_t store_words;  // stored (column, block) entries for a leaf
+};
+
+inline uint64_t align_up_a(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+inline AlignSlotLayout align_slot_layout(const AlignSlotCaps& c) {
+  AlignSlotLayout l{};
+  uint64_t o = 0;
+  auto take = [&](uint64_t bytes) {
+    uint64_t at = o;
+    o = align_up_a(o + bytes, 256);
+    return at;
+  };
+  l.col_left = take(4ull * (c.q + 64));
+  l.col_right = take(4ull * (c.q + 64));
+  l.carry = take(uint64_t(c.t) + 128);
+  l.store_p = take(8ull * c.store_words);
+  l.store_m = take(8ull * c.store_words);
+  l.store_s = take(4ull * c.store_words);
+  l.ops_tmp = take(uint64_t(c.q) + c.t + 64);
+  l.total = o;
+  return l;
+}
+
+struct AlignParams {
+  const AlignPair* pairs;
+  const uint8_t* arena;
+  uint8_t* ops;
+  AlignResult* results;
+  const uint32_t* order;
+  uint32_t n_order;
+  uint32_t* queue_head;
+  uint8_t* scratch;
+  uint64_t slot_bytes;
+  AlignSlotCaps caps;
+  AlignSlotLayout layout;
+};
+
+}  // namespace rh

@@ -1,0 +1,145 @@
+// Shared host/device layouts of the POA batch (windows, layers, per-slot
+// scratch). Plain structs only; no HIP or STL types.
+#pragma once
+
+#include <cstdint>
+
+namespace rh {
+
+// One window as packed in device memory (layers in racon's alignment order).
+struct PoaWindow {
+  uint64_t backbone_off;   // arena offset of the backbone bytes
+  uint64_t backbone_qual;  // arena offset of the backbone quality bytes
+  uint64_t out_off;        // output arena offset
+  uint32_t backbone_len;
+  uint32_t n_layers;
+  uint32_t layer_first;    // index of the first layer in the layer table
+  uint32_t offset;         // (uint32)(0.01 * backbone_len), window.cpp:87
+  uint32_t flags;          // bit 0: TGS window, bit 1: trim
+  uint32_t out_cap;        // bytes reserved for the consensus
+  uint32_t node_bound;     // backbone_len + sum of layer lengths
+  uint32_t edge_bound;     // upper bound on edges
+  uint32_t max_layer_len;
+  uint32_t pad;
+};
+
+struct PoaLayer {
+  uint64_t seq_off;   // arena offset of the layer bytes
+  uint64_t qual_off;  // arena offset of the quality bytes, or kNoQuality
+  uint32_t len;
+  uint32_t begin;
+  uint32_t end;
+  uint32_t pad;
+};
+
+constexpr uint64_t kNoQuality = ~uint64_t(0);
+
+// Per-window result written by the kernel.
+struct PoaResult {
+  uint32_t length;     // consensus bytes written at out_off
+  int32_t status;      // kPoa* below
+  uint64_t cells;      // DP cells computed (sum |V_aligned| * L)
+  uint32_t nodes;      // final graph size
+  uint32_t max_rows;   // largest DP row count used
+};
+
+enum : int32_t {
+  kPoaPolished = 1,
+  kPoaPolishedUntrimmed = 2,   // trimming would empty it (racon warns)
+  kPoaNodeOverflow = -1,       // graph scratch too small: rerun with bigger slot
+  kPoaRowOverflow = -2,        // DP matrix scratch too small
+  kPoaOutOverflow = -3,        // consensus longer than out_cap
+  kPoaLoopLimit = -4,          // heaviest-bundle completion did not end
+  kPoaBadInput = -5,           // layer end outside the graph
+  kPoaNotRun = -100,
+};
+
+// Capacities of one scratch slot.
+struct PoaSlotCaps {
+  uint32_t nodes;   // graph nodes
+  uint32_t edges;   // graph edges
+  uint32_t rows;    // DP rows (subgraph nodes + 1)
+  uint32_t cols;    // DP row pitch in elements (>= max layer length + 1, padded)
+  uint32_t stack;   // DFS stack entries
+  uint32_t pad;
+};
+
+constexpr uint32_t kMaxAligned = 8;  // aligned-set entries kept per node
+
+// Byte offsets of every array inside one slot.
+struct PoaSlotLayout {
+  uint64_t base, in_head, out_head, in_tail, out_tail, n_aligned, aligned, cov,
+      last_label, e_tail, e_head, e_weight, e_next_in, e_next_out, rank_to_node,
+      node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
+      sink, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, total;
+};
+
+inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
+  PoaSlotLayout l{};
+  uint64_t o = 0;
+  auto take = [&](uint64_t bytes) {
+    uint64_t at = o;
+    o = align_up(o + bytes, 256);
+    return at;
+  };
+  const uint64_t n = c.nodes, e = c.edges;
+  l.base = take(n);
+  l.in_head = take(4 * n);
+  l.out_head = take(4 * n);
+  l.in_tail = take(4 * n);
+  l.out_tail = take(4 * n);
+  l.n_aligned = take(n);
+  l.aligned = take(4 * n * kMaxAligned);
+  l.cov = take(4 * n);
+  l.last_label = take(4 * n);
+  l.e_tail = take(4 * e);
+  l.e_head = take(4 * e);
+  l.e_weight = take(4 * e);
+  l.e_next_in = take(4 * e);
+  l.e_next_out = take(4 * e);
+  l.rank_to_node = take(4 * n);
+  l.node_rank = take(4 * n);
+  l.mark = take(n);
+  l.lead = take(n);
+  l.stack = take(4 * uint64_t(c.stack));
+  l.in_sub = take(n);
+  l.row_of = take(4 * n);
+  l.sub_order = take(4 * n);
+  l.pred_off = take(4 * (n + 1));
+  l.pred_rows = take(4 * (e + n));
+  l.sink = take(n);
+  l.H = take(4 * uint64_t(c.rows) * c.cols);
+  l.code = take(2 * uint64_t(c.rows) * c.cols);
+  l.aln_node = take(4 * (n + c.cols));
+  l.aln_pos = take(4 * (n + c.cols));
+  l.scores = take(8 * n);
+  l.preds = take(4 * n);
+  l.path = take(4 * n);
+  l.seq = take(c.cols);
+  l.qual = take(4 * uint64_t(c.cols));
+  l.total = o;
+  return l;
+}
+
+struct PoaParams {
+  const PoaWindow* windows;
+  const PoaLayer* layers;
+  const uint8_t* arena;
+  uint8_t* out;
+  PoaResult* results;
+  const uint32_t* order;     // processing order of window indices
+  uint32_t n_order;          // number of entries in order
+  uint32_t* queue_head;      // work counter (persistent mode)
+  uint8_t* scratch;          // slots
+  uint64_t slot_bytes;
+  PoaSlotCaps caps;
+  PoaSlotLayout layout;
+  int32_t match;
+  int32_t mismatch;
+  int32_t gap;
+  uint32_t n_slots;
+};
+
+}  // namespace rh

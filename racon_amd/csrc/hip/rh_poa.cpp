@@ -1,0 +1,551 @@
+// C ABI: window consensus batches (racon_hip.h, rh_poa_*).
+//
+// Host side of the POA path: orders each window's layers the way
+// Window::generate_consensus does (std::sort by begin, src/window.cpp:78-85),
+// packs windows into one device arena, sizes per-wave scratch slots, runs the
+// persistent kernel over a work queue ordered by estimated cost (largest
+// first) or one launch per window, and reruns the few windows whose graph
+// outgrew a default slot in slots sized from their exact bounds.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "poa_types.hpp"
+#include "rh_common.hpp"
+
+namespace rh {
+
+hipError_t launch_poa_batch(const PoaParams& params, uint32_t grid, hipStream_t stream);
+hipError_t launch_poa_single(const PoaParams& params, uint32_t q, hipStream_t stream);
+hipError_t poa_occupancy(int* blocks_per_cu);
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& message) { g_error = message; }
+void clear_error() { g_error.clear(); }
+const char* last_error() { return g_error.c_str(); }
+
+}  // namespace rh
+
+struct rh_poa_batch {
+  int32_t device = 0;
+  int32_t match = 3, mismatch = -5, gap = -4;
+  bool trim = true;
+  uint32_t tune_slots = 0;
+  uint32_t tune_rows = 0;
+
+  // host copies of the packed batch
+  std::vector<uint8_t> arena;
+  std::vector<rh::PoaWindow> windows;
+  std::vector<rh::PoaLayer> layers;
+  std::vector<uint32_t> gpu_order;        // windows that need the kernel
+  std::vector<uint8_t> shallow;           // 1: < 3 sequences
+  std::vector<uint64_t> backbone_off;     // arena offset of each backbone
+  std::vector<uint32_t> backbone_len;
+  std::vector<rh::PoaResult> results;
+  uint64_t out_bytes = 0;
+
+  rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results, d_order, d_queue,
+      d_scratch;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool uploaded = false;
+  bool ran = false;
+  rh_poa_stats stats{};
+};
+
+namespace {
+
+using rh::PoaLayer;
+using rh::PoaParams;
+using rh::PoaResult;
+using rh::PoaSlotCaps;
+using rh::PoaWindow;
+
+bool is_retry_status(int32_t s) {
+  return s == rh::kPoaNodeOverflow || s == rh::kPoaRowOverflow || s == rh::kPoaOutOverflow;
+}
+
+// Estimated cost of a window: layers x backbone x graph growth.
+uint64_t window_cost(const PoaWindow& w) {
+  return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
+}
+
+int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaSlotCaps& caps,
+                 uint32_t max_slots, int32_t mode, double* kernel_ms) {
+  if (order.empty()) {
+    return RH_OK;
+  }
+  const rh::PoaSlotLayout layout = rh::poa_slot_layout(caps);
+  const uint64_t slot_bytes = rh::align_up(layout.total, 4096);
+
+  size_t free_b = 0, total_b = 0;
+  RH_HIP_CHECK(hipMemGetInfo(&free_b, &total_b), RH_ERR_DEVICE);
+  // keep the slots within ~60% of free memory (plus what we already hold)
+  const uint64_t budget = static_cast<uint64_t>((free_b + b->d_scratch.bytes) * 0.6);
+  uint64_t slots = std::max<uint64_t>(1, budget / slot_bytes);
+  slots = std::min<uint64_t>(slots, max_slots);
+  slots = std::min<uint64_t>(slots, order.size());
+  if (mode == RH_POA_PLAIN) {
+    slots = 1;
+  }
+  RH_HIP_CHECK(b->d_scratch.reserve(slots * slot_bytes), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_order.reserve(order.size() * sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(hipMemcpyAsync(b->d_order.ptr, order.data(), order.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, b->stream),
+               RH_ERR_DEVICE);
+  RH_HIP_CHECK(b->d_queue.reserve(sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(hipMemsetAsync(b->d_queue.ptr, 0, sizeof(uint32_t), b->stream), RH_ERR_DEVICE);
+
+  PoaParams p{};
+  p.windows = b->d_windows.as<PoaWindow>();
+  p.layers = b->d_layers.as<PoaLayer>();
+  p.arena = b->d_arena.as<uint8_t>();
+  p.out = b->d_out.as<uint8_t>();
+  p.results = b->d_results.as<PoaResult>();
+  p.order = b->d_order.as<uint32_t>();
+  p.n_order = static_cast<uint32_t>(order.size());
+  p.queue_head = b->d_queue.as<uint32_t>();
+  p.scratch = b->d_scratch.as<uint8_t>();
+  p.slot_bytes = slot_bytes;
+  p.caps = caps;
+  p.layout = layout;
+  p.match = b->match;
+  p.mismatch = b->mismatch;
+  p.gap = b->gap;
+  p.n_slots = static_cast<uint32_t>(slots);
+
+  RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
+  if (mode == RH_POA_PLAIN) {
+    for (uint32_t q = 0; q < order.size(); ++q) {
+      RH_HIP_CHECK(rh::launch_poa_single(p, q, b->stream), RH_ERR_KERNEL);
+      b->stats.launches++;
+    }
+  } else {
+    RH_HIP_CHECK(rh::launch_poa_batch(p, static_cast<uint32_t>(slots), b->stream),
+                 RH_ERR_KERNEL);
+    b->stats.launches++;
+  }
+  RH_HIP_CHECK(hipEventRecord(b->ev1, b->stream), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipEventSynchronize(b->ev1), RH_ERR_KERNEL);
+  float ms = 0;
+  RH_HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1), RH_ERR_DEVICE);
+  *kernel_ms += ms;
+  b->stats.slots = static_cast<uint32_t>(slots);
+  return RH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rh_last_error(void) { return rh::last_error(); }
+
+int32_t rh_version(void) { return 100; }
+
+int32_t rh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    return 0;
+  }
+  return n;
+}
+
+int32_t rh_poa_create(int32_t device, int8_t match, int8_t mismatch, int8_t gap, int32_t trim,
+                      rh_poa_batch** batch) {
+  rh::clear_error();
+  if (batch == nullptr) {
+    rh::set_error("rh_poa_create: null batch pointer");
+    return RH_ERR_ARGUMENT;
+  }
+  *batch = nullptr;
+  int n = 0;
+  RH_HIP_CHECK(hipGetDeviceCount(&n), RH_ERR_DEVICE);
+  if (device < 0 || device >= n) {
+    rh::set_error("rh_poa_create: no such device");
+    return RH_ERR_DEVICE;
+  }
+  RH_HIP_CHECK(hipSetDevice(device), RH_ERR_DEVICE);
+  rh_poa_batch* b = new rh_poa_batch();
+  b->device = device;
+  b->match = match;
+  b->mismatch = mismatch;
+  b->gap = gap;
+  b->trim = trim != 0;
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess) {
+    rh::set_error("rh_poa_create: stream/event creation failed");
+    delete b;
+    return RH_ERR_DEVICE;
+  }
+  *batch = b;
+  return RH_OK;
+}
+
+void rh_poa_destroy(rh_poa_batch* b) {
+  if (b == nullptr) {
+    return;
+  }
+  (void)hipSetDevice(b->device);
+  if (b->stream != nullptr) {
+    (void)hipStreamSynchronize(b->stream);
+  }
+  b->d_arena.release();
+  b->d_windows.release();
+  b->d_layers.release();
+  b->d_out.release();
+  b->d_results.release();
+  b->d_order.release();
+  b->d_queue.release();
+  b->d_scratch.release();
+  if (b->ev0 != nullptr) (void)hipEventDestroy(b->ev0);
+  if (b->ev1 != nullptr) (void)hipEventDestroy(b->ev1);
+  if (b->stream != nullptr) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int32_t rh_poa_set_tuning(rh_poa_batch* b, uint32_t slots, uint32_t rows) {
+  if (b == nullptr) {
+    rh::set_error("rh_poa_set_tuning: null batch");
+    return RH_ERR_ARGUMENT;
+  }
+  b->tune_slots = slots;
+  b->tune_rows = rows;
+  return RH_OK;
+}
+
+int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
+  rh::clear_error();
+  if (b == nullptr || (n > 0 && windows == nullptr)) {
+    rh::set_error("rh_poa_upload: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  auto t0 = std::chrono::steady_clock::now();
+  b->arena.clear();
+  b->windows.assign(n, PoaWindow{});
+  b->layers.clear();
+  b->gpu_order.clear();
+  b->shallow.assign(n, 0);
+  b->backbone_off.assign(n, 0);
+  b->backbone_len.assign(n, 0);
+  b->results.assign(n, PoaResult{0, rh::kPoaNotRun, 0, 0, 0});
+  b->stats = rh_poa_stats{};
+
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rh_window& w = windows[i];
+    if (w.backbone == nullptr || w.backbone_quality == nullptr || w.backbone_length == 0 ||
+        (w.n_layers > 0 && w.layers == nullptr)) {
+      rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is malformed");
+      return RH_ERR_ARGUMENT;
+    }
+    total += 2ull * w.backbone_length;
+    for (uint32_t k = 0; k < w.n_layers; ++k) {
+      total += 2ull * w.layers[k].length;
+    }
+  }
+  b->arena.reserve(total);
+
+  uint64_t out_off = 0;
+  std::vector<uint32_t> rank;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rh_window& w = windows[i];
+    PoaWindow& d = b->windows[i];
+    d.backbone_off = b->arena.size();
+    b->backbone_off[i] = d.backbone_off;
+    b->backbone_len[i] = w.backbone_length;
+    b->arena.insert(b->arena.end(), w.backbone, w.backbone + w.backbone_length);
+    d.backbone_qual = b->arena.size();
+    b->arena.insert(b->arena.end(), w.backbone_quality,
+                    w.backbone_quality + w.backbone_length);
+    d.backbone_len = w.backbone_length;
+    d.n_layers = w.n_layers;
+    d.layer_first = static_cast<uint32_t>(b->layers.size());
+    d.offset = static_cast<uint32_t>(0.01 * w.backbone_length);
+    d.flags = (w.tgs ? 1u : 0u) | (b->trim ? 2u : 0u);
+
+    for (uint32_t k = 0; k < w.n_layers; ++k) {
+      const rh_layer& l = w.layers[k];
+      if (l.sequence == nullptr || l.length == 0 || l.begin >= l.end ||
+          l.end > w.backbone_length) {
+        rh::set_error("rh_poa_upload: window " + std::to_string(i) + " layer " +
+                      std::to_string(k) + " is invalid");
+        return RH_ERR_ARGUMENT;
+      }
+    }
+    if (w.n_layers + 1 < 3) {
+      b->shallow[i] = 1;
+      d.n_layers = 0;
+      continue;
+    }
+    // racon's layer order: std::sort of indices 1..n by begin (window.cpp:78-85)
+    rank.resize(w.n_layers + 1);
+    std::iota(rank.begin(), rank.end(), 0u);
+    std::sort(rank.begin() + 1, rank.end(), [&](uint32_t lhs, uint32_t rhs) {
+      return w.layers[lhs - 1].begin < w.layers[rhs - 1].begin;
+    });
+    uint64_t sum_len = 0;
+    uint32_t max_len = 0;
+    for (uint32_t k = 1; k <= w.n_layers; ++k) {
+      const rh_layer& l = w.layers[rank[k] - 1];
+      PoaLayer pl{};
+      pl.seq_off = b->arena.size();
+      b->arena.insert(b->arena.end(), l.sequence, l.sequence + l.length);
+      if (l.quality != nullptr) {
+        pl.qual_off = b->arena.size();
+        b->arena.insert(b->arena.end(), l.quality, l.quality + l.length);
+      } else {
+        pl.qual_off = rh::kNoQuality;
+      }
+      pl.len = l.length;
+      pl.begin = l.begin;
+      pl.end = l.end;
+      b->layers.push_back(pl);
+      sum_len += l.length;
+      max_len = std::max(max_len, l.length);
+    }
+    const uint64_t node_bound = w.backbone_length + sum_len;
+    const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
+    if (node_bound > 0x7fffffffull || edge_bound > 0x7fffffffull) {
+      rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is too large");
+      return RH_ERR_ARGUMENT;
+    }
+    d.node_bound = static_cast<uint32_t>(node_bound);
+    d.edge_bound = static_cast<uint32_t>(edge_bound);
+    d.max_layer_len = max_len;
+    d.out_cap = static_cast<uint32_t>(
+        std::min<uint64_t>(node_bound, 2ull * w.backbone_length + 1024));
+    d.out_off = out_off;
+    out_off += rh::align_up(d.out_cap, 16);
+    b->gpu_order.push_back(i);
+  }
+  b->out_bytes = out_off;
+  // largest estimated cost first (longest-processing-time order)
+  std::stable_sort(b->gpu_order.begin(), b->gpu_order.end(), [&](uint32_t x, uint32_t y) {
+    return window_cost(b->windows[x]) > window_cost(b->windows[y]);
+  });
+  b->stats.pack_ms = rh::ms_since(t0);
+  b->stats.windows = n;
+  b->stats.gpu_windows = b->gpu_order.size();
+  b->stats.input_bytes = b->arena.size();
+
+  auto t1 = std::chrono::steady_clock::now();
+  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(b->arena.size(), 1)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_windows.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaWindow)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_layers.reserve(std::max<size_t>(b->layers.size(), 1) * sizeof(PoaLayer)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaResult)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(b->out_bytes, 1)), RH_ERR_MEMORY);
+  if (!b->arena.empty()) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, b->arena.data(), b->arena.size(),
+                                hipMemcpyHostToDevice, b->stream),
+                 RH_ERR_DEVICE);
+  }
+  if (n > 0) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_windows.ptr, b->windows.data(), n * sizeof(PoaWindow),
+                                hipMemcpyHostToDevice, b->stream),
+                 RH_ERR_DEVICE);
+  }
+  if (!b->layers.empty()) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_layers.ptr, b->layers.data(),
+                                b->layers.size() * sizeof(PoaLayer), hipMemcpyHostToDevice,
+                                b->stream),
+                 RH_ERR_DEVICE);
+  }
+  RH_HIP_CHECK(hipStreamSynchronize(b->stream), RH_ERR_DEVICE);
+  b->stats.h2d_ms = rh::ms_since(t1);
+  b->uploaded = true;
+  b->ran = false;
+  return RH_OK;
+}
+
+int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
+  rh::clear_error();
+  if (b == nullptr || !b->uploaded) {
+    rh::set_error("rh_poa_run: nothing uploaded");
+    return RH_ERR_STATE;
+  }
+  if (mode != RH_POA_BATCHED && mode != RH_POA_PLAIN) {
+    rh::set_error("rh_poa_run: unknown mode");
+    return RH_ERR_ARGUMENT;
+  }
+  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  const uint32_t n = static_cast<uint32_t>(b->windows.size());
+  b->stats.kernel_ms = 0;
+  b->stats.launches = 0;
+  b->stats.retried = 0;
+  if (n == 0) {
+    b->ran = true;
+    return RH_OK;
+  }
+  // fresh results: every kernel window starts as not-run
+  std::vector<PoaResult> init(n, PoaResult{0, rh::kPoaNotRun, 0, 0, 0});
+  RH_HIP_CHECK(hipMemcpyAsync(b->d_results.ptr, init.data(), n * sizeof(PoaResult),
+                              hipMemcpyHostToDevice, b->stream),
+               RH_ERR_DEVICE);
+
+  // default slot: graph arrays for the largest window, DP rows capped
+  PoaSlotCaps caps{};
+  uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
+  for (uint32_t w : b->gpu_order) {
+    max_nodes = std::max(max_nodes, b->windows[w].node_bound);
+    max_edges = std::max(max_edges, b->windows[w].edge_bound);
+    max_cols = std::max(max_cols, b->windows[w].max_layer_len + 1);
+  }
+  caps.nodes = max_nodes;
+  caps.edges = max_edges;
+  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols + 8, 16));
+  caps.rows = std::min<uint32_t>(max_nodes + 1, b->tune_rows ? b->tune_rows : 4096);
+  caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
+
+  int per_cu = 0, cus = 0;
+  RH_HIP_CHECK(rh::poa_occupancy(&per_cu), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
+               RH_ERR_DEVICE);
+  uint32_t resident = static_cast<uint32_t>(std::max(1, per_cu) * std::max(1, cus));
+  if (b->tune_slots != 0) {
+    resident = b->tune_slots;
+  }
+
+  double kernel_ms = 0;
+  int32_t rc = run_pass(b, b->gpu_order, caps, resident, mode, &kernel_ms);
+  if (rc != RH_OK) {
+    return rc;
+  }
+  RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
+                         hipMemcpyDeviceToHost),
+               RH_ERR_DEVICE);
+
+  // windows that outgrew the default slot: rerun with exact bounds
+  std::vector<uint32_t> retry;
+  for (uint32_t w : b->gpu_order) {
+    if (is_retry_status(b->results[w].status)) {
+      retry.push_back(w);
+    }
+  }
+  if (!retry.empty()) {
+    b->stats.retried = retry.size();
+    PoaSlotCaps big = caps;
+    uint64_t extra_out = 0;
+    for (uint32_t w : retry) {
+      big.rows = std::max(big.rows, b->windows[w].node_bound + 1);
+      extra_out += rh::align_up(b->windows[w].node_bound, 16);
+    }
+    // bigger output regions for the retried windows, appended to d_out
+    const uint64_t old_out = b->out_bytes;
+    rh::DeviceBuffer grown;
+    RH_HIP_CHECK(grown.reserve(old_out + extra_out), RH_ERR_MEMORY);
+    RH_HIP_CHECK(hipMemcpy(grown.ptr, b->d_out.ptr, old_out, hipMemcpyDeviceToDevice),
+                 RH_ERR_DEVICE);
+    b->d_out.release();
+    b->d_out = grown;
+    uint64_t off = old_out;
+    for (uint32_t w : retry) {
+      b->windows[w].out_off = off;
+      b->windows[w].out_cap = b->windows[w].node_bound;
+      off += rh::align_up(b->windows[w].node_bound, 16);
+      RH_HIP_CHECK(hipMemcpy(b->d_windows.as<PoaWindow>() + w, &b->windows[w],
+                             sizeof(PoaWindow), hipMemcpyHostToDevice),
+                   RH_ERR_DEVICE);
+    }
+    b->out_bytes = off;
+    rc = run_pass(b, retry, big, resident, RH_POA_BATCHED, &kernel_ms);
+    if (rc != RH_OK) {
+      return rc;
+    }
+    RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
+                           hipMemcpyDeviceToHost),
+                 RH_ERR_DEVICE);
+  }
+  b->stats.kernel_ms = kernel_ms;
+  b->stats.cells = 0;
+  for (uint32_t w : b->gpu_order) {
+    const PoaResult& r = b->results[w];
+    if (r.status != rh::kPoaPolished && r.status != rh::kPoaPolishedUntrimmed) {
+      rh::set_error("rh_poa_run: window " + std::to_string(w) + " failed on the device (status " +
+                    std::to_string(r.status) + ")");
+      return RH_ERR_KERNEL;
+    }
+    b->stats.cells += r.cells;
+  }
+  b->ran = true;
+  return RH_OK;
+}
+
+uint64_t rh_poa_output_bound(const rh_poa_batch* b) {
+  if (b == nullptr) {
+    return 0;
+  }
+  uint64_t total = 0;
+  for (size_t i = 0; i < b->windows.size(); ++i) {
+    total += b->shallow[i] ? b->backbone_len[i] : b->windows[i].out_cap;
+  }
+  return total;
+}
+
+int32_t rh_poa_download(rh_poa_batch* b, char* out, uint64_t capacity, uint64_t* offsets,
+                        uint32_t* lengths, uint8_t* status) {
+  rh::clear_error();
+  if (b == nullptr || !b->ran) {
+    rh::set_error("rh_poa_download: batch has not run");
+    return RH_ERR_STATE;
+  }
+  if (out == nullptr || offsets == nullptr || lengths == nullptr || status == nullptr) {
+    rh::set_error("rh_poa_download: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<uint8_t> dev_out(b->out_bytes);
+  if (b->out_bytes > 0) {
+    RH_HIP_CHECK(hipMemcpy(dev_out.data(), b->d_out.ptr, b->out_bytes, hipMemcpyDeviceToHost),
+                 RH_ERR_DEVICE);
+  }
+  uint64_t pos = 0;
+  for (size_t i = 0; i < b->windows.size(); ++i) {
+    const uint8_t* src;
+    uint32_t len;
+    uint8_t st;
+    if (b->shallow[i]) {
+      src = b->arena.data() + b->backbone_off[i];
+      len = b->backbone_len[i];
+      st = RH_WINDOW_UNPOLISHED;
+    } else {
+      src = dev_out.data() + b->windows[i].out_off;
+      len = b->results[i].length;
+      st = b->results[i].status == rh::kPoaPolishedUntrimmed ? RH_WINDOW_POLISHED_UNTRIMMED
+                                                              : RH_WINDOW_POLISHED;
+    }
+    if (pos + len > capacity) {
+      rh::set_error("rh_poa_download: output capacity too small");
+      return RH_ERR_ARGUMENT;
+    }
+    memcpy(out + pos, src, len);
+    offsets[i] = pos;
+    lengths[i] = len;
+    status[i] = st;
+    pos += len;
+  }
+  b->stats.d2h_ms = rh::ms_since(t0);
+  return RH_OK;
+}
+
+int32_t rh_poa_get_stats(const rh_poa_batch* b, rh_poa_stats* stats) {
+  if (b == nullptr || stats == nullptr) {
+    rh::set_error("rh_poa_get_stats: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  *stats = b->stats;
+  return RH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// The hot-path interface of the polisher.
+//
+// racon runs its hot path item by item on a thread pool: one edlib NW
+// alignment per overlap (Polisher::find_overlap_breaking_points,
+// polisher.cpp:462-484) and one spoa consensus per window
+// (Polisher::polish -> Window::generate_consensus, polisher.cpp:486-504,
+// window.cpp:65-142). A PolishEngine takes the same work as whole batches so
+// that a device engine can keep thousands of items in flight.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace racon {
+
+class Window;
+
+// One pairwise global alignment (edlib's query/target orientation).
+struct AlignTask {
+  const char* query;
+  uint32_t query_length;
+  const char* target;
+  uint32_t target_length;
+};
+
+// Scores and switches shared by every engine (racon CLI defaults:
+// main.cpp:51-66).
+struct EngineConfig {
+  int8_t match = 3;
+  int8_t mismatch = -5;
+  int8_t gap = -4;
+  uint32_t num_threads = 1;
+  uint32_t window_length = 500;
+  // device engine knobs (racon's -c/--cudapoa-batches and
+  // --cudaaligner-batches select the device path in the reference)
+  uint32_t poa_batches = 0;
+  uint32_t aligner_batches = 0;
+  bool banded = false;
+  uint32_t aligner_band_width = 0;
+};
+
+// Per-window outcome of generate_consensus.
+enum : uint8_t {
+  kWindowUnpolished = 0,  // < 3 sequences: consensus is the backbone
+  kWindowPolished = 1,
+  kWindowPolishedChimeric = 2,  // polished, trimming skipped (racon warns)
+};
+
+class PolishEngine {
+ public:
+  virtual ~PolishEngine() = default;
+
+  virtual std::string name() const = 0;
+
+  // Aligns every task (edlib NW, path) and calls done(i, ops, n_ops) once per
+  // task with the edit operations in alignment order. done() may be called
+  // from several threads, but never twice for the same i.
+  virtual void align_overlaps(
+      const std::vector<AlignTask>& tasks,
+      const std::function<void(uint64_t, const uint8_t*, uint64_t)>& done) = 0;
+
+  // Generates the consensus of every window (set through
+  // Window::set_consensus) and fills status[i] with kWindow* codes.
+  virtual void generate_consensus(std::vector<std::shared_ptr<Window>>& windows,
+                                  bool trim, std::vector<uint8_t>& status) = 0;
+};
+
+// Defined once per executable: the product links the HIP engine, the oracle
+// build links the CPU restatement.
+std::unique_ptr<PolishEngine> createEngine(const EngineConfig& config);
+
+}  // namespace racon

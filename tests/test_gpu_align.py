@@ -1,0 +1,62 @@
+"""Overlap alignment on the GPU vs the CPU oracle (edlib NW path)."""
+import sys
+
+import numpy as np
+import pytest
+
+from racon_amd import _lib
+from synth import make_pair, mutate, random_seq
+
+sys.path.insert(0, _lib.PKG_DIR + "/../oracle")
+import oracle_py  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def check(pairs):
+    b = _lib.AlignBatch()
+    b.upload(pairs)
+    b.run()
+    ops, dist = b.download()
+    b.close()
+    for i, (q, t) in enumerate(pairs):
+        d, oops, st = oracle_py.edlib_align(q, t)
+        print(f"pair {i}: |q|={len(q)} |t|={len(t)} gpu d={dist[i]} oracle d={d}", flush=True)
+        assert st == 0
+        assert dist[i] == d, f"pair {i} distance"
+        assert ops[i] == oops, f"pair {i} path differs"
+
+
+def test_tiny_pairs():
+    rng = np.random.default_rng(1)
+    pairs = [(b"A", b"A"), (b"A", b"C"), (b"AC", b"A"), (b"ACGT", b"TGCA"), (b"", b"ACG"),
+             (b"ACG", b"")]
+    for n in (1, 3, 10, 63, 64, 65, 127, 200):
+        pairs.append(make_pair(rng, n))
+    check(pairs)
+
+
+def test_leaf_sized_pairs():
+    rng = np.random.default_rng(2)
+    check([make_pair(rng, int(n)) for n in rng.integers(300, 2000, size=12)])
+
+
+def test_hirschberg_pairs():
+    rng = np.random.default_rng(3)
+    check([make_pair(rng, int(n)) for n in (3000, 5000, 8000)])
+
+
+def test_multi_strip_query():
+    # queries longer than 64 blocks (4096 rows) use several strips
+    rng = np.random.default_rng(4)
+    t = random_seq(rng, 9000)
+    q = mutate(rng, t, 0.12)
+    check([(q, t), (t[:4200], q[:4100])])
+
+
+def test_unequal_and_other_bytes():
+    rng = np.random.default_rng(5)
+    t = random_seq(rng, 1500)
+    q = mutate(rng, t, 0.15)
+    q2 = q[:700] + b"NNNN" + q[700:]
+    check([(q[:900], t), (q2, t), (q2.replace(b"A", b"R", 3), t)])

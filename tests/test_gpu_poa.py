@@ -1,0 +1,78 @@
+"""Window consensus on the GPU vs the CPU oracle, byte for byte."""
+import sys
+
+import pytest
+
+from racon_amd import _lib
+from synth import make_windows, oracle_args
+
+sys.path.insert(0, _lib.PKG_DIR + "/../oracle")
+import oracle_py  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SCORES = [(3, -5, -4), (5, -4, -8), (1, -1, -1)]
+
+
+def run_gpu(windows, scores, trim=True, mode=_lib.RH_POA_BATCHED):
+    b = _lib.PoaBatch(*scores, trim=trim)
+    b.upload(windows)
+    b.run(mode)
+    out, status = b.download()
+    st = b.stats()
+    b.close()
+    return out, status, st
+
+
+def check(windows, scores, trim=True, mode=_lib.RH_POA_BATCHED):
+    out, status, st = run_gpu(windows, scores, trim, mode)
+    cells = 0
+    for i, w in enumerate(windows):
+        bb, bq, layers, tgs = oracle_args(w)
+        cons, ost, ocells, _ = oracle_py.window_consensus(bb, bq, layers, tgs, trim, *scores)
+        print(f"window {i}: gpu {len(out[i])} bp status {status[i]} / oracle {len(cons)} bp "
+              f"status {ost}", flush=True)
+        assert status[i] == ost, f"window {i} status"
+        assert out[i] == cons, f"window {i} consensus differs"
+        cells += ocells
+    assert st.cells == cells
+    return st
+
+
+@pytest.mark.parametrize("scores", SCORES)
+def test_small_windows(scores):
+    wins = make_windows(11, 6, length=120, depth=8)
+    check(wins, scores)
+
+
+def test_quality_mix_and_shallow():
+    wins = make_windows(12, 4, length=300, depth=12, quality=False)
+    wins += make_windows(13, 4, length=300, depth=12, backbone_quality=True)
+    wins += make_windows(14, 2, length=200, depth=1)  # < 3 sequences
+    wins += make_windows(15, 2, length=200, depth=0)
+    check(wins, (3, -5, -4))
+
+
+def test_no_trim_and_ngs():
+    wins = make_windows(16, 4, length=250, depth=10, tgs=False)
+    check(wins, (3, -5, -4))
+    wins = make_windows(17, 4, length=250, depth=10)
+    check(wins, (3, -5, -4), trim=False)
+
+
+def test_racon_sized_windows():
+    wins = make_windows(18, 24, length=500, depth=30)
+    check(wins, (3, -5, -4))
+
+
+def test_plain_matches_batched():
+    wins = make_windows(19, 5, length=400, depth=15)
+    out_b, st_b, _ = run_gpu(wins, (3, -5, -4), mode=_lib.RH_POA_BATCHED)
+    out_p, st_p, _ = run_gpu(wins, (3, -5, -4), mode=_lib.RH_POA_PLAIN)
+    assert out_b == out_p and st_b == st_p
+
+
+def test_long_layers_two_tiles():
+    # layers longer than one 512-column tile
+    wins = make_windows(20, 3, length=1000, depth=12)
+    check(wins, (5, -4, -8))
