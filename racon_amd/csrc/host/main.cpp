@@ -3,8 +3,10 @@
 // -b/--cuda-banded-alignment, --cudaaligner-batches and
 // --cudaaligner-band-width are always compiled in.
 //
-// One extra long option, --report <path>, writes the phase timings of the
-// run as JSON (used by the benchmark driver; it does not change the output).
+// Extra long options for the benchmark driver (they do not change the
+// output): --report <path> writes the phase timings as JSON;
+// --dump-overlaps <path> writes the overlap alignment inputs;
+// --dump-windows <path> writes the windows after initialization and exits.
 #include <getopt.h>
 
 #include <cstdio>
@@ -23,6 +25,8 @@ static const char* version = RACON_VERSION;
 static const int32_t CUDAALIGNER_INPUT_CODE = 10000;
 static const int32_t CUDAALIGNER_BAND_WIDTH_INPUT_CODE = 10001;
 static const int32_t REPORT_INPUT_CODE = 10002;
+static const int32_t DUMP_WINDOWS_CODE = 10003;
+static const int32_t DUMP_OVERLAPS_CODE = 10004;
 
 static struct option options[] = {
     {"include-unpolished", no_argument, 0, 'u'},
@@ -42,6 +46,8 @@ static struct option options[] = {
     {"cudaaligner-batches", required_argument, 0, CUDAALIGNER_INPUT_CODE},
     {"cudaaligner-band-width", required_argument, 0, CUDAALIGNER_BAND_WIDTH_INPUT_CODE},
     {"report", required_argument, 0, REPORT_INPUT_CODE},
+    {"dump-windows", required_argument, 0, DUMP_WINDOWS_CODE},
+    {"dump-overlaps", required_argument, 0, DUMP_OVERLAPS_CODE},
     {0, 0, 0, 0}};
 
 static void help();
@@ -67,6 +73,7 @@ int main(int argc, char** argv) {
   uint32_t cudaaligner_band_width = 0;
   bool cuda_banded_alignment = false;
   std::string report_path;
+  std::string dump_windows_path, dump_overlaps_path;
 
   std::string optstring = "ufw:q:e:m:x:g:t:hbc::";
 
@@ -99,6 +106,8 @@ int main(int argc, char** argv) {
       case CUDAALIGNER_INPUT_CODE: cudaaligner_batches = atoi(optarg); break;
       case CUDAALIGNER_BAND_WIDTH_INPUT_CODE: cudaaligner_band_width = atoi(optarg); break;
       case REPORT_INPUT_CODE: report_path = optarg; break;
+      case DUMP_WINDOWS_CODE: dump_windows_path = optarg; break;
+      case DUMP_OVERLAPS_CODE: dump_overlaps_path = optarg; break;
       default: exit(1);
     }
   }
@@ -123,7 +132,13 @@ int main(int argc, char** argv) {
         gap, num_threads, cudapoa_batches, cuda_banded_alignment,
         cudaaligner_batches, cudaaligner_band_width);
 
+    polisher->set_dump_paths(dump_windows_path, dump_overlaps_path);
     polisher->initialize();
+    if (!dump_windows_path.empty()) {
+      // benchmark input only: windows written, no polishing
+      polisher->dump_windows();
+      return 0;
+    }
     polisher->polish(polished_sequences, drop_unpolished_sequences);
     timings = polisher->timings();
   }

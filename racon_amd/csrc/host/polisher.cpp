@@ -374,6 +374,47 @@ void Polisher::initialize() {
   logger_->log("[racon::Polisher::initialize] transformed data into windows");
 }
 
+void Polisher::dump_windows() const {
+  // "RHW1", u64 n, then per window: u32 length, u8 tgs, 3 pad bytes,
+  // backbone, backbone quality, u32 layers, and per layer (insertion order):
+  // u32 length, u32 begin, u32 end, u8 has_quality, bytes, [quality]
+  FILE* f = fopen(dump_windows_path_.c_str(), "wb");
+  if (f == nullptr) {
+    fprintf(stderr, "[racon::Polisher] error: unable to write %s\n",
+            dump_windows_path_.c_str());
+    exit(1);
+  }
+  const uint64_t n = windows_.size();
+  fwrite("RHW1", 1, 4, f);
+  fwrite(&n, 8, 1, f);
+  const uint8_t pad[3] = {0, 0, 0};
+  for (const auto& w : windows_) {
+    const uint32_t len = w->sequence(0).second;
+    const uint8_t tgs = w->type() == WindowType::kTGS ? 1 : 0;
+    fwrite(&len, 4, 1, f);
+    fwrite(&tgs, 1, 1, f);
+    fwrite(pad, 1, 3, f);
+    fwrite(w->sequence(0).first, 1, len, f);
+    fwrite(w->quality(0).first, 1, len, f);
+    const uint32_t layers = w->size() - 1;
+    fwrite(&layers, 4, 1, f);
+    for (uint32_t k = 1; k < w->size(); ++k) {
+      const uint32_t l = w->sequence(k).second;
+      const uint32_t b = w->position(k).first, e = w->position(k).second;
+      const uint8_t hq = w->quality(k).first != nullptr ? 1 : 0;
+      fwrite(&l, 4, 1, f);
+      fwrite(&b, 4, 1, f);
+      fwrite(&e, 4, 1, f);
+      fwrite(&hq, 1, 1, f);
+      fwrite(w->sequence(k).first, 1, l, f);
+      if (hq) {
+        fwrite(w->quality(k).first, 1, l, f);
+      }
+    }
+  }
+  fclose(f);
+}
+
 void Polisher::find_overlap_breaking_points(
     std::vector<std::unique_ptr<Overlap>>& overlaps) {
   // polisher.cpp:462-484 with the per-overlap edlib calls batched through
@@ -392,6 +433,25 @@ void Polisher::find_overlap_breaking_points(
     }
   }
   timings_.num_overlaps_aligned = tasks.size();
+  if (!dump_overlaps_path_.empty()) {
+    // "RHA1", u64 n, then per pair: u32 |q|, u32 |t|, q bytes, t bytes
+    FILE* f = fopen(dump_overlaps_path_.c_str(), "wb");
+    if (f == nullptr) {
+      fprintf(stderr, "[racon::Polisher] error: unable to write %s\n",
+              dump_overlaps_path_.c_str());
+      exit(1);
+    }
+    const uint64_t n = tasks.size();
+    fwrite("RHA1", 1, 4, f);
+    fwrite(&n, 8, 1, f);
+    for (const auto& t : tasks) {
+      fwrite(&t.query_length, 4, 1, f);
+      fwrite(&t.target_length, 4, 1, f);
+      fwrite(t.query, 1, t.query_length, f);
+      fwrite(t.target, 1, t.target_length, f);
+    }
+    fclose(f);
+  }
 
   {
     std::vector<std::future<void>> futures;

@@ -70,6 +70,14 @@ class Polisher {
                       bool drop_unpolished_sequences);
 
   const PolisherTimings& timings() const { return timings_; }
+
+  // Writes the windows (after initialize) or the overlap alignment inputs
+  // (during initialize) to binary files for the benchmark driver.
+  void set_dump_paths(const std::string& windows_path, const std::string& overlaps_path) {
+    dump_windows_path_ = windows_path;
+    dump_overlaps_path_ = overlaps_path;
+  }
+  void dump_windows() const;
   PolishEngine& engine() { return *engine_; }
 
  protected:
@@ -99,6 +107,8 @@ class Polisher {
   std::unique_ptr<Logger> logger_;
   std::unique_ptr<PolishEngine> engine_;
   PolisherTimings timings_;
+  std::string dump_windows_path_;
+  std::string dump_overlaps_path_;
 };
 
 }  // namespace racon

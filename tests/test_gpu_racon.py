@@ -1,0 +1,55 @@
+"""The racon CLI with the GPU hot path vs the CPU oracle CLI, byte for byte.
+
+Inputs are racon's own test fixtures (test/data of the reference, copied to
+tests/golden/racon_data) with the options of test/racon_test.cpp:88-290.
+"""
+import os
+import subprocess
+
+import pytest
+
+from racon_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_CLI = os.path.join(ROOT, "oracle", "_build", "racon_cpu")
+
+CASES = [
+    ("with_qualities", "sample_reads.fastq.gz", "sample_overlaps.paf.gz",
+     "sample_layout.fasta.gz", ["-m", "5", "-x", "-4", "-g", "-8", "-w", "500"]),
+    ("without_qualities", "sample_reads.fasta.gz", "sample_overlaps.paf.gz",
+     "sample_layout.fasta.gz", ["-m", "5", "-x", "-4", "-g", "-8", "-w", "500"]),
+    ("sam_with_qualities", "sample_reads.fastq.gz", "sample_overlaps.sam.gz",
+     "sample_layout.fasta.gz", ["-m", "5", "-x", "-4", "-g", "-8", "-w", "500"]),
+    ("sam_without_qualities", "sample_reads.fasta.gz", "sample_overlaps.sam.gz",
+     "sample_layout.fasta.gz", ["-m", "5", "-x", "-4", "-g", "-8", "-w", "500"]),
+    ("larger_window", "sample_reads.fastq.gz", "sample_overlaps.paf.gz",
+     "sample_layout.fasta.gz", ["-m", "5", "-x", "-4", "-g", "-8", "-w", "1000"]),
+    ("edit_distance_scores", "sample_reads.fastq.gz", "sample_overlaps.paf.gz",
+     "sample_layout.fasta.gz", ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500"]),
+    ("fragment_kc", "sample_reads.fastq.gz", "sample_ava_overlaps.paf.gz",
+     "sample_reads.fastq.gz", ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500"]),
+    ("fragment_kf", "sample_reads.fastq.gz", "sample_ava_overlaps.paf.gz",
+     "sample_reads.fastq.gz", ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500", "-f", "-u"]),
+    ("fragment_kf_fasta", "sample_reads.fasta.gz", "sample_ava_overlaps.paf.gz",
+     "sample_reads.fasta.gz", ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500", "-f", "-u"]),
+    ("fragment_kf_mhap", "sample_reads.fastq.gz", "sample_ava_overlaps.mhap.gz",
+     "sample_reads.fastq.gz", ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500", "-f", "-u"]),
+]
+
+
+def run(binary, args, data_dir, files):
+    cmd = [binary, "-t", "8"] + args + [os.path.join(data_dir, f) for f in files]
+    p = subprocess.run(cmd, capture_output=True, timeout=600)
+    assert p.returncode == 0, p.stderr.decode()[-2000:]
+    return p.stdout
+
+
+@pytest.mark.parametrize("name,reads,overlaps,targets,args", CASES, ids=[c[0] for c in CASES])
+def test_cli_matches_oracle(name, reads, overlaps, targets, args, data_dir):
+    files = (reads, overlaps, targets)
+    gpu = run(_lib.RACON_BIN, args, data_dir, files)
+    cpu = run(ORACLE_CLI, args, data_dir, files)
+    print(f"{name}: gpu {len(gpu)} bytes, oracle {len(cpu)} bytes", flush=True)
+    assert gpu == cpu
