@@ -123,6 +123,10 @@ def main() -> int:
         log(f"step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms (kernel {st.kernel_ms:.1f} ms, "
             f"slots {st.slots}, retried {st.retried})")
     st = batch.stats()
+    phases = batch.phase_seconds()
+    total_phase = sum(phases.values()) or 1.0
+    log("phase share of wave time: " + ", ".join(
+        f"{k} {100 * v / total_phase:.1f}%" for k, v in phases.items()))
     gpu_out, gpu_status = batch.download()
     n_windows = len(windows)
     cells = st.cells
@@ -216,6 +220,7 @@ def main() -> int:
                           "frac": round(cells * POA_OPS_PER_CELL / (kernel_avg / 1e3) / 1e12
                                         / VALU_PEAK_TOPS, 5)},
         "cpu_baseline": cpu,
+        "poa_phase_share": {k: round(v / total_phase, 4) for k, v in phases.items()},
     }
     print(json.dumps(result), flush=True)
     return 0

@@ -130,6 +130,14 @@ int32_t rh_poa_get_stats(const rh_poa_batch* batch, rh_poa_stats* stats);
  * and the DP row capacity of a default slot. */
 int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows);
 
+/* Profiling: device clock ticks of the last run summed over windows, per
+ * kernel phase (subgraph set, subgraph order, DP row map, DP column 0, DP
+ * rows, traceback, insertion, graph order, consensus, setup); n <= 10.
+ * ticks_per_second receives the device constant-clock rate. No reference
+ * counterpart (racon's Logger times whole phases only). */
+int32_t rh_poa_phase_ticks(const rh_poa_batch* batch, uint64_t* ticks, uint32_t n,
+                           uint64_t* ticks_per_second);
+
 /* ---- (a) overlap alignment ---- */
 
 /* One pair in edlib's orientation: query = read span (reverse-complemented

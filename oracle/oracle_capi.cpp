@@ -66,6 +66,63 @@ int32_t oracle_window_consensus(const char* backbone, const char* backbone_quali
   return status;
 }
 
+// Aligns `sequence` to a graph built from the given chains (each added with
+// an empty alignment, i.e. as a new path) plus, when `layer` is given, that
+// layer aligned to the chain and inserted: used by tests to check the DP on
+// small graphs. Writes (node, position) pairs, returns their count.
+int32_t oracle_poa_align(const char* const* chains, const uint32_t* chain_lengths,
+                         uint32_t n_chains, const char* layer, uint32_t layer_length,
+                         const char* sequence, uint32_t length, int32_t match,
+                         int32_t mismatch, int32_t gap, int32_t* out_nodes,
+                         int32_t* out_positions, uint32_t capacity, char* node_bases,
+                         uint32_t* n_nodes) {
+  oracle::PoaGraph g;
+  for (uint32_t c = 0; c < n_chains; ++c) {
+    std::vector<uint32_t> w(chain_lengths[c], 1);
+    g.add_alignment(oracle::Alignment(), chains[c], chain_lengths[c], w);
+  }
+  if (layer != nullptr && layer_length > 0) {
+    auto a = oracle::poa_align_nw(layer, layer_length, g, match, mismatch, gap, nullptr);
+    std::vector<uint32_t> w(layer_length, 1);
+    g.add_alignment(a, layer, layer_length, w);
+  }
+  auto a = oracle::poa_align_nw(sequence, length, g, match, mismatch, gap, nullptr);
+  *n_nodes = static_cast<uint32_t>(g.nodes().size());
+  if (node_bases != nullptr) {
+    for (uint32_t i = 0; i < g.nodes().size() && i < capacity; ++i) node_bases[i] = g.nodes()[i].base;
+  }
+  if (a.size() > capacity) return -1;
+  for (size_t i = 0; i < a.size(); ++i) {
+    out_nodes[i] = a[i].first;
+    out_positions[i] = a[i].second;
+  }
+  return static_cast<int32_t>(a.size());
+}
+
+// Edges of the graph used by oracle_poa_align (same construction), for the
+// brute-force path enumeration of the tests: writes tail/head pairs.
+int32_t oracle_poa_graph_edges(const char* const* chains, const uint32_t* chain_lengths,
+                               uint32_t n_chains, const char* layer, uint32_t layer_length,
+                               int32_t match, int32_t mismatch, int32_t gap, int32_t* tails,
+                               int32_t* heads, uint32_t capacity) {
+  oracle::PoaGraph g;
+  for (uint32_t c = 0; c < n_chains; ++c) {
+    std::vector<uint32_t> w(chain_lengths[c], 1);
+    g.add_alignment(oracle::Alignment(), chains[c], chain_lengths[c], w);
+  }
+  if (layer != nullptr && layer_length > 0) {
+    auto a = oracle::poa_align_nw(layer, layer_length, g, match, mismatch, gap, nullptr);
+    std::vector<uint32_t> w(layer_length, 1);
+    g.add_alignment(a, layer, layer_length, w);
+  }
+  if (g.edges().size() > capacity) return -1;
+  for (size_t i = 0; i < g.edges().size(); ++i) {
+    tails[i] = static_cast<int32_t>(g.edges()[i].tail);
+    heads[i] = static_cast<int32_t>(g.edges()[i].head);
+  }
+  return static_cast<int32_t>(g.edges().size());
+}
+
 // racon's layer order (std::sort of indices 1..n by begin, window.cpp:78-85),
 // returned as 0-based layer indices.
 void oracle_sorted_layer_order(uint32_t n, const uint32_t* begins, uint32_t* out) {

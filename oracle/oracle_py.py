@@ -40,6 +40,18 @@ def load() -> ctypes.CDLL:
         ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, ctypes.c_uint64,
         ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
         ctypes.POINTER(ctypes.c_uint64)]
+    lib.oracle_poa_align.restype = ctypes.c_int32
+    lib.oracle_poa_align.argtypes = [
+        ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+        ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+        ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32, ctypes.c_char_p,
+        ctypes.POINTER(ctypes.c_uint32)]
+    lib.oracle_poa_graph_edges.restype = ctypes.c_int32
+    lib.oracle_poa_graph_edges.argtypes = [
+        ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+        ctypes.c_char_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32]
     lib.oracle_sorted_layer_order.restype = None
     lib.oracle_sorted_layer_order.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                               ctypes.POINTER(ctypes.c_uint32)]
@@ -103,3 +115,33 @@ def window_consensus(backbone: bytes, backbone_quality: bytes, layers, tgs: bool
     if st < 0:
         raise RuntimeError("oracle window output overflow")
     return out.raw[:out_len.value], st, cells.value, nodes.value
+
+
+def poa_align(chains, sequence: bytes, match: int, mismatch: int, gap: int,
+              layer: bytes = None):
+    """Aligns sequence to a graph of parallel chains (+ an inserted layer).
+
+    Returns (pairs, node_bases, edges): the alignment as (node, position)
+    pairs, the base of every node and the (tail, head) edge list.
+    """
+    lib = load()
+    n = len(chains)
+    arr = (ctypes.c_char_p * max(1, n))(*chains)
+    lens = (ctypes.c_uint32 * max(1, n))(*[len(c) for c in chains])
+    cap = sum(len(c) for c in chains) + len(sequence) + (len(layer) if layer else 0) + 8
+    nodes = (ctypes.c_int32 * cap)()
+    pos = (ctypes.c_int32 * cap)()
+    bases = ctypes.create_string_buffer(cap)
+    nn = ctypes.c_uint32()
+    k = lib.oracle_poa_align(arr, lens, n, layer, len(layer) if layer else 0, sequence,
+                             len(sequence), match, mismatch, gap, nodes, pos, cap, bases,
+                             ctypes.byref(nn))
+    if k < 0:
+        raise RuntimeError("oracle_poa_align capacity")
+    ecap = 4 * cap
+    tails = (ctypes.c_int32 * ecap)()
+    heads = (ctypes.c_int32 * ecap)()
+    ne = lib.oracle_poa_graph_edges(arr, lens, n, layer, len(layer) if layer else 0, match,
+                                    mismatch, gap, tails, heads, ecap)
+    return ([(nodes[i], pos[i]) for i in range(k)], bases.raw[:nn.value],
+            [(tails[i], heads[i]) for i in range(ne)])

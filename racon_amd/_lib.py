@@ -102,6 +102,8 @@ _SIGNATURES = {
     "rh_poa_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaStats)]),
     "rh_poa_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
                                            ctypes.c_uint32]),
+    "rh_poa_phase_ticks": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "rh_align_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "rh_align_destroy": (None, [ctypes.c_void_p]),
     "rh_align_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
@@ -220,6 +222,17 @@ class PoaBatch:
         raw = out.raw
         return ([raw[offsets[i]:offsets[i] + lengths[i]] for i in range(self.n)],
                 [status[i] for i in range(self.n)])
+
+    PHASES = ("include", "subsort", "rows", "column0", "dp", "traceback", "insert", "sort",
+              "consensus", "setup")
+
+    def phase_seconds(self) -> dict:
+        """Device time per kernel phase of the last run, summed over windows."""
+        ticks = (ctypes.c_uint64 * len(self.PHASES))()
+        rate = ctypes.c_uint64()
+        _check(self.lib.rh_poa_phase_ticks(self.handle, ticks, len(self.PHASES),
+                                           ctypes.byref(rate)), "rh_poa_phase_ticks")
+        return {k: ticks[i] / max(1, rate.value) for i, k in enumerate(self.PHASES)}
 
     def stats(self) -> RhPoaStats:
         st = RhPoaStats()

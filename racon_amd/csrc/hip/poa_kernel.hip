@@ -48,6 +48,17 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
+// Phase clock: accumulates device constant-clock ticks into R.phase[k].
+struct PhaseClock {
+  uint64_t t;
+  __device__ PhaseClock() : t(wall_clock64()) {}
+  __device__ void lap(PoaResult& R, int k) {
+    const uint64_t now = wall_clock64();
+    R.phase[k] += now - t;
+    t = now;
+  }
+};
+
 // Views of one slot's arrays.
 struct Slot {
   uint8_t* base;
@@ -74,7 +85,7 @@ struct Slot {
   int32_t* sub_order;
   int32_t* pred_off;
   int32_t* pred_rows;
-  uint8_t* sink;
+  int32_t* row_meta;
   int32_t* H;
   uint16_t* code;
   int32_t* aln_node;
@@ -117,7 +128,7 @@ __device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
   s.sub_order = reinterpret_cast<int32_t*>(p + l.sub_order);
   s.pred_off = reinterpret_cast<int32_t*>(p + l.pred_off);
   s.pred_rows = reinterpret_cast<int32_t*>(p + l.pred_rows);
-  s.sink = p + l.sink;
+  s.row_meta = reinterpret_cast<int32_t*>(p + l.row_meta);
   s.H = reinterpret_cast<int32_t*>(p + l.H);
   s.code = reinterpret_cast<uint16_t*>(p + l.code);
   s.aln_node = reinterpret_cast<int32_t*>(p + l.aln_node);
@@ -305,130 +316,199 @@ __device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
   return x;
 }
 
-// One DP row: row r (1-based) of node base b with predecessor rows
-// preds[0..np). H rows have pitch `pitch`; column j of a row is at index j.
-__device__ void dp_row(const Slot& s, int r, uint8_t b, const int32_t* preds, int np,
-                       int L, int pitch, int32_t match, int32_t mismatch, int32_t gap) {
+// Wavefront-scope ordering of LDS/global traffic between lanes of this wave
+// (a single-wave workgroup needs no s_barrier; the fence keeps the compiler
+// from moving memory operations across it).
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Waits for this wave's outstanding global stores (vmcnt(0)).
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+constexpr int kRowMetaSink = 0x100;
+constexpr int kRowMetaFar = 0x200;
+
+// Value of predecessor row p at column j while row r is computed: row 0 is
+// the virtual start (j * gap); rows less than R back live in the LDS ring;
+// older rows were also written to the slot's global H matrix.
+template <typename T>
+__device__ __forceinline__ const T* ring_row(const T* ring, int pitch, int R, int p) {
+  return ring + (p % R) * pitch;
+}
+
+// All DP rows of one layer. Lanes own 8-column strips of 512-column tiles;
+// predecessor rows come from the LDS ring (global H for rows >= R back);
+// the horizontal gap chain is a wave prefix-max; every cell stores the
+// first move that reproduces its score (diagonal, then vertical, each over
+// predecessors in in-edge order, then horizontal). Returns the DP row of the
+// end cell: the first sink row with the highest score in column L.
+template <typename T>
+__device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, int L,
+                        int R, uint64_t* ticks) {
+  (void)ticks;
   const int lane = lane_id();
-  int32_t* Hr = s.H + static_cast<int64_t>(r) * pitch;
-  uint16_t* Cr = s.code + static_cast<int64_t>(r) * pitch;
-  int32_t carry = Hr[0];  // column 0, set by the column pass (u_0 = H[r][0])
-  for (int t0 = 1; t0 <= L; t0 += kTile) {
-    const int j0 = t0 + lane * kStrip;
-    int32_t sc[kStrip];
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      const int j = j0 + u;
-      sc[u] = (j <= L && s.seq[j - 1] == b) ? match : mismatch;
-    }
-    int32_t t[kStrip];
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      t[u] = kNegInf;
-    }
+  const int pitch = static_cast<int>(P.caps.cols);
+  const int32_t gap = P.gap;
+  T* ring = reinterpret_cast<T*>(lds);
+  const uint8_t* seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
+  int32_t best = kNegInf;
+  int best_row = 0;
+  for (int r = 1; r <= V; ++r) {
+    const int off = uniform(s.pred_off[r - 1]);
+    const int np = uniform(s.pred_off[r]) - off;
+    const int meta = uniform(s.row_meta[r - 1]);
+    const uint8_t b = static_cast<uint8_t>(meta & 0xff);
+    T* Hr = ring + (r % R) * pitch;
+    int32_t* Hg = s.H + static_cast<int64_t>(r) * pitch;
+    uint16_t* Cr = s.code + static_cast<int64_t>(r) * pitch;
+    const bool far = (meta & kRowMetaFar) != 0;
+
+    // column 0: max over predecessors of their column 0, plus gap
+    int32_t c0 = kNegInf;
+    int k0 = 0;
     for (int k = 0; k < np; ++k) {
-      const int32_t* Hp = s.H + static_cast<int64_t>(preds[k]) * pitch;
-      int32_t left = j0 - 1 <= L ? Hp[j0 - 1] : kNegInf;
-#pragma unroll
-      for (int u = 0; u < kStrip; ++u) {
-        const int j = j0 + u;
-        const int32_t up = j <= L ? Hp[j] : kNegInf;
-        t[u] = max(t[u], max(left + sc[u], up + gap));
-        left = up;
+      const int p = uniform(s.pred_rows[off + k]);
+      int32_t h0;
+      if (p == 0) {
+        h0 = 0;
+      } else if (r - p < R) {
+        h0 = ring_row(ring, pitch, R, p)[0];
+      } else {
+        h0 = s.H[static_cast<int64_t>(p) * pitch];
+      }
+      if (h0 + gap > c0) {
+        c0 = h0 + gap;
+        k0 = k;
       }
     }
-    // horizontal chain: H[j] = j*g + max_{k<=j} (t_k - k*g)
-    int32_t pm = kNegInf;
-    int32_t v[kStrip];
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      const int j = j0 + u;
-      v[u] = j <= L ? t[u] - j * gap : kNegInf;
-      pm = max(pm, v[u]);
-      v[u] = pm;
-    }
-    int32_t incl = wave_prefix_max(pm);
-    int32_t excl = __shfl_up(incl, 1, kWave);
     if (lane == 0) {
-      excl = kNegInf;
+      Hr[0] = static_cast<T>(c0);
+      Cr[0] = kCodeVert | static_cast<uint16_t>(k0);
+      if (far) Hg[0] = c0;
     }
-    const int32_t in_carry = max(excl, carry - (t0 - 1) * gap);
-    int32_t h[kStrip];
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      const int j = j0 + u;
-      h[u] = max(v[u], in_carry) + j * gap;
-    }
-    // store H
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      if (j0 + u <= L) {
-        Hr[j0 + u] = h[u];
-      }
-    }
-    // left neighbour of the strip's first column
-    int32_t hleft = __shfl_up(h[kStrip - 1], 1, kWave);
-    if (lane == 0) {
-      hleft = carry;
-    }
-    // move codes: first diagonal predecessor, else first vertical, else left
-    uint16_t cd[kStrip];
-#pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      cd[u] = 0xffff;
-    }
-    for (int k = 0; k < np; ++k) {
-      const int32_t* Hp = s.H + static_cast<int64_t>(preds[k]) * pitch;
-      int32_t left = j0 - 1 <= L ? Hp[j0 - 1] : kNegInf;
+    int32_t carry = c0;
+    int32_t h_last = kNegInf;
+
+    for (int t0 = 1; t0 <= L; t0 += kTile) {
+      const int j0 = t0 + lane * kStrip;
+      int32_t sc[kStrip];
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
         const int j = j0 + u;
-        const int32_t up = j <= L ? Hp[j] : kNegInf;
-        if (cd[u] == 0xffff && h[u] == left + sc[u]) {
-          cd[u] = kCodeDiag | static_cast<uint16_t>(k);
-        }
-        left = up;
+        sc[u] = (j <= L && seq[j - 1] == b) ? P.match : P.mismatch;
       }
-    }
-    for (int k = 0; k < np; ++k) {
-      const int32_t* Hp = s.H + static_cast<int64_t>(preds[k]) * pitch;
+      int32_t bd[kStrip], bv[kStrip];
+      uint32_t kk[kStrip];  // diagonal k (low 16 bits), vertical k (high 16 bits)
+#pragma unroll
+      for (int u = 0; u < kStrip; ++u) {
+        bd[u] = kNegInf;
+        bv[u] = kNegInf;
+        kk[u] = 0;
+      }
+      for (int k = 0; k < np; ++k) {
+        const int p = uniform(s.pred_rows[off + k]);
+        int32_t hp[kStrip + 1];
+        if (p == 0) {
+#pragma unroll
+          for (int u = 0; u <= kStrip; ++u) hp[u] = (j0 - 1 + u) * gap;
+        } else if (r - p < R) {
+          const T* row = ring_row(ring, pitch, R, p);
+#pragma unroll
+          for (int u = 0; u <= kStrip; ++u) {
+            const int j = j0 - 1 + u;
+            hp[u] = j <= L ? static_cast<int32_t>(row[j]) : kNegInf;
+          }
+        } else {
+          const int32_t* row = s.H + static_cast<int64_t>(p) * pitch;
+#pragma unroll
+          for (int u = 0; u <= kStrip; ++u) {
+            const int j = j0 - 1 + u;
+            hp[u] = j <= L ? row[j] : kNegInf;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kStrip; ++u) {
+          const int32_t d = hp[u] + sc[u];
+          const int32_t v = hp[u + 1] + gap;
+          if (d > bd[u]) {
+            bd[u] = d;
+            kk[u] = (kk[u] & 0xffff0000u) | static_cast<uint32_t>(k);
+          }
+          if (v > bv[u]) {
+            bv[u] = v;
+            kk[u] = (kk[u] & 0xffffu) | (static_cast<uint32_t>(k) << 16);
+          }
+        }
+      }
+      // horizontal chain: H[j] = j*g + max_{k<=j} (t_k - k*g)
+      int32_t pm = kNegInf;
+      int32_t pre[kStrip];
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
         const int j = j0 + u;
-        if (j <= L && cd[u] == 0xffff && h[u] == Hp[j] + gap) {
-          cd[u] = kCodeVert | static_cast<uint16_t>(k);
+        const int32_t t = max(bd[u], bv[u]);
+        pre[u] = j <= L ? t - j * gap : kNegInf;
+        pm = max(pm, pre[u]);
+        pre[u] = pm;
+      }
+      const int32_t incl = wave_prefix_max(pm);
+      int32_t excl = __shfl_up(incl, 1, kWave);
+      if (lane == 0) {
+        excl = kNegInf;
+      }
+      const int32_t in_carry = max(excl, carry - (t0 - 1) * gap);
+      int32_t h[kStrip];
+#pragma unroll
+      for (int u = 0; u < kStrip; ++u) {
+        h[u] = max(pre[u], in_carry) + (j0 + u) * gap;
+      }
+#pragma unroll
+      for (int u = 0; u < kStrip; ++u) {
+        const int j = j0 + u;
+        if (j <= L) {
+          uint16_t cd;
+          if (bd[u] == h[u]) {
+            cd = kCodeDiag | static_cast<uint16_t>(kk[u] & 0xffff);
+          } else if (bv[u] == h[u]) {
+            cd = kCodeVert | static_cast<uint16_t>(kk[u] >> 16);
+          } else {
+            cd = kCodeHorz;
+          }
+          Hr[j] = static_cast<T>(h[u]);
+          Cr[j] = cd;
+          if (far) Hg[j] = h[u];
         }
       }
-    }
+      // value of the tile's last column: carry into the next tile and, in
+      // the last tile, the end-cell candidate
+      const int last_col = min(L, t0 + kTile - 1);
+      const int owner = (last_col - t0) / kStrip;
+      const int last_u = (last_col - t0) % kStrip;
+      int32_t lv = h[0];
 #pragma unroll
-    for (int u = 0; u < kStrip; ++u) {
-      const int j = j0 + u;
-      if (j <= L) {
-        const int32_t hl = u == 0 ? hleft : h[u - 1];
-        if (cd[u] == 0xffff && h[u] == hl + gap) {
-          cd[u] = kCodeHorz;
-        }
-        Cr[j] = cd[u];
+      for (int u = 1; u < kStrip; ++u) {
+        if (u == last_u) lv = h[u];
       }
+      carry = __shfl(lv, owner, kWave);
+      h_last = carry;
     }
-    // carry into the next tile: the value of column t0 + kTile - 1
-    const int last_lane_col = min(L, t0 + kTile - 1);
-    const int owner = (last_lane_col - t0) / kStrip;
-    const int last_u = (last_lane_col - t0) % kStrip;
-    int32_t last_val = h[0];
-#pragma unroll
-    for (int u = 1; u < kStrip; ++u) {
-      if (u == last_u) {
-        last_val = h[u];
-      }
+    if ((meta & kRowMetaSink) != 0 && h_last > best) {
+      best = h_last;
+      best_row = r;
     }
-    carry = __shfl(last_val, owner, kWave);
+    if (far) {
+      wait_stores();
+    }
+    wave_fence();
   }
+  return best_row;
 }
 
 // ---- the window ----
 
-__device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
+__device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const PoaWindow& W,
                                PoaResult& R) {
   const int lane = lane_id();
   const PoaSlotCaps& caps = P.caps;
@@ -438,6 +518,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
   const int n = static_cast<int>(W.backbone_len);
   uint64_t cells = 0;
   int max_rows = 0;
+  PhaseClock clk;
 
   // backbone chain: nodes 0..n-1, edges i-1 -> i (weights q-33), label 0
   if (n > static_cast<int>(caps.nodes) || n - 1 > static_cast<int>(caps.edges)) {
@@ -466,6 +547,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
   s.n_edges = n > 0 ? n - 1 : 0;
   s.n_seqs = 1;
   wave_sync();
+  clk.lap(R, kPhaseSetup);
 
   for (uint32_t li = 0; li < W.n_layers; ++li) {
     const PoaLayer& Lr = P.layers[W.layer_first + li];
@@ -475,8 +557,11 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
       return;
     }
     // layer bytes and weights into the slot
+    uint8_t* lds_seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
     for (int j = lane; j < L; j += kWave) {
-      s.seq[j] = P.arena[Lr.seq_off + j];
+      const uint8_t c = P.arena[Lr.seq_off + j];
+      s.seq[j] = c;
+      lds_seq[j] = c;
       s.qual[j] = Lr.qual_off == kNoQuality
                       ? 1
                       : static_cast<int>(P.arena[Lr.qual_off + j]) - 33;
@@ -519,6 +604,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
       }
     }
 
+    clk.lap(R, kPhaseInclude);
     // 2. topological order of the included nodes
     int V;
     if (full) {
@@ -539,8 +625,15 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
     }
     max_rows = max(max_rows, V + 1);
     wave_sync();
+    clk.lap(R, kPhaseSubSort);
 
-    // DP row of every included node; predecessor rows and sink flags
+    // score width: 16-bit ring rows when every score of the layer fits
+    const int32_t max_abs = max(max(abs(P.match), abs(P.mismatch)), abs(P.gap));
+    const bool narrow = static_cast<int64_t>(max_abs) * (V + L + 2) < 32000;
+    const int ring = narrow ? static_cast<int>(P.ring_rows)
+                            : max(1, static_cast<int>(P.ring_rows) / 2);
+
+    // DP row of every included node; predecessor rows, sink and far flags
     for (int r = lane; r < V; r += kWave) {
       s.row_of[s.sub_order[r]] = r + 1;
     }
@@ -562,12 +655,16 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
             cnt = 1;  // the virtual start row
           }
           int out_cnt = 0;
+          int last_use = 0;
           for (int e = s.out_head[v]; e != -1; e = s.e_next_out[e]) {
-            if (s.in_sub[s.e_head[e]] != 0) {
+            const int h = s.e_head[e];
+            if (s.in_sub[h] != 0) {
               ++out_cnt;
+              last_use = max(last_use, s.row_of[h]);
             }
           }
-          s.sink[r] = out_cnt == 0 ? 1 : 0;
+          s.row_meta[r] = static_cast<int>(s.base[v]) | (out_cnt == 0 ? kRowMetaSink : 0) |
+                          (last_use - (r + 1) >= ring ? kRowMetaFar : 0);
         }
         // inclusive scan of cnt
         int x = cnt;
@@ -602,62 +699,14 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
     }
     wave_sync();
 
-    // 3. DP. Row 0 and column 0 first.
-    for (int j = lane; j <= L; j += kWave) {
-      s.H[j] = j * P.gap;
-    }
+    clk.lap(R, kPhaseRows);
+    // 3. DP over the layer (column 0 folded into each row)
+    clk.lap(R, kPhaseColumn0);
+    const int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, ring, nullptr)
+                                        : dp_layer<int32_t>(P, s, lds, V, L, ring, nullptr));
     wave_sync();
-    // column 0: H[r][0] = max_p H[p][0] + g, code = first such p (vertical)
-    for (int r0 = 0; r0 < V; r0 += 1) {
-      const int r = r0 + 1;
-      const int b = uniform(s.pred_off[r0]);
-      const int e = uniform(s.pred_off[r0 + 1]);
-      int32_t best = kNegInf;
-      for (int k = b; k < e; ++k) {
-        best = max(best, s.H[static_cast<int64_t>(s.pred_rows[k]) * pitch]);
-      }
-      best += P.gap;
-      int first = 0;
-      for (int k = b; k < e; ++k) {
-        if (s.H[static_cast<int64_t>(s.pred_rows[k]) * pitch] + P.gap == best) {
-          first = k - b;
-          break;
-        }
-      }
-      s.H[static_cast<int64_t>(r) * pitch] = best;
-      s.code[static_cast<int64_t>(r) * pitch] = kCodeVert | static_cast<uint16_t>(first);
-    }
-    wave_sync();
-    for (int r = 1; r <= V; ++r) {
-      const int v = uniform(s.sub_order[r - 1]);
-      const int b = uniform(s.pred_off[r - 1]);
-      const int e = uniform(s.pred_off[r]);
-      dp_row(s, r, s.base[v], s.pred_rows + b, e - b, L, pitch, P.match, P.mismatch, P.gap);
-      wave_sync();
-    }
     cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(L);
-
-    // end cell: first sink row with the highest last-column score
-    int32_t best = kNegInf;
-    int best_row = 0x7fffffff;
-    for (int r = lane + 1; r <= V; r += kWave) {
-      if (s.sink[r - 1] != 0) {
-        const int32_t h = s.H[static_cast<int64_t>(r) * pitch + L];
-        if (h > best) {
-          best = h;
-          best_row = r;
-        }
-      }
-    }
-    for (int d = 1; d < kWave; d <<= 1) {
-      const int32_t ob = __shfl_xor(best, d, kWave);
-      const int orow = __shfl_xor(best_row, d, kWave);
-      if (ob > best || (ob == best && orow < best_row)) {
-        best = ob;
-        best_row = orow;
-      }
-    }
-    best_row = uniform(best_row);
+    clk.lap(R, kPhaseDp);
 
     // 4. traceback (lane-uniform), pairs collected backwards
     int na = 0;
@@ -700,6 +749,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
       }
     }
 
+    clk.lap(R, kPhaseTraceback);
     // 5. insert the layer (alignment walked forwards = pairs backwards)
     {
       int first_pos = -1, last_pos = -1;
@@ -779,6 +829,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
       return;
     }
     wave_sync();
+    clk.lap(R, kPhaseInsert);
     // new topological order of the whole graph
     topo_sort(s, nullptr, s.rank_to_node, static_cast<int>(caps.stack));
     if (s.overflow) {
@@ -786,6 +837,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
       return;
     }
     wave_sync();
+    clk.lap(R, kPhaseSort);
   }
 
   // heaviest bundle (lane-uniform)
@@ -919,6 +971,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
   for (int i = lane; i < out_len; i += kWave) {
     P.out[W.out_off + i] = s.base[s.path[plen - 1 - (begin + i)]];
   }
+  clk.lap(R, kPhaseConsensus);
   R.length = static_cast<uint32_t>(out_len);
   R.status = status;
   R.cells = cells;
@@ -930,6 +983,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, const PoaWindow& W,
 
 // Persistent batch kernel: each wave takes windows from the shared queue.
 __global__ __launch_bounds__(64) void poa_batch_kernel(PoaParams P) {
+  extern __shared__ uint8_t lds[];
   uint8_t* slot_mem = P.scratch + static_cast<uint64_t>(blockIdx.x) * P.slot_bytes;
   __shared__ uint32_t next;
   while (true) {
@@ -944,8 +998,9 @@ __global__ __launch_bounds__(64) void poa_batch_kernel(PoaParams P) {
     }
     const uint32_t w = P.order[q];
     Slot s = carve(slot_mem, P.layout);
-    PoaResult res{0, kPoaNotRun, 0, 0, 0};
-    process_window(P, s, P.windows[w], res);
+    PoaResult res{};
+    res.status = kPoaNotRun;
+    process_window(P, s, lds, P.windows[w], res);
     __syncthreads();
     if (threadIdx.x == 0) {
       P.results[w] = res;
@@ -956,10 +1011,12 @@ __global__ __launch_bounds__(64) void poa_batch_kernel(PoaParams P) {
 
 // Plain kernel: one launch per window (single wave, window index given).
 __global__ __launch_bounds__(64) void poa_single_kernel(PoaParams P, uint32_t q) {
+  extern __shared__ uint8_t lds[];
   const uint32_t w = P.order[q];
   Slot s = carve(P.scratch, P.layout);
-  PoaResult res{0, kPoaNotRun, 0, 0, 0};
-  process_window(P, s, P.windows[w], res);
+  PoaResult res{};
+  res.status = kPoaNotRun;
+  process_window(P, s, lds, P.windows[w], res);
   __syncthreads();
   if (threadIdx.x == 0) {
     P.results[w] = res;
@@ -972,17 +1029,18 @@ namespace rh {
 
 // Host launchers (kept in this translation unit with the kernels).
 hipError_t launch_poa_batch(const PoaParams& params, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL(poa_batch_kernel, dim3(grid), dim3(64), 0, stream, params);
+  hipLaunchKernelGGL(poa_batch_kernel, dim3(grid), dim3(64), params.lds_bytes, stream, params);
   return hipGetLastError();
 }
 
 hipError_t launch_poa_single(const PoaParams& params, uint32_t q, hipStream_t stream) {
-  hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), 0, stream, params, q);
+  hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), params.lds_bytes, stream, params, q);
   return hipGetLastError();
 }
 
-hipError_t poa_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, poa_batch_kernel, 64, 0);
+hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, poa_batch_kernel, 64,
+                                                      lds_bytes);
 }
 
 }  // namespace rh

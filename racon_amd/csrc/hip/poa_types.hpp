@@ -34,6 +34,21 @@ struct PoaLayer {
 
 constexpr uint64_t kNoQuality = ~uint64_t(0);
 
+// Phases of the per-window work, timed with the device's constant clock.
+enum : int {
+  kPhaseInclude = 0,   // subgraph node set
+  kPhaseSubSort,       // subgraph topological order
+  kPhaseRows,          // DP row map and predecessor lists
+  kPhaseColumn0,       // DP column 0
+  kPhaseDp,            // DP rows
+  kPhaseTraceback,     // end cell and traceback
+  kPhaseInsert,        // graph insertion
+  kPhaseSort,          // topological order of the whole graph
+  kPhaseConsensus,     // heaviest bundle, coverage, trimming
+  kPhaseSetup,         // backbone chain
+  kPoaPhases
+};
+
 // Per-window result written by the kernel.
 struct PoaResult {
   uint32_t length;     // consensus bytes written at out_off
@@ -41,6 +56,7 @@ struct PoaResult {
   uint64_t cells;      // DP cells computed (sum |V_aligned| * L)
   uint32_t nodes;      // final graph size
   uint32_t max_rows;   // largest DP row count used
+  uint64_t phase[kPoaPhases];  // device clock ticks spent per phase
 };
 
 enum : int32_t {
@@ -71,10 +87,10 @@ struct PoaSlotLayout {
   uint64_t base, in_head, out_head, in_tail, out_tail, n_aligned, aligned, cov,
       last_label, e_tail, e_head, e_weight, e_next_in, e_next_out, rank_to_node,
       node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
-      sink, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, total;
+      row_meta, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, total;
 };
 
-inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   PoaSlotLayout l{};
@@ -109,7 +125,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.sub_order = take(4 * n);
   l.pred_off = take(4 * (n + 1));
   l.pred_rows = take(4 * (e + n));
-  l.sink = take(n);
+  l.row_meta = take(4 * n);
   l.H = take(4 * uint64_t(c.rows) * c.cols);
   l.code = take(2 * uint64_t(c.rows) * c.cols);
   l.aln_node = take(4 * (n + c.cols));
@@ -140,6 +156,14 @@ struct PoaParams {
   int32_t mismatch;
   int32_t gap;
   uint32_t n_slots;
+  uint32_t ring_rows;  // DP rows of 16-bit scores kept in LDS (int32 mode: half)
+  uint32_t lds_bytes;  // dynamic LDS per wave: ring + layer bytes
 };
+
+// Dynamic LDS of one wave: the DP row ring (ring_rows x cols x 2 bytes)
+// followed by the current layer's bytes (cols).
+inline uint32_t poa_lds_bytes(uint32_t cols, uint32_t ring_rows) {
+  return static_cast<uint32_t>(align_up(uint64_t(ring_rows) * cols * 2, 16) + align_up(cols, 16));
+}
 
 }  // namespace rh

@@ -21,7 +21,7 @@ namespace rh {
 
 hipError_t launch_poa_batch(const PoaParams& params, uint32_t grid, hipStream_t stream);
 hipError_t launch_poa_single(const PoaParams& params, uint32_t q, hipStream_t stream);
-hipError_t poa_occupancy(int* blocks_per_cu);
+hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes);
 
 namespace {
 thread_local std::string g_error;
@@ -68,6 +68,12 @@ using rh::PoaResult;
 using rh::PoaSlotCaps;
 using rh::PoaWindow;
 
+PoaResult make_result() {
+  PoaResult r{};
+  r.status = rh::kPoaNotRun;
+  return r;
+}
+
 bool is_retry_status(int32_t s) {
   return s == rh::kPoaNodeOverflow || s == rh::kPoaRowOverflow || s == rh::kPoaOutOverflow;
 }
@@ -75,6 +81,15 @@ bool is_retry_status(int32_t s) {
 // Estimated cost of a window: layers x backbone x graph growth.
 uint64_t window_cost(const PoaWindow& w) {
   return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
+}
+
+// DP rows kept in LDS: 8 rows of 16-bit scores when they fit in 56 KiB.
+uint32_t ring_rows_for(uint32_t cols) {
+  uint32_t rows = 8;
+  while (rows > 2 && rh::poa_lds_bytes(cols, rows) > 56u * 1024u) {
+    rows -= 2;
+  }
+  return rows;
 }
 
 int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaSlotCaps& caps,
@@ -120,6 +135,8 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   p.mismatch = b->mismatch;
   p.gap = b->gap;
   p.n_slots = static_cast<uint32_t>(slots);
+  p.ring_rows = ring_rows_for(caps.cols);
+  p.lds_bytes = rh::poa_lds_bytes(caps.cols, p.ring_rows);
 
   RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
   if (mode == RH_POA_PLAIN) {
@@ -235,7 +252,7 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   b->shallow.assign(n, 0);
   b->backbone_off.assign(n, 0);
   b->backbone_len.assign(n, 0);
-  b->results.assign(n, PoaResult{0, rh::kPoaNotRun, 0, 0, 0});
+  b->results.assign(n, make_result());
   b->stats = rh_poa_stats{};
 
   uint64_t total = 0;
@@ -388,7 +405,7 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
     return RH_OK;
   }
   // fresh results: every kernel window starts as not-run
-  std::vector<PoaResult> init(n, PoaResult{0, rh::kPoaNotRun, 0, 0, 0});
+  std::vector<PoaResult> init(n, make_result());
   RH_HIP_CHECK(hipMemcpyAsync(b->d_results.ptr, init.data(), n * sizeof(PoaResult),
                               hipMemcpyHostToDevice, b->stream),
                RH_ERR_DEVICE);
@@ -407,8 +424,13 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   caps.rows = std::min<uint32_t>(max_nodes + 1, b->tune_rows ? b->tune_rows : 4096);
   caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
 
+  if (rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols)) > 64u * 1024u) {
+    rh::set_error("rh_poa_run: layers longer than the LDS ring allows");
+    return RH_ERR_ARGUMENT;
+  }
   int per_cu = 0, cus = 0;
-  RH_HIP_CHECK(rh::poa_occupancy(&per_cu), RH_ERR_DEVICE);
+  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols))),
+               RH_ERR_DEVICE);
   RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
                RH_ERR_DEVICE);
   uint32_t resident = static_cast<uint32_t>(std::max(1, per_cu) * std::max(1, cus));
@@ -536,6 +558,27 @@ int32_t rh_poa_download(rh_poa_batch* b, char* out, uint64_t capacity, uint64_t*
     pos += len;
   }
   b->stats.d2h_ms = rh::ms_since(t0);
+  return RH_OK;
+}
+
+int32_t rh_poa_phase_ticks(const rh_poa_batch* b, uint64_t* ticks, uint32_t n,
+                           uint64_t* ticks_per_second) {
+  if (b == nullptr || ticks == nullptr) {
+    rh::set_error("rh_poa_phase_ticks: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  if (ticks_per_second != nullptr) {
+    int khz = 0;
+    RH_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device),
+                 RH_ERR_DEVICE);
+    *ticks_per_second = static_cast<uint64_t>(khz) * 1000ull;
+  }
+  for (uint32_t k = 0; k < n; ++k) ticks[k] = 0;
+  for (uint32_t w : b->gpu_order) {
+    for (uint32_t k = 0; k < n && k < static_cast<uint32_t>(rh::kPoaPhases); ++k) {
+      ticks[k] += b->results[w].phase[k];
+    }
+  }
   return RH_OK;
 }
 

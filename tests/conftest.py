@@ -7,8 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 TESTS = os.path.dirname(os.path.abspath(__file__))
-if TESTS not in sys.path:
-    sys.path.insert(0, TESTS)
+ORACLE = os.path.join(ROOT, "oracle")
+for _p in (TESTS, ORACLE):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
 
 DATA = os.path.join(ROOT, "tests", "golden", "racon_data")
 
