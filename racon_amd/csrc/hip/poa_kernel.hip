@@ -95,6 +95,9 @@ struct Slot {
   int32_t* path;
   uint8_t* seq;
   int32_t* qual;  // per-base weights of the current layer
+  int32_t* anchor;      // per new node of a layer: rank it is placed after (-1: front)
+  int32_t* lpath;       // graph nodes of the inserted layer, in path order
+  int32_t* sink_score;  // last-column score of each sink DP row
 
   int32_t n_nodes;
   int32_t n_edges;
@@ -138,6 +141,9 @@ __device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
   s.path = reinterpret_cast<int32_t*>(p + l.path);
   s.seq = p + l.seq;
   s.qual = reinterpret_cast<int32_t*>(p + l.qual);
+  s.anchor = reinterpret_cast<int32_t*>(p + l.anchor);
+  s.lpath = reinterpret_cast<int32_t*>(p + l.lpath);
+  s.sink_score = reinterpret_cast<int32_t*>(p + l.sink_score);
   s.n_nodes = 0;
   s.n_edges = 0;
   s.n_seqs = 0;
@@ -346,8 +352,7 @@ __device__ __forceinline__ const T* ring_row(const T* ring, int pitch, int R, in
 // end cell: the first sink row with the highest score in column L.
 template <typename T>
 __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, int L,
-                        int R, uint64_t* ticks) {
-  (void)ticks;
+                        int R, int* best_score, bool* tied) {
   const int lane = lane_id();
   const int pitch = static_cast<int>(P.caps.cols);
   const int32_t gap = P.gap;
@@ -355,6 +360,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   const uint8_t* seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
   int32_t best = kNegInf;
   int best_row = 0;
+  int ties = 0;
   for (int r = 1; r <= V; ++r) {
     const int off = uniform(s.pred_off[r - 1]);
     const int np = uniform(s.pred_off[r]) - off;
@@ -494,15 +500,25 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       carry = __shfl(lv, owner, kWave);
       h_last = carry;
     }
-    if ((meta & kRowMetaSink) != 0 && h_last > best) {
-      best = h_last;
-      best_row = r;
+    if ((meta & kRowMetaSink) != 0) {
+      if (lane == 0) {
+        s.sink_score[r] = h_last;
+      }
+      if (h_last > best) {
+        best = h_last;
+        best_row = r;
+        ties = 0;
+      } else if (h_last == best) {
+        ++ties;
+      }
     }
     if (far) {
       wait_stores();
     }
     wave_fence();
   }
+  *best_score = best;
+  *tied = ties > 0;
   return best_row;
 }
 
@@ -535,6 +551,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     s.cov[i] = n >= 2 ? 1 : 0;
     s.last_label[i] = n >= 2 ? 0 : -1;
     s.rank_to_node[i] = i;
+    s.node_rank[i] = i;
     if (i + 1 < n) {
       s.e_tail[i] = i;
       s.e_head[i] = i + 1;
@@ -605,18 +622,26 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     }
 
     clk.lap(R, kPhaseInclude);
-    // 2. topological order of the included nodes
-    int V;
+    // 2. DP row order: the maintained topological order restricted to the
+    // included nodes (the DP values and move codes do not depend on which
+    // topological order is used; spoa's exact DFS order is computed only
+    // where it decides a tie)
+    int V = 0;
     if (full) {
       V = nn;
       for (int r = lane; r < V; r += kWave) {
         s.sub_order[r] = s.rank_to_node[r];
       }
     } else {
-      V = topo_sort(s, s.in_sub, s.sub_order, static_cast<int>(caps.stack));
-      if (s.overflow) {
-        R.status = kPoaNodeOverflow;
-        return;
+      for (int r0 = 0; r0 < nn; r0 += kWave) {
+        const int r = r0 + lane;
+        const int v = r < nn ? s.rank_to_node[r] : 0;
+        const bool keep = r < nn && s.in_sub[v] != 0;
+        const unsigned long long mask = __ballot(keep);
+        if (keep) {
+          s.sub_order[V + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+        }
+        V += __popcll(mask);
       }
     }
     if (V + 1 > static_cast<int>(caps.rows)) {
@@ -702,9 +727,37 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     clk.lap(R, kPhaseRows);
     // 3. DP over the layer (column 0 folded into each row)
     clk.lap(R, kPhaseColumn0);
-    const int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, ring, nullptr)
-                                        : dp_layer<int32_t>(P, s, lds, V, L, ring, nullptr));
+    int best_score = 0;
+    bool tied = false;
+    int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, ring, &best_score, &tied)
+                                  : dp_layer<int32_t>(P, s, lds, V, L, ring, &best_score, &tied));
     wave_sync();
+    if (tied) {
+      // several sinks share the best score: spoa takes the first one in its
+      // DFS order of the (sub)graph
+      const int cnt = topo_sort(s, full ? nullptr : s.in_sub, s.path,
+                                static_cast<int>(caps.stack));
+      if (s.overflow) {
+        R.status = kPoaNodeOverflow;
+        return;
+      }
+      wave_sync();
+      int found = -1;
+      for (int i0 = 0; i0 < cnt && found < 0; i0 += kWave) {
+        const int i = i0 + lane;
+        bool hit = false;
+        int row = 0;
+        if (i < cnt) {
+          row = s.row_of[s.path[i]];
+          hit = (s.row_meta[row - 1] & kRowMetaSink) != 0 && s.sink_score[row] == best_score;
+        }
+        const unsigned long long mask = __ballot(hit);
+        if (mask != 0) {
+          found = __shfl(row, __ffsll(static_cast<long long>(mask)) - 1, kWave);
+        }
+      }
+      best_row = uniform(found);
+    }
     cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(L);
     clk.lap(R, kPhaseDp);
 
@@ -751,6 +804,9 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
 
     clk.lap(R, kPhaseTraceback);
     // 5. insert the layer (alignment walked forwards = pairs backwards)
+    bool order_ok = true;
+    int n_path = 0;
+    const int nodes_before = s.n_nodes;
     {
       int first_pos = -1, last_pos = -1;
       for (int q = na - 1; q >= 0; --q) {
@@ -769,7 +825,11 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
       add_chain(s, caps, 0, first_pos);
       int head = before == s.n_nodes ? -1 : s.n_nodes - 1;
       const int tail = add_chain(s, caps, last_pos + 1, L);
+      if (s.n_nodes != before) {
+        order_ok = false;  // chains outside the alignment (not produced by NW)
+      }
       int prev_w = head == -1 ? 0 : s.qual[first_pos - 1];
+      int head_anchor = -1;  // anchor of the previous path node when it is new
       for (int q = na - 1; q >= 0 && !s.overflow; --q) {
         const int pos = uniform(s.aln_pos[q]);
         if (pos == -1) {
@@ -813,6 +873,37 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
             nv = same;
           }
         }
+        if (nv >= before) {
+          // a new node: placed after its path predecessor's aligned set
+          // (mismatch nodes: after the aligned set they join)
+          int anchor;
+          if (node != -1) {
+            anchor = s.node_rank[node];
+            const int cnt = s.n_aligned[node];
+            for (int k = 0; k < cnt; ++k) {
+              const int a = uniform(s.aligned[node * kMaxAligned + k]);
+              if (a < before) anchor = max(anchor, s.node_rank[a]);
+            }
+          } else if (head == -1) {
+            anchor = -1;
+          } else if (head >= before) {
+            anchor = head_anchor;
+          } else {
+            anchor = s.node_rank[head];
+            const int cnt = s.n_aligned[head];
+            for (int k = 0; k < cnt; ++k) {
+              const int a = uniform(s.aligned[head * kMaxAligned + k]);
+              if (a < before) anchor = max(anchor, s.node_rank[a]);
+            }
+          }
+          const int k_new = nv - before;
+          if (k_new > 0 && anchor < s.anchor[k_new - 1]) {
+            order_ok = false;
+          }
+          s.anchor[k_new] = anchor;
+          head_anchor = anchor;
+        }
+        s.lpath[n_path++] = nv;
         if (head != -1) {
           add_edge(s, caps, head, nv, prev_w + s.qual[pos]);
         }
@@ -830,15 +921,66 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     }
     wave_sync();
     clk.lap(R, kPhaseInsert);
-    // new topological order of the whole graph
+    // topological order of the grown graph: new nodes merged after their
+    // anchors (old nodes keep their relative order), then checked along the
+    // inserted path; any violation falls back to the exact DFS order
+    {
+      const int M = s.n_nodes - nodes_before;
+      const int total = s.n_nodes;
+      if (order_ok && M > 0) {
+        for (int r = lane; r < nodes_before; r += kWave) {
+          // number of anchors < r (anchors are non-decreasing)
+          int lo = 0, hi = M;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s.anchor[mid] < r) lo = mid + 1; else hi = mid;
+          }
+          s.sub_order[r + lo] = s.rank_to_node[r];
+        }
+        for (int k = lane; k < M; k += kWave) {
+          s.sub_order[s.anchor[k] + 1 + k] = nodes_before + k;
+        }
+        wave_sync();
+        for (int r = lane; r < total; r += kWave) {
+          const int v = s.sub_order[r];
+          s.rank_to_node[r] = v;
+          s.node_rank[v] = r;
+        }
+        wave_sync();
+      }
+      bool bad = !order_ok;
+      if (!bad) {
+        for (int i = lane; i + 1 < n_path; i += kWave) {
+          if (s.node_rank[s.lpath[i]] >= s.node_rank[s.lpath[i + 1]]) bad = true;
+        }
+        bad = __ballot(bad) != 0;
+      }
+      if (bad) {
+        topo_sort(s, nullptr, s.rank_to_node, static_cast<int>(caps.stack));
+        if (s.overflow) {
+          R.status = kPoaNodeOverflow;
+          return;
+        }
+        wave_sync();
+        for (int r = lane; r < total; r += kWave) {
+          s.node_rank[s.rank_to_node[r]] = r;
+        }
+      }
+    }
+    wave_sync();
+    clk.lap(R, kPhaseSort);
+  }
+
+  // spoa's exact DFS order of the final graph for the heaviest bundle
+  if (W.n_layers > 0) {
     topo_sort(s, nullptr, s.rank_to_node, static_cast<int>(caps.stack));
     if (s.overflow) {
       R.status = kPoaNodeOverflow;
       return;
     }
     wave_sync();
-    clk.lap(R, kPhaseSort);
   }
+  clk.lap(R, kPhaseSort);
 
   // heaviest bundle (lane-uniform)
   const int N = s.n_nodes;

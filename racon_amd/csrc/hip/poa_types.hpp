@@ -87,7 +87,8 @@ struct PoaSlotLayout {
   uint64_t base, in_head, out_head, in_tail, out_tail, n_aligned, aligned, cov,
       last_label, e_tail, e_head, e_weight, e_next_in, e_next_out, rank_to_node,
       node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
-      row_meta, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, total;
+      row_meta, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, anchor, lpath,
+      sink_score, total;
 };
 
 constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -135,6 +136,9 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.path = take(4 * n);
   l.seq = take(c.cols);
   l.qual = take(4 * uint64_t(c.cols));
+  l.anchor = take(4 * uint64_t(c.cols));
+  l.lpath = take(4 * uint64_t(c.cols));
+  l.sink_score = take(4 * (n + 1));
   l.total = o;
   return l;
 }
