@@ -35,8 +35,8 @@ namespace rh {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kStrip = 8;                  // DP columns per lane per tile
-constexpr int kTile = kWave * kStrip;      // 512 columns per tile
+constexpr int kStrip = 4;                  // DP columns per lane per tile
+constexpr int kTile = kWave * kStrip;      // 256 columns per tile
 constexpr int32_t kNegInf = -(1 << 29);
 constexpr uint16_t kCodeDiag = 0x0000;
 constexpr uint16_t kCodeVert = 0x4000;
@@ -59,45 +59,49 @@ struct PhaseClock {
   }
 };
 
-// Views of one slot's arrays.
+// Views of one slot's arrays: one base pointer plus the layout's offsets
+// (kernel arguments), so the array addresses are rebuilt where used instead of
+// occupying registers.
 struct Slot {
-  uint8_t* base;
-  int32_t* in_head;
-  int32_t* out_head;
-  int32_t* in_tail;
-  int32_t* out_tail;
-  uint8_t* n_aligned;
-  int32_t* aligned;
-  int32_t* cov;
-  int32_t* last_label;
-  int32_t* e_tail;
-  int32_t* e_head;
-  int32_t* e_weight;
-  int32_t* e_next_in;
-  int32_t* e_next_out;
-  int32_t* rank_to_node;
-  int32_t* node_rank;
-  uint8_t* mark;
-  uint8_t* lead;
-  int32_t* stack;
-  uint8_t* in_sub;
-  int32_t* row_of;
-  int32_t* sub_order;
-  int32_t* pred_off;
-  int32_t* pred_rows;
-  int32_t* row_meta;
-  int32_t* H;
-  uint16_t* code;
-  int32_t* aln_node;
-  int32_t* aln_pos;
-  int64_t* scores;
-  int32_t* preds;
-  int32_t* path;
-  uint8_t* seq;
-  int32_t* qual;  // per-base weights of the current layer
-  int32_t* anchor;      // per new node of a layer: rank it is placed after (-1: front)
-  int32_t* lpath;       // graph nodes of the inserted layer, in path order
-  int32_t* sink_score;  // last-column score of each sink DP row
+  uint8_t* p;
+  const PoaSlotLayout* L;
+  __device__ uint8_t* base() const { return reinterpret_cast<uint8_t*>(p + L->base); }
+  __device__ int32_t* in_head() const { return reinterpret_cast<int32_t*>(p + L->in_head); }
+  __device__ int32_t* out_head() const { return reinterpret_cast<int32_t*>(p + L->out_head); }
+  __device__ int32_t* in_tail() const { return reinterpret_cast<int32_t*>(p + L->in_tail); }
+  __device__ int32_t* out_tail() const { return reinterpret_cast<int32_t*>(p + L->out_tail); }
+  __device__ uint8_t* n_aligned() const { return reinterpret_cast<uint8_t*>(p + L->n_aligned); }
+  __device__ int32_t* aligned() const { return reinterpret_cast<int32_t*>(p + L->aligned); }
+  __device__ int32_t* cov() const { return reinterpret_cast<int32_t*>(p + L->cov); }
+  __device__ int32_t* last_label() const { return reinterpret_cast<int32_t*>(p + L->last_label); }
+  __device__ int32_t* e_tail() const { return reinterpret_cast<int32_t*>(p + L->e_tail); }
+  __device__ int32_t* e_head() const { return reinterpret_cast<int32_t*>(p + L->e_head); }
+  __device__ int32_t* e_weight() const { return reinterpret_cast<int32_t*>(p + L->e_weight); }
+  __device__ int32_t* e_next_in() const { return reinterpret_cast<int32_t*>(p + L->e_next_in); }
+  __device__ int32_t* e_next_out() const { return reinterpret_cast<int32_t*>(p + L->e_next_out); }
+  __device__ int32_t* rank_to_node() const { return reinterpret_cast<int32_t*>(p + L->rank_to_node); }
+  __device__ int32_t* node_rank() const { return reinterpret_cast<int32_t*>(p + L->node_rank); }
+  __device__ uint8_t* mark() const { return reinterpret_cast<uint8_t*>(p + L->mark); }
+  __device__ uint8_t* lead() const { return reinterpret_cast<uint8_t*>(p + L->lead); }
+  __device__ int32_t* stack() const { return reinterpret_cast<int32_t*>(p + L->stack); }
+  __device__ uint8_t* in_sub() const { return reinterpret_cast<uint8_t*>(p + L->in_sub); }
+  __device__ int32_t* row_of() const { return reinterpret_cast<int32_t*>(p + L->row_of); }
+  __device__ int32_t* sub_order() const { return reinterpret_cast<int32_t*>(p + L->sub_order); }
+  __device__ int32_t* pred_off() const { return reinterpret_cast<int32_t*>(p + L->pred_off); }
+  __device__ int32_t* pred_rows() const { return reinterpret_cast<int32_t*>(p + L->pred_rows); }
+  __device__ int32_t* row_meta() const { return reinterpret_cast<int32_t*>(p + L->row_meta); }
+  __device__ int32_t* H() const { return reinterpret_cast<int32_t*>(p + L->H); }
+  __device__ uint16_t* code() const { return reinterpret_cast<uint16_t*>(p + L->code); }
+  __device__ int32_t* aln_node() const { return reinterpret_cast<int32_t*>(p + L->aln_node); }
+  __device__ int32_t* aln_pos() const { return reinterpret_cast<int32_t*>(p + L->aln_pos); }
+  __device__ int64_t* scores() const { return reinterpret_cast<int64_t*>(p + L->scores); }
+  __device__ int32_t* preds() const { return reinterpret_cast<int32_t*>(p + L->preds); }
+  __device__ int32_t* path() const { return reinterpret_cast<int32_t*>(p + L->path); }
+  __device__ uint8_t* seq() const { return reinterpret_cast<uint8_t*>(p + L->seq); }
+  __device__ int32_t* qual() const { return reinterpret_cast<int32_t*>(p + L->qual); }
+  __device__ int32_t* anchor() const { return reinterpret_cast<int32_t*>(p + L->anchor); }
+  __device__ int32_t* lpath() const { return reinterpret_cast<int32_t*>(p + L->lpath); }
+  __device__ int32_t* sink_score() const { return reinterpret_cast<int32_t*>(p + L->sink_score); }
 
   int32_t n_nodes;
   int32_t n_edges;
@@ -107,43 +111,8 @@ struct Slot {
 
 __device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
   Slot s;
-  s.base = p + l.base;
-  s.in_head = reinterpret_cast<int32_t*>(p + l.in_head);
-  s.out_head = reinterpret_cast<int32_t*>(p + l.out_head);
-  s.in_tail = reinterpret_cast<int32_t*>(p + l.in_tail);
-  s.out_tail = reinterpret_cast<int32_t*>(p + l.out_tail);
-  s.n_aligned = p + l.n_aligned;
-  s.aligned = reinterpret_cast<int32_t*>(p + l.aligned);
-  s.cov = reinterpret_cast<int32_t*>(p + l.cov);
-  s.last_label = reinterpret_cast<int32_t*>(p + l.last_label);
-  s.e_tail = reinterpret_cast<int32_t*>(p + l.e_tail);
-  s.e_head = reinterpret_cast<int32_t*>(p + l.e_head);
-  s.e_weight = reinterpret_cast<int32_t*>(p + l.e_weight);
-  s.e_next_in = reinterpret_cast<int32_t*>(p + l.e_next_in);
-  s.e_next_out = reinterpret_cast<int32_t*>(p + l.e_next_out);
-  s.rank_to_node = reinterpret_cast<int32_t*>(p + l.rank_to_node);
-  s.node_rank = reinterpret_cast<int32_t*>(p + l.node_rank);
-  s.mark = p + l.mark;
-  s.lead = p + l.lead;
-  s.stack = reinterpret_cast<int32_t*>(p + l.stack);
-  s.in_sub = p + l.in_sub;
-  s.row_of = reinterpret_cast<int32_t*>(p + l.row_of);
-  s.sub_order = reinterpret_cast<int32_t*>(p + l.sub_order);
-  s.pred_off = reinterpret_cast<int32_t*>(p + l.pred_off);
-  s.pred_rows = reinterpret_cast<int32_t*>(p + l.pred_rows);
-  s.row_meta = reinterpret_cast<int32_t*>(p + l.row_meta);
-  s.H = reinterpret_cast<int32_t*>(p + l.H);
-  s.code = reinterpret_cast<uint16_t*>(p + l.code);
-  s.aln_node = reinterpret_cast<int32_t*>(p + l.aln_node);
-  s.aln_pos = reinterpret_cast<int32_t*>(p + l.aln_pos);
-  s.scores = reinterpret_cast<int64_t*>(p + l.scores);
-  s.preds = reinterpret_cast<int32_t*>(p + l.preds);
-  s.path = reinterpret_cast<int32_t*>(p + l.path);
-  s.seq = p + l.seq;
-  s.qual = reinterpret_cast<int32_t*>(p + l.qual);
-  s.anchor = reinterpret_cast<int32_t*>(p + l.anchor);
-  s.lpath = reinterpret_cast<int32_t*>(p + l.lpath);
-  s.sink_score = reinterpret_cast<int32_t*>(p + l.sink_score);
+  s.p = p;
+  s.L = &l;
   s.n_nodes = 0;
   s.n_edges = 0;
   s.n_seqs = 0;
@@ -154,9 +123,9 @@ __device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
 // ---- graph construction (lane-uniform) ----
 
 __device__ void touch(Slot& s, int node, int label) {
-  if (s.last_label[node] != label) {
-    s.last_label[node] = label;
-    s.cov[node] = s.cov[node] + 1;
+  if (s.last_label()[node] != label) {
+    s.last_label()[node] = label;
+    s.cov()[node] = s.cov()[node] + 1;
   }
 }
 
@@ -166,21 +135,21 @@ __device__ int add_node(Slot& s, const PoaSlotCaps& caps, uint8_t base) {
     return 0;
   }
   const int v = s.n_nodes++;
-  s.base[v] = base;
-  s.in_head[v] = -1;
-  s.out_head[v] = -1;
-  s.in_tail[v] = -1;
-  s.out_tail[v] = -1;
-  s.n_aligned[v] = 0;
-  s.cov[v] = 0;
-  s.last_label[v] = -1;
+  s.base()[v] = base;
+  s.in_head()[v] = -1;
+  s.out_head()[v] = -1;
+  s.in_tail()[v] = -1;
+  s.out_tail()[v] = -1;
+  s.n_aligned()[v] = 0;
+  s.cov()[v] = 0;
+  s.last_label()[v] = -1;
   return v;
 }
 
 __device__ void add_edge(Slot& s, const PoaSlotCaps& caps, int tail, int head, int weight) {
-  for (int e = s.out_head[tail]; e != -1; e = s.e_next_out[e]) {
-    if (s.e_head[e] == head) {
-      s.e_weight[e] = s.e_weight[e] + weight;
+  for (int e = s.out_head()[tail]; e != -1; e = s.e_next_out()[e]) {
+    if (s.e_head()[e] == head) {
+      s.e_weight()[e] = s.e_weight()[e] + weight;
       touch(s, tail, s.n_seqs);
       touch(s, head, s.n_seqs);
       return;
@@ -191,23 +160,23 @@ __device__ void add_edge(Slot& s, const PoaSlotCaps& caps, int tail, int head, i
     return;
   }
   const int e = s.n_edges++;
-  s.e_tail[e] = tail;
-  s.e_head[e] = head;
-  s.e_weight[e] = weight;
-  s.e_next_in[e] = -1;
-  s.e_next_out[e] = -1;
-  if (s.out_tail[tail] == -1) {
-    s.out_head[tail] = e;
+  s.e_tail()[e] = tail;
+  s.e_head()[e] = head;
+  s.e_weight()[e] = weight;
+  s.e_next_in()[e] = -1;
+  s.e_next_out()[e] = -1;
+  if (s.out_tail()[tail] == -1) {
+    s.out_head()[tail] = e;
   } else {
-    s.e_next_out[s.out_tail[tail]] = e;
+    s.e_next_out()[s.out_tail()[tail]] = e;
   }
-  s.out_tail[tail] = e;
-  if (s.in_tail[head] == -1) {
-    s.in_head[head] = e;
+  s.out_tail()[tail] = e;
+  if (s.in_tail()[head] == -1) {
+    s.in_head()[head] = e;
   } else {
-    s.e_next_in[s.in_tail[head]] = e;
+    s.e_next_in()[s.in_tail()[head]] = e;
   }
-  s.in_tail[head] = e;
+  s.in_tail()[head] = e;
   touch(s, tail, s.n_seqs);
   touch(s, head, s.n_seqs);
 }
@@ -217,10 +186,10 @@ __device__ int add_chain(Slot& s, const PoaSlotCaps& caps, int begin, int end) {
   if (begin >= end) {
     return -1;
   }
-  const int first = add_node(s, caps, s.seq[begin]);
+  const int first = add_node(s, caps, s.seq()[begin]);
   for (int i = begin + 1; i < end && !s.overflow; ++i) {
-    const int v = add_node(s, caps, s.seq[i]);
-    add_edge(s, caps, v - 1, v, s.qual[i - 1] + s.qual[i]);
+    const int v = add_node(s, caps, s.seq()[i]);
+    add_edge(s, caps, v - 1, v, s.qual()[i - 1] + s.qual()[i]);
   }
   return first;
 }
@@ -230,8 +199,8 @@ __device__ int add_chain(Slot& s, const PoaSlotCaps& caps, int begin, int end) {
 __device__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* out, int stack_cap) {
   const int n = s.n_nodes;
   for (int v = lane_id(); v < n; v += kWave) {
-    s.mark[v] = 0;
-    s.lead[v] = 1;
+    s.mark()[v] = 0;
+    s.lead()[v] = 1;
   }
   wave_sync();
   int count = 0;
@@ -239,65 +208,65 @@ __device__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* out, int stack_c
     if (keep != nullptr && keep[root] == 0) {
       continue;
     }
-    if (s.mark[root] != 0) {
+    if (s.mark()[root] != 0) {
       continue;
     }
     int top = 0;
-    s.stack[top++] = root;
+    s.stack()[top++] = root;
     while (top > 0) {
-      const int v = uniform(s.stack[top - 1]);
+      const int v = uniform(s.stack()[top - 1]);
       bool ready = true;
-      if (s.mark[v] != 2) {
-        for (int e = uniform(s.in_head[v]); e != -1; e = uniform(s.e_next_in[e])) {
-          const int u = uniform(s.e_tail[e]);
+      if (s.mark()[v] != 2) {
+        for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+          const int u = uniform(s.e_tail()[e]);
           if (keep != nullptr && keep[u] == 0) {
             continue;
           }
-          if (s.mark[u] != 2) {
+          if (s.mark()[u] != 2) {
             if (top >= stack_cap) {
               s.overflow = true;
               return count;
             }
-            s.stack[top++] = u;
+            s.stack()[top++] = u;
             ready = false;
           }
         }
-        if (s.lead[v] != 0) {
-          const int na = s.n_aligned[v];
+        if (s.lead()[v] != 0) {
+          const int na = s.n_aligned()[v];
           for (int k = 0; k < na; ++k) {
-            const int a = uniform(s.aligned[v * kMaxAligned + k]);
+            const int a = uniform(s.aligned()[v * kMaxAligned + k]);
             if (keep != nullptr && keep[a] == 0) {
               continue;
             }
-            if (s.mark[a] != 2) {
+            if (s.mark()[a] != 2) {
               if (top >= stack_cap) {
                 s.overflow = true;
                 return count;
               }
-              s.stack[top++] = a;
-              s.lead[a] = 0;
+              s.stack()[top++] = a;
+              s.lead()[a] = 0;
               ready = false;
             }
           }
         }
         if (ready) {
-          s.mark[v] = 2;
-          if (s.lead[v] != 0) {
-            if (count + 1 + s.n_aligned[v] > n) {
+          s.mark()[v] = 2;
+          if (s.lead()[v] != 0) {
+            if (count + 1 + s.n_aligned()[v] > n) {
               s.overflow = true;
               return count;
             }
             out[count++] = v;
-            const int na = s.n_aligned[v];
+            const int na = s.n_aligned()[v];
             for (int k = 0; k < na; ++k) {
-              const int a = uniform(s.aligned[v * kMaxAligned + k]);
+              const int a = uniform(s.aligned()[v * kMaxAligned + k]);
               if (keep == nullptr || keep[a] != 0) {
                 out[count++] = a;
               }
             }
           }
         } else {
-          s.mark[v] = 1;
+          s.mark()[v] = 1;
         }
       }
       if (ready) {
@@ -362,27 +331,27 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   int best_row = 0;
   int ties = 0;
   for (int r = 1; r <= V; ++r) {
-    const int off = uniform(s.pred_off[r - 1]);
-    const int np = uniform(s.pred_off[r]) - off;
-    const int meta = uniform(s.row_meta[r - 1]);
+    const int off = uniform(s.pred_off()[r - 1]);
+    const int np = uniform(s.pred_off()[r]) - off;
+    const int meta = uniform(s.row_meta()[r - 1]);
     const uint8_t b = static_cast<uint8_t>(meta & 0xff);
     T* Hr = ring + (r % R) * pitch;
-    int32_t* Hg = s.H + static_cast<int64_t>(r) * pitch;
-    uint16_t* Cr = s.code + static_cast<int64_t>(r) * pitch;
+    int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+    uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
     const bool far = (meta & kRowMetaFar) != 0;
 
     // column 0: max over predecessors of their column 0, plus gap
     int32_t c0 = kNegInf;
     int k0 = 0;
     for (int k = 0; k < np; ++k) {
-      const int p = uniform(s.pred_rows[off + k]);
+      const int p = uniform(s.pred_rows()[off + k]);
       int32_t h0;
       if (p == 0) {
         h0 = 0;
       } else if (r - p < R) {
         h0 = ring_row(ring, pitch, R, p)[0];
       } else {
-        h0 = s.H[static_cast<int64_t>(p) * pitch];
+        h0 = s.H()[static_cast<int64_t>(p) * pitch];
       }
       if (h0 + gap > c0) {
         c0 = h0 + gap;
@@ -414,7 +383,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
         kk[u] = 0;
       }
       for (int k = 0; k < np; ++k) {
-        const int p = uniform(s.pred_rows[off + k]);
+        const int p = uniform(s.pred_rows()[off + k]);
         int32_t hp[kStrip + 1];
         if (p == 0) {
 #pragma unroll
@@ -427,7 +396,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
             hp[u] = j <= L ? static_cast<int32_t>(row[j]) : kNegInf;
           }
         } else {
-          const int32_t* row = s.H + static_cast<int64_t>(p) * pitch;
+          const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
 #pragma unroll
           for (int u = 0; u <= kStrip; ++u) {
             const int j = j0 - 1 + u;
@@ -502,7 +471,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
     }
     if ((meta & kRowMetaSink) != 0) {
       if (lane == 0) {
-        s.sink_score[r] = h_last;
+        s.sink_score()[r] = h_last;
       }
       if (h_last > best) {
         best = h_last;
@@ -542,22 +511,22 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     return;
   }
   for (int i = lane; i < n; i += kWave) {
-    s.base[i] = bb[i];
-    s.in_head[i] = i > 0 ? i - 1 : -1;
-    s.in_tail[i] = i > 0 ? i - 1 : -1;
-    s.out_head[i] = i + 1 < n ? i : -1;
-    s.out_tail[i] = i + 1 < n ? i : -1;
-    s.n_aligned[i] = 0;
-    s.cov[i] = n >= 2 ? 1 : 0;
-    s.last_label[i] = n >= 2 ? 0 : -1;
-    s.rank_to_node[i] = i;
-    s.node_rank[i] = i;
+    s.base()[i] = bb[i];
+    s.in_head()[i] = i > 0 ? i - 1 : -1;
+    s.in_tail()[i] = i > 0 ? i - 1 : -1;
+    s.out_head()[i] = i + 1 < n ? i : -1;
+    s.out_tail()[i] = i + 1 < n ? i : -1;
+    s.n_aligned()[i] = 0;
+    s.cov()[i] = n >= 2 ? 1 : 0;
+    s.last_label()[i] = n >= 2 ? 0 : -1;
+    s.rank_to_node()[i] = i;
+    s.node_rank()[i] = i;
     if (i + 1 < n) {
-      s.e_tail[i] = i;
-      s.e_head[i] = i + 1;
-      s.e_weight[i] = (static_cast<int>(bq[i]) - 33) + (static_cast<int>(bq[i + 1]) - 33);
-      s.e_next_in[i] = -1;
-      s.e_next_out[i] = -1;
+      s.e_tail()[i] = i;
+      s.e_head()[i] = i + 1;
+      s.e_weight()[i] = (static_cast<int>(bq[i]) - 33) + (static_cast<int>(bq[i + 1]) - 33);
+      s.e_next_in()[i] = -1;
+      s.e_next_out()[i] = -1;
     }
   }
   s.n_nodes = n;
@@ -577,9 +546,9 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     uint8_t* lds_seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
     for (int j = lane; j < L; j += kWave) {
       const uint8_t c = P.arena[Lr.seq_off + j];
-      s.seq[j] = c;
+      s.seq()[j] = c;
       lds_seq[j] = c;
-      s.qual[j] = Lr.qual_off == kNoQuality
+      s.qual()[j] = Lr.qual_off == kNoQuality
                       ? 1
                       : static_cast<int>(P.arena[Lr.qual_off + j]) - 33;
     }
@@ -592,31 +561,31 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     // 1. included node set
     const int nn = s.n_nodes;
     for (int v = lane; v < nn; v += kWave) {
-      s.in_sub[v] = full ? 1 : 0;
+      s.in_sub()[v] = full ? 1 : 0;
     }
     wave_sync();
     if (!full) {
       int top = 0;
-      s.stack[top++] = static_cast<int>(Lr.end);
+      s.stack()[top++] = static_cast<int>(Lr.end);
       while (top > 0) {
-        const int v = uniform(s.stack[--top]);
-        if (s.in_sub[v] == 0 && v >= static_cast<int>(Lr.begin)) {
-          const int na = s.n_aligned[v];
+        const int v = uniform(s.stack()[--top]);
+        if (s.in_sub()[v] == 0 && v >= static_cast<int>(Lr.begin)) {
+          const int na = s.n_aligned()[v];
           int deg = na;
-          for (int e = uniform(s.in_head[v]); e != -1; e = uniform(s.e_next_in[e])) {
+          for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
             ++deg;
           }
           if (top + deg > static_cast<int>(caps.stack)) {
             R.status = kPoaNodeOverflow;
             return;
           }
-          for (int e = uniform(s.in_head[v]); e != -1; e = uniform(s.e_next_in[e])) {
-            s.stack[top++] = s.e_tail[e];
+          for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+            s.stack()[top++] = s.e_tail()[e];
           }
           for (int k = 0; k < na; ++k) {
-            s.stack[top++] = s.aligned[v * kMaxAligned + k];
+            s.stack()[top++] = s.aligned()[v * kMaxAligned + k];
           }
-          s.in_sub[v] = 1;
+          s.in_sub()[v] = 1;
         }
       }
     }
@@ -630,16 +599,16 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     if (full) {
       V = nn;
       for (int r = lane; r < V; r += kWave) {
-        s.sub_order[r] = s.rank_to_node[r];
+        s.sub_order()[r] = s.rank_to_node()[r];
       }
     } else {
       for (int r0 = 0; r0 < nn; r0 += kWave) {
         const int r = r0 + lane;
-        const int v = r < nn ? s.rank_to_node[r] : 0;
-        const bool keep = r < nn && s.in_sub[v] != 0;
+        const int v = r < nn ? s.rank_to_node()[r] : 0;
+        const bool keep = r < nn && s.in_sub()[v] != 0;
         const unsigned long long mask = __ballot(keep);
         if (keep) {
-          s.sub_order[V + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+          s.sub_order()[V + __popcll(mask & ((1ull << lane) - 1ull))] = v;
         }
         V += __popcll(mask);
       }
@@ -660,7 +629,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
 
     // DP row of every included node; predecessor rows, sink and far flags
     for (int r = lane; r < V; r += kWave) {
-      s.row_of[s.sub_order[r]] = r + 1;
+      s.row_of()[s.sub_order()[r]] = r + 1;
     }
     wave_sync();
     // predecessor counts per row (lane-parallel), then offsets
@@ -670,9 +639,9 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
         const int r = r0 + lane;
         int cnt = 0;
         if (r < V) {
-          const int v = s.sub_order[r];
-          for (int e = s.in_head[v]; e != -1; e = s.e_next_in[e]) {
-            if (s.in_sub[s.e_tail[e]] != 0) {
+          const int v = s.sub_order()[r];
+          for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
+            if (s.in_sub()[s.e_tail()[e]] != 0) {
               ++cnt;
             }
           }
@@ -681,14 +650,14 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           }
           int out_cnt = 0;
           int last_use = 0;
-          for (int e = s.out_head[v]; e != -1; e = s.e_next_out[e]) {
-            const int h = s.e_head[e];
-            if (s.in_sub[h] != 0) {
+          for (int e = s.out_head()[v]; e != -1; e = s.e_next_out()[e]) {
+            const int h = s.e_head()[e];
+            if (s.in_sub()[h] != 0) {
               ++out_cnt;
-              last_use = max(last_use, s.row_of[h]);
+              last_use = max(last_use, s.row_of()[h]);
             }
           }
-          s.row_meta[r] = static_cast<int>(s.base[v]) | (out_cnt == 0 ? kRowMetaSink : 0) |
+          s.row_meta()[r] = static_cast<int>(s.base()[v]) | (out_cnt == 0 ? kRowMetaSink : 0) |
                           (last_use - (r + 1) >= ring ? kRowMetaFar : 0);
         }
         // inclusive scan of cnt
@@ -698,27 +667,27 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           if (lane >= d) x += y;
         }
         if (r < V) {
-          s.pred_off[r] = run + x - cnt;
+          s.pred_off()[r] = run + x - cnt;
         }
         run += __shfl(x, kWave - 1, kWave);
       }
       if (lane == 0) {
-        s.pred_off[V] = run;
+        s.pred_off()[V] = run;
       }
       wave_sync();
       for (int r = lane; r < V; r += kWave) {
-        const int v = s.sub_order[r];
-        int o = s.pred_off[r];
+        const int v = s.sub_order()[r];
+        int o = s.pred_off()[r];
         bool any = false;
-        for (int e = s.in_head[v]; e != -1; e = s.e_next_in[e]) {
-          const int u = s.e_tail[e];
-          if (s.in_sub[u] != 0) {
-            s.pred_rows[o++] = s.row_of[u];
+        for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
+          const int u = s.e_tail()[e];
+          if (s.in_sub()[u] != 0) {
+            s.pred_rows()[o++] = s.row_of()[u];
             any = true;
           }
         }
         if (!any) {
-          s.pred_rows[o] = 0;
+          s.pred_rows()[o] = 0;
         }
       }
     }
@@ -735,7 +704,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     if (tied) {
       // several sinks share the best score: spoa takes the first one in its
       // DFS order of the (sub)graph
-      const int cnt = topo_sort(s, full ? nullptr : s.in_sub, s.path,
+      const int cnt = topo_sort(s, full ? nullptr : s.in_sub(), s.path(),
                                 static_cast<int>(caps.stack));
       if (s.overflow) {
         R.status = kPoaNodeOverflow;
@@ -748,8 +717,8 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
         bool hit = false;
         int row = 0;
         if (i < cnt) {
-          row = s.row_of[s.path[i]];
-          hit = (s.row_meta[row - 1] & kRowMetaSink) != 0 && s.sink_score[row] == best_score;
+          row = s.row_of()[s.path()[i]];
+          hit = (s.row_meta()[row - 1] & kRowMetaSink) != 0 && s.sink_score()[row] == best_score;
         }
         const unsigned long long mask = __ballot(hit);
         if (mask != 0) {
@@ -772,30 +741,30 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           return;
         }
         if (i == 0) {
-          s.aln_node[na] = -1;
-          s.aln_pos[na] = j - 1;
+          s.aln_node()[na] = -1;
+          s.aln_pos()[na] = j - 1;
           ++na;
           --j;
           continue;
         }
-        const int c = uniform(s.code[static_cast<int64_t>(i) * pitch + j]);
+        const int c = uniform(s.code()[static_cast<int64_t>(i) * pitch + j]);
         const int kind = c & 0xc000;
         const int k = c & 0x3fff;
-        const int node = uniform(s.sub_order[i - 1]);
+        const int node = uniform(s.sub_order()[i - 1]);
         if (kind == kCodeDiag) {
-          s.aln_node[na] = node;
-          s.aln_pos[na] = j - 1;
+          s.aln_node()[na] = node;
+          s.aln_pos()[na] = j - 1;
           ++na;
-          i = uniform(s.pred_rows[s.pred_off[i - 1] + k]);
+          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]);
           --j;
         } else if (kind == kCodeVert) {
-          s.aln_node[na] = node;
-          s.aln_pos[na] = -1;
+          s.aln_node()[na] = node;
+          s.aln_pos()[na] = -1;
           ++na;
-          i = uniform(s.pred_rows[s.pred_off[i - 1] + k]);
+          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]);
         } else {
-          s.aln_node[na] = -1;
-          s.aln_pos[na] = j - 1;
+          s.aln_node()[na] = -1;
+          s.aln_pos()[na] = j - 1;
           ++na;
           --j;
         }
@@ -810,14 +779,14 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     {
       int first_pos = -1, last_pos = -1;
       for (int q = na - 1; q >= 0; --q) {
-        if (s.aln_pos[q] != -1) {
-          first_pos = s.aln_pos[q];
+        if (s.aln_pos()[q] != -1) {
+          first_pos = s.aln_pos()[q];
           break;
         }
       }
       for (int q = 0; q < na; ++q) {
-        if (s.aln_pos[q] != -1) {
-          last_pos = s.aln_pos[q];
+        if (s.aln_pos()[q] != -1) {
+          last_pos = s.aln_pos()[q];
           break;
         }
       }
@@ -828,26 +797,26 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
       if (s.n_nodes != before) {
         order_ok = false;  // chains outside the alignment (not produced by NW)
       }
-      int prev_w = head == -1 ? 0 : s.qual[first_pos - 1];
+      int prev_w = head == -1 ? 0 : s.qual()[first_pos - 1];
       int head_anchor = -1;  // anchor of the previous path node when it is new
       for (int q = na - 1; q >= 0 && !s.overflow; --q) {
-        const int pos = uniform(s.aln_pos[q]);
+        const int pos = uniform(s.aln_pos()[q]);
         if (pos == -1) {
           continue;
         }
-        const int node = uniform(s.aln_node[q]);
-        const uint8_t letter = s.seq[pos];
+        const int node = uniform(s.aln_node()[q]);
+        const uint8_t letter = s.seq()[pos];
         int nv;
         if (node == -1) {
           nv = add_node(s, caps, letter);
-        } else if (s.base[node] == letter) {
+        } else if (s.base()[node] == letter) {
           nv = node;
         } else {
           int same = -1;
-          const int cnt = s.n_aligned[node];
+          const int cnt = s.n_aligned()[node];
           for (int k = 0; k < cnt; ++k) {
-            const int a = uniform(s.aligned[node * kMaxAligned + k]);
-            if (s.base[a] == letter) {
+            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+            if (s.base()[a] == letter) {
               same = a;
               break;
             }
@@ -859,16 +828,16 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
               break;
             }
             for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned[node * kMaxAligned + k]);
-              const int ca = s.n_aligned[a];
-              s.aligned[nv * kMaxAligned + k] = a;
-              s.aligned[a * kMaxAligned + ca] = nv;
-              s.n_aligned[a] = static_cast<uint8_t>(ca + 1);
+              const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+              const int ca = s.n_aligned()[a];
+              s.aligned()[nv * kMaxAligned + k] = a;
+              s.aligned()[a * kMaxAligned + ca] = nv;
+              s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
             }
-            s.aligned[nv * kMaxAligned + cnt] = node;
-            s.n_aligned[nv] = static_cast<uint8_t>(cnt + 1);
-            s.aligned[node * kMaxAligned + cnt] = nv;
-            s.n_aligned[node] = static_cast<uint8_t>(cnt + 1);
+            s.aligned()[nv * kMaxAligned + cnt] = node;
+            s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
+            s.aligned()[node * kMaxAligned + cnt] = nv;
+            s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
           } else {
             nv = same;
           }
@@ -878,40 +847,40 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           // (mismatch nodes: after the aligned set they join)
           int anchor;
           if (node != -1) {
-            anchor = s.node_rank[node];
-            const int cnt = s.n_aligned[node];
+            anchor = s.node_rank()[node];
+            const int cnt = s.n_aligned()[node];
             for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned[node * kMaxAligned + k]);
-              if (a < before) anchor = max(anchor, s.node_rank[a]);
+              const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+              if (a < before) anchor = max(anchor, s.node_rank()[a]);
             }
           } else if (head == -1) {
             anchor = -1;
           } else if (head >= before) {
             anchor = head_anchor;
           } else {
-            anchor = s.node_rank[head];
-            const int cnt = s.n_aligned[head];
+            anchor = s.node_rank()[head];
+            const int cnt = s.n_aligned()[head];
             for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned[head * kMaxAligned + k]);
-              if (a < before) anchor = max(anchor, s.node_rank[a]);
+              const int a = uniform(s.aligned()[head * kMaxAligned + k]);
+              if (a < before) anchor = max(anchor, s.node_rank()[a]);
             }
           }
           const int k_new = nv - before;
-          if (k_new > 0 && anchor < s.anchor[k_new - 1]) {
+          if (k_new > 0 && anchor < s.anchor()[k_new - 1]) {
             order_ok = false;
           }
-          s.anchor[k_new] = anchor;
+          s.anchor()[k_new] = anchor;
           head_anchor = anchor;
         }
-        s.lpath[n_path++] = nv;
+        s.lpath()[n_path++] = nv;
         if (head != -1) {
-          add_edge(s, caps, head, nv, prev_w + s.qual[pos]);
+          add_edge(s, caps, head, nv, prev_w + s.qual()[pos]);
         }
         head = nv;
-        prev_w = s.qual[pos];
+        prev_w = s.qual()[pos];
       }
       if (tail != -1 && !s.overflow) {
-        add_edge(s, caps, head, tail, prev_w + s.qual[last_pos + 1]);
+        add_edge(s, caps, head, tail, prev_w + s.qual()[last_pos + 1]);
       }
       s.n_seqs = s.n_seqs + 1;
     }
@@ -933,37 +902,37 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           int lo = 0, hi = M;
           while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (s.anchor[mid] < r) lo = mid + 1; else hi = mid;
+            if (s.anchor()[mid] < r) lo = mid + 1; else hi = mid;
           }
-          s.sub_order[r + lo] = s.rank_to_node[r];
+          s.sub_order()[r + lo] = s.rank_to_node()[r];
         }
         for (int k = lane; k < M; k += kWave) {
-          s.sub_order[s.anchor[k] + 1 + k] = nodes_before + k;
+          s.sub_order()[s.anchor()[k] + 1 + k] = nodes_before + k;
         }
         wave_sync();
         for (int r = lane; r < total; r += kWave) {
-          const int v = s.sub_order[r];
-          s.rank_to_node[r] = v;
-          s.node_rank[v] = r;
+          const int v = s.sub_order()[r];
+          s.rank_to_node()[r] = v;
+          s.node_rank()[v] = r;
         }
         wave_sync();
       }
       bool bad = !order_ok;
       if (!bad) {
         for (int i = lane; i + 1 < n_path; i += kWave) {
-          if (s.node_rank[s.lpath[i]] >= s.node_rank[s.lpath[i + 1]]) bad = true;
+          if (s.node_rank()[s.lpath()[i]] >= s.node_rank()[s.lpath()[i + 1]]) bad = true;
         }
         bad = __ballot(bad) != 0;
       }
       if (bad) {
-        topo_sort(s, nullptr, s.rank_to_node, static_cast<int>(caps.stack));
+        topo_sort(s, nullptr, s.rank_to_node(), static_cast<int>(caps.stack));
         if (s.overflow) {
           R.status = kPoaNodeOverflow;
           return;
         }
         wave_sync();
         for (int r = lane; r < total; r += kWave) {
-          s.node_rank[s.rank_to_node[r]] = r;
+          s.node_rank()[s.rank_to_node()[r]] = r;
         }
       }
     }
@@ -973,7 +942,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
 
   // spoa's exact DFS order of the final graph for the heaviest bundle
   if (W.n_layers > 0) {
-    topo_sort(s, nullptr, s.rank_to_node, static_cast<int>(caps.stack));
+    topo_sort(s, nullptr, s.rank_to_node(), static_cast<int>(caps.stack));
     if (s.overflow) {
       R.status = kPoaNodeOverflow;
       return;
@@ -985,80 +954,80 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
   // heaviest bundle (lane-uniform)
   const int N = s.n_nodes;
   for (int v = lane; v < N; v += kWave) {
-    s.scores[v] = -1;
-    s.preds[v] = -1;
+    s.scores()[v] = -1;
+    s.preds()[v] = -1;
   }
   wave_sync();
   int max_node = 0;
   for (int r = 0; r < N; ++r) {
-    const int v = uniform(s.rank_to_node[r]);
-    for (int e = uniform(s.in_head[v]); e != -1; e = uniform(s.e_next_in[e])) {
-      const int t = uniform(s.e_tail[e]);
-      const int64_t w = s.e_weight[e];
-      const int64_t sv = s.scores[v];
-      const int pv = s.preds[v];
-      if (sv < w || (sv == w && pv != -1 && s.scores[pv] <= s.scores[t])) {
-        s.scores[v] = w;
-        s.preds[v] = t;
+    const int v = uniform(s.rank_to_node()[r]);
+    for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+      const int t = uniform(s.e_tail()[e]);
+      const int64_t w = s.e_weight()[e];
+      const int64_t sv = s.scores()[v];
+      const int pv = s.preds()[v];
+      if (sv < w || (sv == w && pv != -1 && s.scores()[pv] <= s.scores()[t])) {
+        s.scores()[v] = w;
+        s.preds()[v] = t;
       }
     }
-    const int pv = s.preds[v];
+    const int pv = s.preds()[v];
     if (pv != -1) {
-      s.scores[v] = s.scores[v] + s.scores[pv];
+      s.scores()[v] = s.scores()[v] + s.scores()[pv];
     }
-    if (s.scores[max_node] < s.scores[v]) {
+    if (s.scores()[max_node] < s.scores()[v]) {
       max_node = v;
     }
   }
-  if (s.out_head[max_node] != -1) {
+  if (s.out_head()[max_node] != -1) {
     for (int r = lane; r < N; r += kWave) {
-      s.node_rank[s.rank_to_node[r]] = r;
+      s.node_rank()[s.rank_to_node()[r]] = r;
     }
     wave_sync();
     int guard = 0;
-    while (s.out_head[max_node] != -1) {
+    while (s.out_head()[max_node] != -1) {
       if (++guard > N + 1) {
         R.status = kPoaLoopLimit;
         return;
       }
       // branch completion from the rank of max_node
-      const int rank = s.node_rank[max_node];
+      const int rank = s.node_rank()[max_node];
       const int v0 = max_node;
-      for (int e = uniform(s.out_head[v0]); e != -1; e = uniform(s.e_next_out[e])) {
-        const int h = uniform(s.e_head[e]);
-        for (int f = uniform(s.in_head[h]); f != -1; f = uniform(s.e_next_in[f])) {
-          const int t = uniform(s.e_tail[f]);
+      for (int e = uniform(s.out_head()[v0]); e != -1; e = uniform(s.e_next_out()[e])) {
+        const int h = uniform(s.e_head()[e]);
+        for (int f = uniform(s.in_head()[h]); f != -1; f = uniform(s.e_next_in()[f])) {
+          const int t = uniform(s.e_tail()[f]);
           if (t != v0) {
-            s.scores[t] = -1;
+            s.scores()[t] = -1;
           }
         }
       }
       int64_t best_score = 0;
       int best_node = 0;
       for (int r = rank + 1; r < N; ++r) {
-        const int u = uniform(s.rank_to_node[r]);
-        s.scores[u] = -1;
-        s.preds[u] = -1;
-        for (int e = uniform(s.in_head[u]); e != -1; e = uniform(s.e_next_in[e])) {
-          const int t = uniform(s.e_tail[e]);
-          const int64_t st = s.scores[t];
+        const int u = uniform(s.rank_to_node()[r]);
+        s.scores()[u] = -1;
+        s.preds()[u] = -1;
+        for (int e = uniform(s.in_head()[u]); e != -1; e = uniform(s.e_next_in()[e])) {
+          const int t = uniform(s.e_tail()[e]);
+          const int64_t st = s.scores()[t];
           if (st == -1) {
             continue;
           }
-          const int64_t w = s.e_weight[e];
-          const int64_t su = s.scores[u];
-          const int pu = s.preds[u];
-          if (su < w || (su == w && pu != -1 && s.scores[pu] <= st)) {
-            s.scores[u] = w;
-            s.preds[u] = t;
+          const int64_t w = s.e_weight()[e];
+          const int64_t su = s.scores()[u];
+          const int pu = s.preds()[u];
+          if (su < w || (su == w && pu != -1 && s.scores()[pu] <= st)) {
+            s.scores()[u] = w;
+            s.preds()[u] = t;
           }
         }
-        const int pu = s.preds[u];
+        const int pu = s.preds()[u];
         if (pu != -1) {
-          s.scores[u] = s.scores[u] + s.scores[pu];
+          s.scores()[u] = s.scores()[u] + s.scores()[pu];
         }
-        if (best_score < s.scores[u]) {
-          best_score = s.scores[u];
+        if (best_score < s.scores()[u]) {
+          best_score = s.scores()[u];
           best_node = u;
         }
       }
@@ -1070,8 +1039,8 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
   {
     int v = max_node;
     while (true) {
-      s.path[plen++] = v;
-      const int p = uniform(s.preds[v]);
+      s.path()[plen++] = v;
+      const int p = uniform(s.preds()[v]);
       if (p == -1 || plen >= N) {
         break;
       }
@@ -1085,11 +1054,11 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
   if ((W.flags & 3u) == 3u) {
     const int threshold = static_cast<int>(W.n_layers) / 2;  // (n_seqs - 1) / 2
     auto coverage = [&](int idx) -> int {
-      const int v = s.path[plen - 1 - idx];
-      int c = s.cov[v];
-      const int na2 = s.n_aligned[v];
+      const int v = s.path()[plen - 1 - idx];
+      int c = s.cov()[v];
+      const int na2 = s.n_aligned()[v];
       for (int k = 0; k < na2; ++k) {
-        c += s.cov[s.aligned[v * kMaxAligned + k]];
+        c += s.cov()[s.aligned()[v * kMaxAligned + k]];
       }
       return c;
     };
@@ -1111,7 +1080,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     return;
   }
   for (int i = lane; i < out_len; i += kWave) {
-    P.out[W.out_off + i] = s.base[s.path[plen - 1 - (begin + i)]];
+    P.out[W.out_off + i] = s.base()[s.path()[plen - 1 - (begin + i)]];
   }
   clk.lap(R, kPhaseConsensus);
   R.length = static_cast<uint32_t>(out_len);
