@@ -83,6 +83,7 @@ def main() -> int:
                     help="budget of the single-core CPU baseline sample")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +111,20 @@ def main() -> int:
     up = batch.stats()
     log(f"uploaded {up.input_bytes / 1e6:.1f} MB (pack {up.pack_ms:.0f} ms, "
         f"H2D {up.h2d_ms:.0f} ms)")
+    # autotune the LDS ring depth (DP rows kept on chip per wave) on the
+    # resident batch; 0 = the occupancy-derived default
+    tuned = {}
+    if not args.no_tune:
+        for ring in (0, 4, 8):
+            batch.set_tuning(ring_rows=ring)
+            t0 = time.perf_counter()
+            batch.run(racon_amd.RH_POA_BATCHED)
+            tuned[ring] = (time.perf_counter() - t0) * 1e3
+            log(f"tune ring_rows={ring or 'auto'}: {tuned[ring]:.1f} ms")
+        best_ring = min(tuned, key=tuned.get)
+        batch.set_tuning(ring_rows=best_ring)
+    else:
+        best_ring = 0
     for i in range(args.warmup):
         batch.run(racon_amd.RH_POA_BATCHED)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -221,6 +236,8 @@ def main() -> int:
                                         / VALU_PEAK_TOPS, 5)},
         "cpu_baseline": cpu,
         "poa_phase_share": {k: round(v / total_phase, 4) for k, v in phases.items()},
+        "autotune_ms": {str(k): round(v, 1) for k, v in tuned.items()},
+        "ring_rows": best_ring,
     }
     print(json.dumps(result), flush=True)
     return 0

@@ -126,9 +126,12 @@ int32_t rh_poa_download(rh_poa_batch* batch, char* out, uint64_t capacity,
 
 int32_t rh_poa_get_stats(const rh_poa_batch* batch, rh_poa_stats* stats);
 
-/* Autotuning knobs (0 = default): resident slots of the persistent kernel
- * and the DP row capacity of a default slot. */
-int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows);
+/* Autotuning knobs (0 = default): resident slots of the persistent kernel,
+ * the DP row capacity of a default slot, and the DP rows kept in LDS per
+ * wave (2..16; the default fits 16 waves per CU). No reference counterpart
+ * (GenomeWorks fixes its batch shape in BatchConfig, cudabatch.cpp:56-59). */
+int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows,
+                          uint32_t ring_rows);
 
 /* Profiling: device clock ticks of the last run summed over windows, per
  * kernel phase (subgraph set, subgraph order, DP row map, DP column 0, DP

@@ -101,7 +101,7 @@ _SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_uint8)]),
     "rh_poa_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaStats)]),
     "rh_poa_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
-                                           ctypes.c_uint32]),
+                                           ctypes.c_uint32, ctypes.c_uint32]),
     "rh_poa_phase_ticks": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "rh_align_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
@@ -183,8 +183,9 @@ class PoaBatch:
         except Exception:
             pass
 
-    def set_tuning(self, slots: int = 0, rows: int = 0) -> None:
-        _check(self.lib.rh_poa_set_tuning(self.handle, slots, rows), "rh_poa_set_tuning")
+    def set_tuning(self, slots: int = 0, rows: int = 0, ring_rows: int = 0) -> None:
+        _check(self.lib.rh_poa_set_tuning(self.handle, slots, rows, ring_rows),
+               "rh_poa_set_tuning")
 
     def upload(self, windows: Sequence[Window]) -> None:
         arr = (RhWindow * len(windows))()

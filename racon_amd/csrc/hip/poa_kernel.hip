@@ -279,16 +279,25 @@ __device__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* out, int stack_c
 
 // ---- DP ----
 
+// Inclusive prefix max over the 64 lanes with DPP row shifts and row
+// broadcasts (no LDS crossbar traffic).
+
 __device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    int32_t y = __shfl_up(x, d, kWave);
-    if (lane >= d) {
-      x = max(x, y);
-    }
-  }
+  // within rows of 16 lanes: shifts 1, 2, 4, 8 (lanes shifted in from
+  // outside the row keep the identity)
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x118, 0xf, 0xf, false));
+  // across rows: lane 15 into rows 1 and 3, lane 31 into rows 2 and 3
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x143, 0xc, 0xf, false));
   return x;
+}
+
+// Value of the previous lane (lane 0 gets `first`), via DPP wave shift.
+__device__ __forceinline__ int32_t wave_shift_right(int32_t x, int32_t first) {
+  return __builtin_amdgcn_update_dpp(first, x, 0x138, 0xf, 0xf, false);
 }
 
 // Wavefront-scope ordering of LDS/global traffic between lanes of this wave
@@ -330,10 +339,32 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   int32_t best = kNegInf;
   int best_row = 0;
   int ties = 0;
-  for (int r = 1; r <= V; ++r) {
-    const int off = uniform(s.pred_off()[r - 1]);
-    const int np = uniform(s.pred_off()[r]) - off;
-    const int meta = uniform(s.row_meta()[r - 1]);
+  const int total_preds = uniform(s.pred_off()[V]);
+  // rows are processed in chunks of 64: lane k holds the metadata of row
+  // r0 + k and a 128-entry window of the predecessor list, broadcast per row
+  for (int r0 = 1; r0 <= V; r0 += kWave) {
+  const int rk = r0 + lane;
+  int off_v = 0, end_v = 0, meta_v = 0;
+  if (rk <= V) {
+    off_v = s.pred_off()[rk - 1];
+    end_v = s.pred_off()[rk];
+    meta_v = s.row_meta()[rk - 1];
+  }
+  const int base = __builtin_amdgcn_readlane(off_v, 0);
+  const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
+  const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+  const int chunk = min(kWave, V - r0 + 1);
+  for (int rr = 0; rr < chunk; ++rr) {
+    const int r = r0 + rr;
+    const int off = __builtin_amdgcn_readlane(off_v, rr);
+    const int np = __builtin_amdgcn_readlane(end_v, rr) - off;
+    const int meta = __builtin_amdgcn_readlane(meta_v, rr);
+    auto pred_at = [&](int k) -> int {
+      const int idx = off + k - base;
+      if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
+      if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
+      return uniform(s.pred_rows()[off + k]);
+    };
     const uint8_t b = static_cast<uint8_t>(meta & 0xff);
     T* Hr = ring + (r % R) * pitch;
     int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
@@ -344,7 +375,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
     int32_t c0 = kNegInf;
     int k0 = 0;
     for (int k = 0; k < np; ++k) {
-      const int p = uniform(s.pred_rows()[off + k]);
+      const int p = pred_at(k);
       int32_t h0;
       if (p == 0) {
         h0 = 0;
@@ -383,7 +414,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
         kk[u] = 0;
       }
       for (int k = 0; k < np; ++k) {
-        const int p = uniform(s.pred_rows()[off + k]);
+        const int p = pred_at(k);
         int32_t hp[kStrip + 1];
         if (p == 0) {
 #pragma unroll
@@ -429,10 +460,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
         pre[u] = pm;
       }
       const int32_t incl = wave_prefix_max(pm);
-      int32_t excl = __shfl_up(incl, 1, kWave);
-      if (lane == 0) {
-        excl = kNegInf;
-      }
+      const int32_t excl = wave_shift_right(incl, kNegInf);
       const int32_t in_carry = max(excl, carry - (t0 - 1) * gap);
       int32_t h[kStrip];
 #pragma unroll
@@ -466,7 +494,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       for (int u = 1; u < kStrip; ++u) {
         if (u == last_u) lv = h[u];
       }
-      carry = __shfl(lv, owner, kWave);
+      carry = __builtin_amdgcn_readlane(lv, owner);
       h_last = carry;
     }
     if ((meta & kRowMetaSink) != 0) {
@@ -485,6 +513,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       wait_stores();
     }
     wave_fence();
+  }
   }
   *best_score = best;
   *tied = ties > 0;
@@ -1093,7 +1122,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
 }  // namespace
 
 // Persistent batch kernel: each wave takes windows from the shared queue.
-__global__ __launch_bounds__(64) void poa_batch_kernel(PoaParams P) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void poa_batch_kernel(PoaParams P) {
   extern __shared__ uint8_t lds[];
   uint8_t* slot_mem = P.scratch + static_cast<uint64_t>(blockIdx.x) * P.slot_bytes;
   __shared__ uint32_t next;
@@ -1121,7 +1150,7 @@ __global__ __launch_bounds__(64) void poa_batch_kernel(PoaParams P) {
 }
 
 // Plain kernel: one launch per window (single wave, window index given).
-__global__ __launch_bounds__(64) void poa_single_kernel(PoaParams P, uint32_t q) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void poa_single_kernel(PoaParams P, uint32_t q) {
   extern __shared__ uint8_t lds[];
   const uint32_t w = P.order[q];
   Slot s = carve(P.scratch, P.layout);

@@ -39,6 +39,7 @@ struct rh_poa_batch {
   bool trim = true;
   uint32_t tune_slots = 0;
   uint32_t tune_rows = 0;
+  uint32_t tune_ring = 0;
 
   // host copies of the packed batch
   std::vector<uint8_t> arena;
@@ -83,13 +84,24 @@ uint64_t window_cost(const PoaWindow& w) {
   return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
 }
 
-// DP rows kept in LDS: 8 rows of 16-bit scores when they fit in 56 KiB.
-uint32_t ring_rows_for(uint32_t cols) {
-  uint32_t rows = 8;
-  while (rows > 2 && rh::poa_lds_bytes(cols, rows) > 56u * 1024u) {
-    rows -= 2;
+// DP rows kept in LDS (16-bit scores): as many as fit while 16 waves share a
+// CU's 160 KiB (12 or 8 waves for long layers), at most 8; a tuned value
+// overrides it.
+uint32_t ring_rows_for(uint32_t cols, uint32_t tuned) {
+  if (tuned != 0) {
+    return std::max<uint32_t>(2, std::min<uint32_t>(16, tuned));
   }
-  return rows;
+  const uint64_t lds_per_cu = 160u * 1024u;
+  for (uint32_t waves : {16u, 12u, 8u}) {
+    const uint64_t per_wave = lds_per_cu / waves;
+    const uint64_t fixed = rh::align_up(cols, 16);
+    if (per_wave <= fixed) continue;
+    const uint64_t rows = (per_wave - fixed) / (2ull * cols);
+    if (rows >= 4 || waves == 8u) {
+      return static_cast<uint32_t>(std::max<uint64_t>(2, std::min<uint64_t>(8, rows)));
+    }
+  }
+  return 2;
 }
 
 int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaSlotCaps& caps,
@@ -135,7 +147,7 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   p.mismatch = b->mismatch;
   p.gap = b->gap;
   p.n_slots = static_cast<uint32_t>(slots);
-  p.ring_rows = ring_rows_for(caps.cols);
+  p.ring_rows = ring_rows_for(caps.cols, b->tune_ring);
   p.lds_bytes = rh::poa_lds_bytes(caps.cols, p.ring_rows);
 
   RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
@@ -227,13 +239,14 @@ void rh_poa_destroy(rh_poa_batch* b) {
   delete b;
 }
 
-int32_t rh_poa_set_tuning(rh_poa_batch* b, uint32_t slots, uint32_t rows) {
+int32_t rh_poa_set_tuning(rh_poa_batch* b, uint32_t slots, uint32_t rows, uint32_t ring_rows) {
   if (b == nullptr) {
     rh::set_error("rh_poa_set_tuning: null batch");
     return RH_ERR_ARGUMENT;
   }
   b->tune_slots = slots;
   b->tune_rows = rows;
+  b->tune_ring = ring_rows;
   return RH_OK;
 }
 
@@ -424,13 +437,13 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   caps.rows = std::min<uint32_t>(max_nodes + 1, b->tune_rows ? b->tune_rows : 4096);
   caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
 
-  if (rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols)) > 64u * 1024u) {
+  const uint32_t lds = rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols, b->tune_ring));
+  if (lds > 64u * 1024u) {
     rh::set_error("rh_poa_run: layers longer than the LDS ring allows");
     return RH_ERR_ARGUMENT;
   }
   int per_cu = 0, cus = 0;
-  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols))),
-               RH_ERR_DEVICE);
+  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, lds), RH_ERR_DEVICE);
   RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
                RH_ERR_DEVICE);
   uint32_t resident = static_cast<uint32_t>(std::max(1, per_cu) * std::max(1, cus));

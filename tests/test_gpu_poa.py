@@ -76,3 +76,42 @@ def test_long_layers_two_tiles():
     # layers longer than one 512-column tile
     wins = make_windows(20, 3, length=1000, depth=12)
     check(wins, (5, -4, -8))
+
+
+@pytest.mark.parametrize("ring_rows", [2, 4, 16])
+def test_lds_ring_depths(ring_rows):
+    # a shallow ring sends most predecessor rows through the slot's global H
+    wins = make_windows(21, 6, length=400, depth=14)
+    b = _lib.PoaBatch(3, -5, -4, trim=True)
+    b.set_tuning(ring_rows=ring_rows)
+    b.upload(wins)
+    b.run()
+    out, status = b.download()
+    b.close()
+    for i, w in enumerate(wins):
+        bb, bq, layers, tgs = oracle_args(w)
+        cons, ost, _, _ = oracle_py.window_consensus(bb, bq, layers, tgs, True, 3, -5, -4)
+        assert (out[i], status[i]) == (cons, ost), f"window {i}"
+
+
+def test_wide_scores_use_32bit_rows():
+    # max(|m|,|x|,|g|) * (V + L) beyond 16 bits: the 32-bit ring path
+    wins = make_windows(22, 4, length=500, depth=20)
+    check(wins, (30, -40, -40))
+
+
+def test_small_dp_row_capacity_reruns():
+    # a DP row cap below the subgraph size forces the rerun in large slots
+    wins = make_windows(23, 5, length=300, depth=10)
+    b = _lib.PoaBatch(3, -5, -4, trim=True)
+    b.set_tuning(rows=64)
+    b.upload(wins)
+    b.run()
+    out, status = b.download()
+    st = b.stats()
+    b.close()
+    assert st.retried == len(wins)
+    for i, w in enumerate(wins):
+        bb, bq, layers, tgs = oracle_args(w)
+        cons, ost, _, _ = oracle_py.window_consensus(bb, bq, layers, tgs, True, 3, -5, -4)
+        assert (out[i], status[i]) == (cons, ost), f"window {i}"
