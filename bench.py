@@ -84,6 +84,12 @@ def main() -> int:
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--traffic-bytes", type=float, default=0.0,
+                    help="HBM bytes per POA launch measured with rocprofv3 (FETCH_SIZE x2 + "
+                         "WRITE_SIZE, copy-calibrated) to report in roofline.traffic")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (torch), to scale "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE readings, then exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -91,6 +97,17 @@ def main() -> int:
                                    "windows across devices (see DESIGN.md), but no multi-GPU "
                                    "figure is produced", "n_gpus": args.gpus}))
         return 2
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "reads": n * 4, "writes": n * 4}))
+        return 0
 
     import racon_amd
     from racon_amd.dumps import read_overlaps, read_windows
@@ -228,7 +245,7 @@ def main() -> int:
         "align_kernel_ms": round(ast.kernel_ms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "traffic": args.traffic_bytes or None},
         "roofline_valu": {"bound": "valu-int32",
                           "achieved": round(cells * POA_OPS_PER_CELL / (kernel_avg / 1e3) / 1e12, 4),
                           "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
