@@ -312,25 +312,76 @@ __device__ __forceinline__ void wave_fence() {
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 constexpr int kRowMetaSink = 0x100;
-constexpr int kRowMetaFar = 0x200;
+// Bits 10..14 of a row's meta word: where the row's scores live while later
+// rows need them. Slots 0..kMaxLdsSlots-1 are LDS rows; kSlotGlobal rows go
+// through the slot's global H matrix; kSlotNone rows are read by no later row.
+constexpr int kRowMetaSlotShift = 10;
+constexpr int kSlotGlobal = 30;
+constexpr int kSlotNone = 31;
+constexpr int kMaxLdsSlots = 30;
+// Predecessor list entries: DP row in the low 24 bits, its slot above.
+constexpr int kPredRowBits = 24;
+constexpr int kPredRowMask = (1 << kPredRowBits) - 1;
 
-// Value of predecessor row p at column j while row r is computed: row 0 is
-// the virtual start (j * gap); rows less than R back live in the LDS ring;
-// older rows were also written to the slot's global H matrix.
+// LDS score rows keep column j at element j + kColShift, so that the four
+// columns of a lane's strip (j0 = 1 + 4k) start on an aligned quad.
+constexpr int kColShift = 3;
+
+// Code matrix index of column j inside a row: columns 1..L at j - 1 (aligned
+// quads per strip), column 0 at the row's last element (pitch >= L + 9).
+__device__ __forceinline__ int code_index(int j, int pitch) { return j == 0 ? pitch - 1 : j - 1; }
+
 template <typename T>
-__device__ __forceinline__ const T* ring_row(const T* ring, int pitch, int R, int p) {
-  return ring + (p % R) * pitch;
-}
+struct Quad;
 
-// All DP rows of one layer. Lanes own 8-column strips of 512-column tiles;
-// predecessor rows come from the LDS ring (global H for rows >= R back);
-// the horizontal gap chain is a wave prefix-max; every cell stores the
-// first move that reproduces its score (diagonal, then vertical, each over
-// predecessors in in-edge order, then horizontal). Returns the DP row of the
-// end cell: the first sink row with the highest score in column L.
+template <>
+struct Quad<int16_t> {
+  static __device__ __forceinline__ void load(const int16_t* p, int32_t* v) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    v[0] = static_cast<int16_t>(q.x & 0xffffu);
+    v[1] = static_cast<int32_t>(q.x) >> 16;
+    v[2] = static_cast<int16_t>(q.y & 0xffffu);
+    v[3] = static_cast<int32_t>(q.y) >> 16;
+  }
+  static __device__ __forceinline__ void store(int16_t* p, const int32_t* v) {
+    uint2 q;
+    q.x = (static_cast<uint32_t>(v[0]) & 0xffffu) | (static_cast<uint32_t>(v[1]) << 16);
+    q.y = (static_cast<uint32_t>(v[2]) & 0xffffu) | (static_cast<uint32_t>(v[3]) << 16);
+    *reinterpret_cast<uint2*>(p) = q;
+  }
+};
+
+template <>
+struct Quad<int32_t> {
+  static __device__ __forceinline__ void load(const int32_t* p, int32_t* v) {
+    const int4 q = *reinterpret_cast<const int4*>(p);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+  }
+  static __device__ __forceinline__ void store(int32_t* p, const int32_t* v) {
+    *reinterpret_cast<int4*>(p) = make_int4(v[0], v[1], v[2], v[3]);
+  }
+};
+
+// All DP rows of one layer. Lanes own 4-column strips of 256-column tiles;
+// predecessor rows come from their LDS slots (the global H matrix for rows
+// that found no free slot); the horizontal gap chain is a wave prefix-max;
+// every cell stores the first move that reproduces its score (diagonal,
+// then vertical, each over predecessors in in-edge order, then horizontal).
+// Returns the DP row of the end cell: the first sink row with the highest
+// score in column L.
+//
+// The global-H branch waits for its own loads before leaving the branch, so
+// the common LDS path never waits on vector memory (on gfx9 the vector memory
+// counter also counts the traceback-code stores of earlier rows). Strips
+// that start past column L are idle; the columns past L inside the last
+// strip are computed from padding and never read back.
 template <typename T>
 __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, int L,
-                        int R, int* best_score, bool* tied) {
+                        int* best_score, bool* tied) {
+  static_assert(kStrip == 4, "strips are loaded and stored as quads");
   const int lane = lane_id();
   const int pitch = static_cast<int>(P.caps.cols);
   const int32_t gap = P.gap;
@@ -353,6 +404,9 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   const int base = __builtin_amdgcn_readlane(off_v, 0);
   const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
   const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+  // complete the chunk's loads here: otherwise the row loop below would wait
+  // for them (and for every code store before them) on each row
+  wait_stores();
   const int chunk = min(kWave, V - r0 + 1);
   for (int rr = 0; rr < chunk; ++rr) {
     const int r = r0 + rr;
@@ -363,26 +417,31 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       const int idx = off + k - base;
       if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
       if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
-      return uniform(s.pred_rows()[off + k]);
+      const int e = uniform(s.pred_rows()[off + k]);
+      wait_stores();
+      return e;
     };
-    const uint8_t b = static_cast<uint8_t>(meta & 0xff);
-    T* Hr = ring + (r % R) * pitch;
+    const uint32_t b = static_cast<uint32_t>(meta & 0xff);
+    const int own = (meta >> kRowMetaSlotShift) & 31;
+    T* Hr = ring + (own < kSlotGlobal ? own : 0) * pitch;
     int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
     uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
-    const bool far = (meta & kRowMetaFar) != 0;
 
     // column 0: max over predecessors of their column 0, plus gap
     int32_t c0 = kNegInf;
     int k0 = 0;
     for (int k = 0; k < np; ++k) {
-      const int p = pred_at(k);
+      const int e = pred_at(k);
+      const int p = e & kPredRowMask;
+      const int sl = static_cast<unsigned>(e) >> kPredRowBits;
       int32_t h0;
       if (p == 0) {
         h0 = 0;
-      } else if (r - p < R) {
-        h0 = ring_row(ring, pitch, R, p)[0];
+      } else if (sl < kSlotGlobal) {
+        h0 = ring[sl * pitch + kColShift];
       } else {
-        h0 = s.H()[static_cast<int64_t>(p) * pitch];
+        h0 = uniform(s.H()[static_cast<int64_t>(p) * pitch]);
+        wait_stores();
       }
       if (h0 + gap > c0) {
         c0 = h0 + gap;
@@ -390,20 +449,24 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       }
     }
     if (lane == 0) {
-      Hr[0] = static_cast<T>(c0);
-      Cr[0] = kCodeVert | static_cast<uint16_t>(k0);
-      if (far) Hg[0] = c0;
+      if (own < kSlotGlobal) Hr[kColShift] = static_cast<T>(c0);
+      if (own == kSlotGlobal) Hg[0] = c0;
+      Cr[pitch - 1] = kCodeVert | static_cast<uint16_t>(k0);
     }
     int32_t carry = c0;
     int32_t h_last = kNegInf;
 
     for (int t0 = 1; t0 <= L; t0 += kTile) {
       const int j0 = t0 + lane * kStrip;
+      const bool live = j0 <= L;
+      const int jr = live ? j0 : 1;  // idle strips read column 0..4 (in bounds)
+      const uint32_t w4 = *reinterpret_cast<const uint32_t*>(seq + (jr - 1));
       int32_t sc[kStrip];
+      bool in[kStrip];
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
-        const int j = j0 + u;
-        sc[u] = (j <= L && seq[j - 1] == b) ? P.match : P.mismatch;
+        sc[u] = ((w4 >> (8 * u)) & 0xffu) == b ? P.match : P.mismatch;
+        in[u] = j0 + u <= L;
       }
       int32_t bd[kStrip], bv[kStrip];
       uint32_t kk[kStrip];  // diagonal k (low 16 bits), vertical k (high 16 bits)
@@ -414,18 +477,17 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
         kk[u] = 0;
       }
       for (int k = 0; k < np; ++k) {
-        const int p = pred_at(k);
+        const int e = pred_at(k);
+        const int p = e & kPredRowMask;
+        const int sl = static_cast<unsigned>(e) >> kPredRowBits;
         int32_t hp[kStrip + 1];
         if (p == 0) {
 #pragma unroll
           for (int u = 0; u <= kStrip; ++u) hp[u] = (j0 - 1 + u) * gap;
-        } else if (r - p < R) {
-          const T* row = ring_row(ring, pitch, R, p);
-#pragma unroll
-          for (int u = 0; u <= kStrip; ++u) {
-            const int j = j0 - 1 + u;
-            hp[u] = j <= L ? static_cast<int32_t>(row[j]) : kNegInf;
-          }
+        } else if (sl < kSlotGlobal) {
+          const T* row = ring + sl * pitch + kColShift;
+          hp[0] = row[jr - 1];
+          Quad<T>::load(row + jr, hp + 1);
         } else {
           const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
 #pragma unroll
@@ -433,19 +495,18 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
             const int j = j0 - 1 + u;
             hp[u] = j <= L ? row[j] : kNegInf;
           }
+          wait_stores();
         }
 #pragma unroll
         for (int u = 0; u < kStrip; ++u) {
           const int32_t d = hp[u] + sc[u];
           const int32_t v = hp[u + 1] + gap;
-          if (d > bd[u]) {
-            bd[u] = d;
-            kk[u] = (kk[u] & 0xffff0000u) | static_cast<uint32_t>(k);
-          }
-          if (v > bv[u]) {
-            bv[u] = v;
-            kk[u] = (kk[u] & 0xffffu) | (static_cast<uint32_t>(k) << 16);
-          }
+          const bool dg = d > bd[u];
+          const bool vg = v > bv[u];
+          bd[u] = dg ? d : bd[u];
+          bv[u] = vg ? v : bv[u];
+          kk[u] = dg ? ((kk[u] & 0xffff0000u) | static_cast<uint32_t>(k)) : kk[u];
+          kk[u] = vg ? ((kk[u] & 0xffffu) | (static_cast<uint32_t>(k) << 16)) : kk[u];
         }
       }
       // horizontal chain: H[j] = j*g + max_{k<=j} (t_k - k*g)
@@ -455,7 +516,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       for (int u = 0; u < kStrip; ++u) {
         const int j = j0 + u;
         const int32_t t = max(bd[u], bv[u]);
-        pre[u] = j <= L ? t - j * gap : kNegInf;
+        pre[u] = in[u] ? t - j * gap : kNegInf;
         pm = max(pm, pre[u]);
         pre[u] = pm;
       }
@@ -463,25 +524,27 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       const int32_t excl = wave_shift_right(incl, kNegInf);
       const int32_t in_carry = max(excl, carry - (t0 - 1) * gap);
       int32_t h[kStrip];
+      uint32_t cd[kStrip];
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
         h[u] = max(pre[u], in_carry) + (j0 + u) * gap;
+        const uint32_t c_diag = kCodeDiag | (kk[u] & 0xffffu);
+        const uint32_t c_vert = kCodeVert | (kk[u] >> 16);
+        cd[u] = bd[u] == h[u] ? c_diag : (bv[u] == h[u] ? c_vert : kCodeHorz);
       }
+      if (live) {
+        uint2 q;
+        q.x = cd[0] | (cd[1] << 16);
+        q.y = cd[2] | (cd[3] << 16);
+        *reinterpret_cast<uint2*>(Cr + (j0 - 1)) = q;
+        if (own < kSlotGlobal) {
+          Quad<T>::store(Hr + kColShift + j0, h);
+        }
+      }
+      if (own == kSlotGlobal) {
 #pragma unroll
-      for (int u = 0; u < kStrip; ++u) {
-        const int j = j0 + u;
-        if (j <= L) {
-          uint16_t cd;
-          if (bd[u] == h[u]) {
-            cd = kCodeDiag | static_cast<uint16_t>(kk[u] & 0xffff);
-          } else if (bv[u] == h[u]) {
-            cd = kCodeVert | static_cast<uint16_t>(kk[u] >> 16);
-          } else {
-            cd = kCodeHorz;
-          }
-          Hr[j] = static_cast<T>(h[u]);
-          Cr[j] = cd;
-          if (far) Hg[j] = h[u];
+        for (int u = 0; u < kStrip; ++u) {
+          if (in[u]) Hg[j0 + u] = h[u];
         }
       }
       // value of the tile's last column: carry into the next tile and, in
@@ -509,7 +572,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
         ++ties;
       }
     }
-    if (far) {
+    if (own == kSlotGlobal) {
       wait_stores();
     }
     wave_fence();
@@ -518,6 +581,309 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   *best_score = best;
   *tied = ties > 0;
   return best_row;
+}
+
+// Graph insertion of one layer (spoa add_alignment), position by position,
+// lane-uniform. The alignment pairs are stored backwards (aln_*[0] is the
+// last pair). New nodes record their anchor rank for the incremental order.
+__device__ void insert_sequential(Slot& s, const PoaSlotCaps& caps, int L, int na, bool* order_ok_out,
+                                  int* n_path_out) {
+  bool order_ok = true;
+  int n_path = 0;
+  {
+    int first_pos = -1, last_pos = -1;
+    for (int q = na - 1; q >= 0; --q) {
+      if (s.aln_pos()[q] != -1) {
+        first_pos = s.aln_pos()[q];
+        break;
+      }
+    }
+    for (int q = 0; q < na; ++q) {
+      if (s.aln_pos()[q] != -1) {
+        last_pos = s.aln_pos()[q];
+        break;
+      }
+    }
+    const int before = s.n_nodes;
+    add_chain(s, caps, 0, first_pos);
+    int head = before == s.n_nodes ? -1 : s.n_nodes - 1;
+    const int tail = add_chain(s, caps, last_pos + 1, L);
+    if (s.n_nodes != before) {
+      order_ok = false;  // chains outside the alignment (not produced by NW)
+    }
+    int prev_w = head == -1 ? 0 : s.qual()[first_pos - 1];
+    int head_anchor = -1;  // anchor of the previous path node when it is new
+    for (int q = na - 1; q >= 0 && !s.overflow; --q) {
+      const int pos = uniform(s.aln_pos()[q]);
+      if (pos == -1) {
+        continue;
+      }
+      const int node = uniform(s.aln_node()[q]);
+      const uint8_t letter = s.seq()[pos];
+      int nv;
+      if (node == -1) {
+        nv = add_node(s, caps, letter);
+      } else if (s.base()[node] == letter) {
+        nv = node;
+      } else {
+        int same = -1;
+        const int cnt = s.n_aligned()[node];
+        for (int k = 0; k < cnt; ++k) {
+          const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+          if (s.base()[a] == letter) {
+            same = a;
+            break;
+          }
+        }
+        if (same == -1) {
+          nv = add_node(s, caps, letter);
+          if (s.overflow || cnt + 1 > static_cast<int>(kMaxAligned)) {
+            s.overflow = true;
+            break;
+          }
+          for (int k = 0; k < cnt; ++k) {
+            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+            const int ca = s.n_aligned()[a];
+            s.aligned()[nv * kMaxAligned + k] = a;
+            s.aligned()[a * kMaxAligned + ca] = nv;
+            s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
+          }
+          s.aligned()[nv * kMaxAligned + cnt] = node;
+          s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
+          s.aligned()[node * kMaxAligned + cnt] = nv;
+          s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
+        } else {
+          nv = same;
+        }
+      }
+      if (nv >= before) {
+        // a new node: placed after its path predecessor's aligned set
+        // (mismatch nodes: after the aligned set they join)
+        int anchor;
+        if (node != -1) {
+          anchor = s.node_rank()[node];
+          const int cnt = s.n_aligned()[node];
+          for (int k = 0; k < cnt; ++k) {
+            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+            if (a < before) anchor = max(anchor, s.node_rank()[a]);
+          }
+        } else if (head == -1) {
+          anchor = -1;
+        } else if (head >= before) {
+          anchor = head_anchor;
+        } else {
+          anchor = s.node_rank()[head];
+          const int cnt = s.n_aligned()[head];
+          for (int k = 0; k < cnt; ++k) {
+            const int a = uniform(s.aligned()[head * kMaxAligned + k]);
+            if (a < before) anchor = max(anchor, s.node_rank()[a]);
+          }
+        }
+        const int k_new = nv - before;
+        if (k_new > 0 && anchor < s.anchor()[k_new - 1]) {
+          order_ok = false;
+        }
+        s.anchor()[k_new] = anchor;
+        head_anchor = anchor;
+      }
+      s.lpath()[n_path++] = nv;
+      if (head != -1) {
+        add_edge(s, caps, head, nv, prev_w + s.qual()[pos]);
+      }
+      head = nv;
+      prev_w = s.qual()[pos];
+    }
+    if (tail != -1 && !s.overflow) {
+      add_edge(s, caps, head, tail, prev_w + s.qual()[last_pos + 1]);
+    }
+    s.n_seqs = s.n_seqs + 1;
+  }
+  *order_ok_out = order_ok;
+  *n_path_out = n_path;
+}
+
+// Graph insertion of one layer, one lane per layer position. A global
+// alignment covers every layer position once (positions 0..L-1 in order), so
+// the path is the whole layer and its graph nodes are distinct. When no two
+// of them share an aligned set, every position reads and writes only its own
+// node, aligned set, in-edge list and out-edge list, and the result equals
+// the position-by-position insertion: node ids and edge ids are handed out
+// in position order with wave prefix counts, and the anchor of an insertion
+// (the rank bound of the last non-insertion position before it) is a wave
+// prefix scan. Returns 1 when two path nodes share an aligned set (nothing
+// in the graph has changed then), 0 when inserted, or a negative status.
+__device__ int insert_parallel(Slot& s, const PoaSlotCaps& caps, const uint8_t* lseq, int L, int na,
+                               bool* order_ok) {
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int32_t* pnode = s.path();   // aligned graph node of each layer position (-1: none)
+  int32_t* pnv = s.lpath();    // graph node of each layer position after insertion
+  uint8_t* stamp = s.in_sub(); // 2 on the path's nodes (the DP no longer needs it)
+  for (int q = lane; q < na; q += kWave) {
+    const int pos = s.aln_pos()[q];
+    if (pos != -1) pnode[pos] = s.aln_node()[q];
+  }
+  wave_sync();
+  for (int p = lane; p < L; p += kWave) {
+    const int v = pnode[p];
+    if (v != -1) stamp[v] = 2;
+  }
+  wave_sync();
+  // existing node per position (match, or an aligned node with the letter)
+  bool conflict = false, full = false;
+  int n_new = 0;
+  for (int p0 = 0; p0 < L; p0 += kWave) {
+    const int p = p0 + lane;
+    bool isnew = false;
+    if (p < L) {
+      const int v = pnode[p];
+      int nv = -1;
+      if (v != -1) {
+        const uint8_t letter = lseq[p];
+        const int cnt = s.n_aligned()[v];
+        nv = s.base()[v] == letter ? v : -1;
+        for (int k = 0; k < cnt; ++k) {
+          const int a = s.aligned()[v * kMaxAligned + k];
+          conflict |= stamp[a] == 2;
+          if (nv == -1 && s.base()[a] == letter) nv = a;
+        }
+        full |= nv == -1 && cnt + 1 > static_cast<int>(kMaxAligned);
+      }
+      isnew = nv == -1;
+      pnv[p] = nv;
+    }
+    n_new += __popcll(__ballot(isnew));
+  }
+  if (__ballot(conflict) != 0) {
+    return 1;
+  }
+  if (__ballot(full) != 0 || s.n_nodes + n_new > static_cast<int>(caps.nodes)) {
+    return kPoaNodeOverflow;
+  }
+  wave_sync();
+  // new nodes (ids in position order), aligned sets, anchors
+  const int before = s.n_nodes;
+  int next_id = before;
+  int carry = -1;  // rank bound of the last non-insertion position so far
+  for (int p0 = 0; p0 < L; p0 += kWave) {
+    const int p = p0 + lane;
+    const bool in = p < L;
+    const int v = in ? pnode[p] : -1;
+    int nv = in ? pnv[p] : 0;
+    const bool isnew = in && nv == -1;
+    const unsigned long long m = __ballot(isnew);
+    if (isnew) nv = next_id + __popcll(m & below);
+    next_id += __popcll(m);
+    int bound = -1;  // highest rank in v's aligned set, v included
+    if (v != -1) {
+      bound = s.node_rank()[v];
+      const int cnt = s.n_aligned()[v];
+      for (int k = 0; k < cnt; ++k) {
+        bound = max(bound, s.node_rank()[s.aligned()[v * kMaxAligned + k]]);
+      }
+    }
+    const int incl = wave_prefix_max(v != -1 ? lane : -1);
+    const int excl = wave_shift_right(incl, -1);
+    const int from_lane = __shfl(bound, max(excl, 0), kWave);
+    const int prev_bound = excl >= 0 ? from_lane : carry;
+    if (isnew) {
+      s.anchor()[nv - before] = v != -1 ? bound : prev_bound;
+      s.base()[nv] = lseq[p];
+      s.in_head()[nv] = -1;
+      s.out_head()[nv] = -1;
+      s.in_tail()[nv] = -1;
+      s.out_tail()[nv] = -1;
+      s.n_aligned()[nv] = 0;
+      s.cov()[nv] = 0;
+      s.last_label()[nv] = -1;
+      if (v != -1) {
+        // the mismatch node joins v's aligned set
+        const int cnt = s.n_aligned()[v];
+        for (int k = 0; k < cnt; ++k) {
+          const int a = s.aligned()[v * kMaxAligned + k];
+          const int ca = s.n_aligned()[a];
+          s.aligned()[nv * kMaxAligned + k] = a;
+          s.aligned()[a * kMaxAligned + ca] = nv;
+          s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
+        }
+        s.aligned()[nv * kMaxAligned + cnt] = v;
+        s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
+        s.aligned()[v * kMaxAligned + cnt] = nv;
+        s.n_aligned()[v] = static_cast<uint8_t>(cnt + 1);
+      }
+    }
+    if (in) pnv[p] = nv;
+    const int last = __builtin_amdgcn_readlane(incl, kWave - 1);
+    if (last >= 0) carry = __builtin_amdgcn_readlane(bound, last);
+  }
+  s.n_nodes = next_id;
+  wave_sync();
+  // anchors of the new nodes must be non-decreasing for the merged order
+  bool bad = false;
+  for (int k = 1 + lane; k < n_new; k += kWave) {
+    bad |= s.anchor()[k] < s.anchor()[k - 1];
+  }
+  *order_ok = __ballot(bad) == 0;
+  // edges between consecutive positions: weight w[p-1] + w[p]
+  for (int p0 = 1; p0 < L; p0 += kWave) {
+    const int p = p0 + lane;
+    const bool in = p < L;
+    int tail = 0, head = 0, w = 0, found = -1;
+    if (in) {
+      tail = pnv[p - 1];
+      head = pnv[p];
+      w = s.qual()[p - 1] + s.qual()[p];
+      for (int e = s.out_head()[tail]; e != -1; e = s.e_next_out()[e]) {
+        if (s.e_head()[e] == head) {
+          found = e;
+          break;
+        }
+      }
+      if (found != -1) s.e_weight()[found] = s.e_weight()[found] + w;
+    }
+    const bool need = in && found == -1;
+    const unsigned long long m = __ballot(need);
+    if (s.n_edges + __popcll(m) > static_cast<int>(caps.edges)) {
+      return kPoaNodeOverflow;
+    }
+    if (need) {
+      const int e = s.n_edges + __popcll(m & below);
+      s.e_tail()[e] = tail;
+      s.e_head()[e] = head;
+      s.e_weight()[e] = w;
+      s.e_next_in()[e] = -1;
+      s.e_next_out()[e] = -1;
+      const int ot = s.out_tail()[tail];
+      if (ot == -1) {
+        s.out_head()[tail] = e;
+      } else {
+        s.e_next_out()[ot] = e;
+      }
+      s.out_tail()[tail] = e;
+      const int it = s.in_tail()[head];
+      if (it == -1) {
+        s.in_head()[head] = e;
+      } else {
+        s.e_next_in()[it] = e;
+      }
+      s.in_tail()[head] = e;
+    }
+    s.n_edges += __popcll(m);
+  }
+  // every node of the layer has an incident edge of this sequence when L > 1
+  if (L > 1) {
+    const int label = s.n_seqs;
+    for (int p = lane; p < L; p += kWave) {
+      const int v = pnv[p];
+      if (s.last_label()[v] != label) {
+        s.last_label()[v] = label;
+        s.cov()[v] = s.cov()[v] + 1;
+      }
+    }
+  }
+  s.n_seqs = s.n_seqs + 1;
+  wave_sync();
+  return 0;
 }
 
 // ---- the window ----
@@ -653,20 +1019,25 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     // score width: 16-bit ring rows when every score of the layer fits
     const int32_t max_abs = max(max(abs(P.match), abs(P.mismatch)), abs(P.gap));
     const bool narrow = static_cast<int64_t>(max_abs) * (V + L + 2) < 32000;
-    const int ring = narrow ? static_cast<int>(P.ring_rows)
-                            : max(1, static_cast<int>(P.ring_rows) / 2);
+    const int n_slots = min(kMaxLdsSlots, narrow ? static_cast<int>(P.ring_rows)
+                                                 : max(1, static_cast<int>(P.ring_rows) / 2));
 
-    // DP row of every included node; predecessor rows, sink and far flags
+    // DP row of every included node; predecessor rows, sink flags, slots
     for (int r = lane; r < V; r += kWave) {
       s.row_of()[s.sub_order()[r]] = r + 1;
     }
     wave_sync();
-    // predecessor counts per row (lane-parallel), then offsets
+    // predecessor counts per row (lane-parallel), then offsets; each row that
+    // a later row reads gets the lowest LDS slot whose previous row is no
+    // longer needed (lane k < n_slots holds the last row reading slot k)
     {
       int run = 0;
+      int busy = lane < n_slots ? 0 : 0x7fffffff;
       for (int r0 = 0; r0 < V; r0 += kWave) {
         const int r = r0 + lane;
         int cnt = 0;
+        int last_use = 0;
+        int sink = 0;
         if (r < V) {
           const int v = s.sub_order()[r];
           for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
@@ -678,7 +1049,6 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
             cnt = 1;  // the virtual start row
           }
           int out_cnt = 0;
-          int last_use = 0;
           for (int e = s.out_head()[v]; e != -1; e = s.e_next_out()[e]) {
             const int h = s.e_head()[e];
             if (s.in_sub()[h] != 0) {
@@ -686,8 +1056,26 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
               last_use = max(last_use, s.row_of()[h]);
             }
           }
-          s.row_meta()[r] = static_cast<int>(s.base()[v]) | (out_cnt == 0 ? kRowMetaSink : 0) |
-                          (last_use - (r + 1) >= ring ? kRowMetaFar : 0);
+          sink = out_cnt == 0 ? kRowMetaSink : 0;
+          sink |= static_cast<int>(s.base()[v]);
+        }
+        int my_slot = kSlotNone;
+        const int n_chunk = min(kWave, V - r0);
+        for (int rr = 0; rr < n_chunk; ++rr) {
+          const int lu = __builtin_amdgcn_readlane(last_use, rr);
+          if (lu > 0) {
+            const int row = r0 + rr + 1;
+            const unsigned long long free_mask = __ballot(busy < row);
+            int sl = kSlotGlobal;
+            if (free_mask != 0) {
+              sl = __ffsll(static_cast<long long>(free_mask)) - 1;
+              if (lane == sl) busy = lu;
+            }
+            if (lane == rr) my_slot = sl;
+          }
+        }
+        if (r < V) {
+          s.row_meta()[r] = sink | (my_slot << kRowMetaSlotShift);
         }
         // inclusive scan of cnt
         int x = cnt;
@@ -711,7 +1099,9 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
         for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
           const int u = s.e_tail()[e];
           if (s.in_sub()[u] != 0) {
-            s.pred_rows()[o++] = s.row_of()[u];
+            const int pr = s.row_of()[u];
+            const int sl = (s.row_meta()[pr - 1] >> kRowMetaSlotShift) & 31;
+            s.pred_rows()[o++] = pr | (sl << kPredRowBits);
             any = true;
           }
         }
@@ -727,8 +1117,8 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     clk.lap(R, kPhaseColumn0);
     int best_score = 0;
     bool tied = false;
-    int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, ring, &best_score, &tied)
-                                  : dp_layer<int32_t>(P, s, lds, V, L, ring, &best_score, &tied));
+    int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied)
+                                  : dp_layer<int32_t>(P, s, lds, V, L, &best_score, &tied));
     wave_sync();
     if (tied) {
       // several sinks share the best score: spoa takes the first one in its
@@ -776,7 +1166,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           --j;
           continue;
         }
-        const int c = uniform(s.code()[static_cast<int64_t>(i) * pitch + j]);
+        const int c = uniform(s.code()[static_cast<int64_t>(i) * pitch + code_index(j, pitch)]);
         const int kind = c & 0xc000;
         const int k = c & 0x3fff;
         const int node = uniform(s.sub_order()[i - 1]);
@@ -784,13 +1174,13 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           s.aln_node()[na] = node;
           s.aln_pos()[na] = j - 1;
           ++na;
-          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]);
+          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]) & kPredRowMask;
           --j;
         } else if (kind == kCodeVert) {
           s.aln_node()[na] = node;
           s.aln_pos()[na] = -1;
           ++na;
-          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]);
+          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]) & kPredRowMask;
         } else {
           s.aln_node()[na] = -1;
           s.aln_pos()[na] = j - 1;
@@ -801,117 +1191,22 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     }
 
     clk.lap(R, kPhaseTraceback);
-    // 5. insert the layer (alignment walked forwards = pairs backwards)
+    // 5. insert the layer: lane-parallel over layer positions unless two
+    // nodes of the path share an aligned set (then position by position)
     bool order_ok = true;
     int n_path = 0;
     const int nodes_before = s.n_nodes;
     {
-      int first_pos = -1, last_pos = -1;
-      for (int q = na - 1; q >= 0; --q) {
-        if (s.aln_pos()[q] != -1) {
-          first_pos = s.aln_pos()[q];
-          break;
-        }
+      const int rc = insert_parallel(s, caps, lds_seq, L, na, &order_ok);
+      if (rc < 0) {
+        R.status = rc;
+        return;
       }
-      for (int q = 0; q < na; ++q) {
-        if (s.aln_pos()[q] != -1) {
-          last_pos = s.aln_pos()[q];
-          break;
-        }
+      if (rc == 0) {
+        n_path = L;
+      } else {
+        insert_sequential(s, caps, L, na, &order_ok, &n_path);
       }
-      const int before = s.n_nodes;
-      add_chain(s, caps, 0, first_pos);
-      int head = before == s.n_nodes ? -1 : s.n_nodes - 1;
-      const int tail = add_chain(s, caps, last_pos + 1, L);
-      if (s.n_nodes != before) {
-        order_ok = false;  // chains outside the alignment (not produced by NW)
-      }
-      int prev_w = head == -1 ? 0 : s.qual()[first_pos - 1];
-      int head_anchor = -1;  // anchor of the previous path node when it is new
-      for (int q = na - 1; q >= 0 && !s.overflow; --q) {
-        const int pos = uniform(s.aln_pos()[q]);
-        if (pos == -1) {
-          continue;
-        }
-        const int node = uniform(s.aln_node()[q]);
-        const uint8_t letter = s.seq()[pos];
-        int nv;
-        if (node == -1) {
-          nv = add_node(s, caps, letter);
-        } else if (s.base()[node] == letter) {
-          nv = node;
-        } else {
-          int same = -1;
-          const int cnt = s.n_aligned()[node];
-          for (int k = 0; k < cnt; ++k) {
-            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-            if (s.base()[a] == letter) {
-              same = a;
-              break;
-            }
-          }
-          if (same == -1) {
-            nv = add_node(s, caps, letter);
-            if (s.overflow || cnt + 1 > static_cast<int>(kMaxAligned)) {
-              s.overflow = true;
-              break;
-            }
-            for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-              const int ca = s.n_aligned()[a];
-              s.aligned()[nv * kMaxAligned + k] = a;
-              s.aligned()[a * kMaxAligned + ca] = nv;
-              s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
-            }
-            s.aligned()[nv * kMaxAligned + cnt] = node;
-            s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
-            s.aligned()[node * kMaxAligned + cnt] = nv;
-            s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
-          } else {
-            nv = same;
-          }
-        }
-        if (nv >= before) {
-          // a new node: placed after its path predecessor's aligned set
-          // (mismatch nodes: after the aligned set they join)
-          int anchor;
-          if (node != -1) {
-            anchor = s.node_rank()[node];
-            const int cnt = s.n_aligned()[node];
-            for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-              if (a < before) anchor = max(anchor, s.node_rank()[a]);
-            }
-          } else if (head == -1) {
-            anchor = -1;
-          } else if (head >= before) {
-            anchor = head_anchor;
-          } else {
-            anchor = s.node_rank()[head];
-            const int cnt = s.n_aligned()[head];
-            for (int k = 0; k < cnt; ++k) {
-              const int a = uniform(s.aligned()[head * kMaxAligned + k]);
-              if (a < before) anchor = max(anchor, s.node_rank()[a]);
-            }
-          }
-          const int k_new = nv - before;
-          if (k_new > 0 && anchor < s.anchor()[k_new - 1]) {
-            order_ok = false;
-          }
-          s.anchor()[k_new] = anchor;
-          head_anchor = anchor;
-        }
-        s.lpath()[n_path++] = nv;
-        if (head != -1) {
-          add_edge(s, caps, head, nv, prev_w + s.qual()[pos]);
-        }
-        head = nv;
-        prev_w = s.qual()[pos];
-      }
-      if (tail != -1 && !s.overflow) {
-        add_edge(s, caps, head, tail, prev_w + s.qual()[last_pos + 1]);
-      }
-      s.n_seqs = s.n_seqs + 1;
     }
     if (s.overflow) {
       R.status = kPoaNodeOverflow;

@@ -343,7 +343,10 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
     }
     const uint64_t node_bound = w.backbone_length + sum_len;
     const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
-    if (node_bound > 0x7fffffffull || edge_bound > 0x7fffffffull) {
+    // DP rows are packed in 24 bits of a predecessor entry and the in-edge
+    // index (at most one in-edge per node and layer) in 14 bits of a move code
+    if (node_bound >= (1ull << 24) - 1 || edge_bound > 0x7fffffffull ||
+        w.n_layers >= (1u << 14) - 1) {
       rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is too large");
       return RH_ERR_ARGUMENT;
     }
