@@ -159,6 +159,10 @@ def main() -> int:
     total_phase = sum(phases.values()) or 1.0
     log("phase share of wave time: " + ", ".join(
         f"{k} {100 * v / total_phase:.1f}%" for k, v in phases.items()))
+    # busy wave time over resident-slot time: below 1 when waves idle at the
+    # end of the queue (load imbalance) or between windows
+    busy_frac = total_phase / (st.slots * (st.kernel_ms / 1e3)) if st.slots else 0.0
+    log(f"wave busy fraction {busy_frac:.3f} ({st.slots} resident waves)")
     gpu_out, gpu_status = batch.download()
     n_windows = len(windows)
     cells = st.cells
@@ -253,6 +257,8 @@ def main() -> int:
                                         / VALU_PEAK_TOPS, 5)},
         "cpu_baseline": cpu,
         "poa_phase_share": {k: round(v / total_phase, 4) for k, v in phases.items()},
+        "poa_wave_busy_frac": round(busy_frac, 4),
+        "poa_resident_waves": st.slots,
         "autotune_ms": {str(k): round(v, 1) for k, v in tuned.items()},
         "ring_rows": best_ring,
     }

@@ -35,12 +35,42 @@ namespace rh {
 namespace {
 
 constexpr int kWave = 64;
+// Launch-invariant data (parameters, slot layout, window and layer tables)
+// is read through constant-address-space references: uniform scalar loads
+// that no store of the kernel can clobber, hoisted and kept in scalar
+// registers.
+typedef __attribute__((address_space(1))) uint8_t GByte;
+// Device buffers named by the parameters, as global-memory pointers (they
+// are loaded from memory, so the compiler cannot infer their address space).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+typedef const __attribute__((address_space(4))) PoaParams CParams;
+typedef const __attribute__((address_space(4))) PoaSlotLayout CLayout;
+typedef const __attribute__((address_space(4))) PoaSlotCaps CCaps;
+typedef const __attribute__((address_space(4))) PoaWindow CWindow;
+typedef const __attribute__((address_space(4))) PoaLayer CLayer;
+
+// Resident waves per SIMD the kernel is register-allocated for (168 VGPRs
+// at 3, 128 at 4).
+#ifndef RH_POA_WAVES_PER_EU
+#define RH_POA_WAVES_PER_EU 4
+#endif
+
 constexpr int kStrip = 4;                  // DP columns per lane per tile
 constexpr int kTile = kWave * kStrip;      // 256 columns per tile
 constexpr int32_t kNegInf = -(1 << 29);
+constexpr int32_t kNegKey = -(1 << 30);  // below every (score, 255 - k) key
+// Move codes (16 bits per DP cell): bits 15-14 the move (diagonal,
+// vertical, horizontal); for diagonal and vertical moves bits 12-0 name the
+// predecessor row, as the row delta r - p, or, with kCodeViaK set, as the
+// index k of the predecessor in the row's in-edge list.
 constexpr uint16_t kCodeDiag = 0x0000;
 constexpr uint16_t kCodeVert = 0x4000;
 constexpr uint16_t kCodeHorz = 0x8000;
+constexpr uint16_t kCodeViaK = 0x2000;
+constexpr uint32_t kCodePayload = 0x1fff;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
@@ -63,45 +93,45 @@ struct PhaseClock {
 // (kernel arguments), so the array addresses are rebuilt where used instead of
 // occupying registers.
 struct Slot {
-  uint8_t* p;
-  const PoaSlotLayout* L;
-  __device__ uint8_t* base() const { return reinterpret_cast<uint8_t*>(p + L->base); }
-  __device__ int32_t* in_head() const { return reinterpret_cast<int32_t*>(p + L->in_head); }
-  __device__ int32_t* out_head() const { return reinterpret_cast<int32_t*>(p + L->out_head); }
-  __device__ int32_t* in_tail() const { return reinterpret_cast<int32_t*>(p + L->in_tail); }
-  __device__ int32_t* out_tail() const { return reinterpret_cast<int32_t*>(p + L->out_tail); }
-  __device__ uint8_t* n_aligned() const { return reinterpret_cast<uint8_t*>(p + L->n_aligned); }
-  __device__ int32_t* aligned() const { return reinterpret_cast<int32_t*>(p + L->aligned); }
-  __device__ int32_t* cov() const { return reinterpret_cast<int32_t*>(p + L->cov); }
-  __device__ int32_t* last_label() const { return reinterpret_cast<int32_t*>(p + L->last_label); }
-  __device__ int32_t* e_tail() const { return reinterpret_cast<int32_t*>(p + L->e_tail); }
-  __device__ int32_t* e_head() const { return reinterpret_cast<int32_t*>(p + L->e_head); }
-  __device__ int32_t* e_weight() const { return reinterpret_cast<int32_t*>(p + L->e_weight); }
-  __device__ int32_t* e_next_in() const { return reinterpret_cast<int32_t*>(p + L->e_next_in); }
-  __device__ int32_t* e_next_out() const { return reinterpret_cast<int32_t*>(p + L->e_next_out); }
-  __device__ int32_t* rank_to_node() const { return reinterpret_cast<int32_t*>(p + L->rank_to_node); }
-  __device__ int32_t* node_rank() const { return reinterpret_cast<int32_t*>(p + L->node_rank); }
-  __device__ uint8_t* mark() const { return reinterpret_cast<uint8_t*>(p + L->mark); }
-  __device__ uint8_t* lead() const { return reinterpret_cast<uint8_t*>(p + L->lead); }
-  __device__ int32_t* stack() const { return reinterpret_cast<int32_t*>(p + L->stack); }
-  __device__ uint8_t* in_sub() const { return reinterpret_cast<uint8_t*>(p + L->in_sub); }
-  __device__ int32_t* row_of() const { return reinterpret_cast<int32_t*>(p + L->row_of); }
-  __device__ int32_t* sub_order() const { return reinterpret_cast<int32_t*>(p + L->sub_order); }
-  __device__ int32_t* pred_off() const { return reinterpret_cast<int32_t*>(p + L->pred_off); }
-  __device__ int32_t* pred_rows() const { return reinterpret_cast<int32_t*>(p + L->pred_rows); }
-  __device__ int32_t* row_meta() const { return reinterpret_cast<int32_t*>(p + L->row_meta); }
-  __device__ int32_t* H() const { return reinterpret_cast<int32_t*>(p + L->H); }
-  __device__ uint16_t* code() const { return reinterpret_cast<uint16_t*>(p + L->code); }
-  __device__ int32_t* aln_node() const { return reinterpret_cast<int32_t*>(p + L->aln_node); }
-  __device__ int32_t* aln_pos() const { return reinterpret_cast<int32_t*>(p + L->aln_pos); }
-  __device__ int64_t* scores() const { return reinterpret_cast<int64_t*>(p + L->scores); }
-  __device__ int32_t* preds() const { return reinterpret_cast<int32_t*>(p + L->preds); }
-  __device__ int32_t* path() const { return reinterpret_cast<int32_t*>(p + L->path); }
-  __device__ uint8_t* seq() const { return reinterpret_cast<uint8_t*>(p + L->seq); }
-  __device__ int32_t* qual() const { return reinterpret_cast<int32_t*>(p + L->qual); }
-  __device__ int32_t* anchor() const { return reinterpret_cast<int32_t*>(p + L->anchor); }
-  __device__ int32_t* lpath() const { return reinterpret_cast<int32_t*>(p + L->lpath); }
-  __device__ int32_t* sink_score() const { return reinterpret_cast<int32_t*>(p + L->sink_score); }
+  GByte* p;  // slot base (global memory)
+  CLayout* L;
+  __device__ uint8_t* base() const { return (uint8_t*)(p + L->base); }
+  __device__ int32_t* in_head() const { return (int32_t*)(p + L->in_head); }
+  __device__ int32_t* out_head() const { return (int32_t*)(p + L->out_head); }
+  __device__ int32_t* in_tail() const { return (int32_t*)(p + L->in_tail); }
+  __device__ int32_t* out_tail() const { return (int32_t*)(p + L->out_tail); }
+  __device__ uint8_t* n_aligned() const { return (uint8_t*)(p + L->n_aligned); }
+  __device__ int32_t* aligned() const { return (int32_t*)(p + L->aligned); }
+  __device__ int32_t* cov() const { return (int32_t*)(p + L->cov); }
+  __device__ int32_t* last_label() const { return (int32_t*)(p + L->last_label); }
+  __device__ int32_t* e_tail() const { return (int32_t*)(p + L->e_tail); }
+  __device__ int32_t* e_head() const { return (int32_t*)(p + L->e_head); }
+  __device__ int32_t* e_weight() const { return (int32_t*)(p + L->e_weight); }
+  __device__ int32_t* e_next_in() const { return (int32_t*)(p + L->e_next_in); }
+  __device__ int32_t* e_next_out() const { return (int32_t*)(p + L->e_next_out); }
+  __device__ int32_t* rank_to_node() const { return (int32_t*)(p + L->rank_to_node); }
+  __device__ int32_t* node_rank() const { return (int32_t*)(p + L->node_rank); }
+  __device__ uint8_t* mark() const { return (uint8_t*)(p + L->mark); }
+  __device__ uint8_t* lead() const { return (uint8_t*)(p + L->lead); }
+  __device__ int32_t* stack() const { return (int32_t*)(p + L->stack); }
+  __device__ uint8_t* in_sub() const { return (uint8_t*)(p + L->in_sub); }
+  __device__ int32_t* row_of() const { return (int32_t*)(p + L->row_of); }
+  __device__ int32_t* sub_order() const { return (int32_t*)(p + L->sub_order); }
+  __device__ int32_t* pred_off() const { return (int32_t*)(p + L->pred_off); }
+  __device__ int32_t* pred_rows() const { return (int32_t*)(p + L->pred_rows); }
+  __device__ int32_t* row_meta() const { return (int32_t*)(p + L->row_meta); }
+  __device__ int32_t* H() const { return (int32_t*)(p + L->H); }
+  __device__ uint16_t* code() const { return (uint16_t*)(p + L->code); }
+  __device__ int32_t* aln_node() const { return (int32_t*)(p + L->aln_node); }
+  __device__ int32_t* aln_pos() const { return (int32_t*)(p + L->aln_pos); }
+  __device__ int64_t* scores() const { return (int64_t*)(p + L->scores); }
+  __device__ int32_t* preds() const { return (int32_t*)(p + L->preds); }
+  __device__ int32_t* path() const { return (int32_t*)(p + L->path); }
+  __device__ uint8_t* seq() const { return (uint8_t*)(p + L->seq); }
+  __device__ int32_t* qual() const { return (int32_t*)(p + L->qual); }
+  __device__ int32_t* anchor() const { return (int32_t*)(p + L->anchor); }
+  __device__ int32_t* lpath() const { return (int32_t*)(p + L->lpath); }
+  __device__ int32_t* sink_score() const { return (int32_t*)(p + L->sink_score); }
 
   int32_t n_nodes;
   int32_t n_edges;
@@ -109,7 +139,7 @@ struct Slot {
   bool overflow;
 };
 
-__device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
+__device__ __forceinline__ Slot carve(GByte* p, CLayout& l) {
   Slot s;
   s.p = p;
   s.L = &l;
@@ -122,14 +152,14 @@ __device__ Slot carve(uint8_t* p, const PoaSlotLayout& l) {
 
 // ---- graph construction (lane-uniform) ----
 
-__device__ void touch(Slot& s, int node, int label) {
+__device__ __forceinline__ void touch(Slot& s, int node, int label) {
   if (s.last_label()[node] != label) {
     s.last_label()[node] = label;
     s.cov()[node] = s.cov()[node] + 1;
   }
 }
 
-__device__ int add_node(Slot& s, const PoaSlotCaps& caps, uint8_t base) {
+__device__ __forceinline__ int add_node(Slot& s, CCaps& caps, uint8_t base) {
   if (s.n_nodes >= static_cast<int>(caps.nodes)) {
     s.overflow = true;
     return 0;
@@ -146,7 +176,7 @@ __device__ int add_node(Slot& s, const PoaSlotCaps& caps, uint8_t base) {
   return v;
 }
 
-__device__ void add_edge(Slot& s, const PoaSlotCaps& caps, int tail, int head, int weight) {
+__device__ __forceinline__ void add_edge(Slot& s, CCaps& caps, int tail, int head, int weight) {
   for (int e = s.out_head()[tail]; e != -1; e = s.e_next_out()[e]) {
     if (s.e_head()[e] == head) {
       s.e_weight()[e] = s.e_weight()[e] + weight;
@@ -182,7 +212,7 @@ __device__ void add_edge(Slot& s, const PoaSlotCaps& caps, int tail, int head, i
 }
 
 // Chain of new nodes for seq[begin, end); returns the first node or -1.
-__device__ int add_chain(Slot& s, const PoaSlotCaps& caps, int begin, int end) {
+__device__ __forceinline__ int add_chain(Slot& s, CCaps& caps, int begin, int end) {
   if (begin >= end) {
     return -1;
   }
@@ -196,7 +226,7 @@ __device__ int add_chain(Slot& s, const PoaSlotCaps& caps, int begin, int end) {
 
 // DFS topological order over nodes with keep[v] (all nodes when keep is
 // null); writes node ids in rank order to out, returns how many.
-__device__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* out, int stack_cap) {
+__device__ __forceinline__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* out, int stack_cap) {
   const int n = s.n_nodes;
   for (int v = lane_id(); v < n; v += kWave) {
     s.mark()[v] = 0;
@@ -378,13 +408,18 @@ struct Quad<int32_t> {
 // counter also counts the traceback-code stores of earlier rows). Strips
 // that start past column L are idle; the columns past L inside the last
 // strip are computed from padding and never read back.
+// General DP (any score width, layer length and in-degree).
 template <typename T>
-__device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, int L,
+__device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds, int V, int L,
                         int* best_score, bool* tied) {
   static_assert(kStrip == 4, "strips are loaded and stored as quads");
   const int lane = lane_id();
   const int pitch = static_cast<int>(P.caps.cols);
-  const int32_t gap = P.gap;
+  // scores as uniform registers (a select between two parameter fields
+  // would otherwise become a per-lane load from the selected address)
+  const int32_t gap = uniform(P.gap);
+  const int32_t match = uniform(P.match);
+  const int32_t mismatch = uniform(P.mismatch);
   T* ring = reinterpret_cast<T*>(lds);
   const uint8_t* seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
   int32_t best = kNegInf;
@@ -451,7 +486,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
     if (lane == 0) {
       if (own < kSlotGlobal) Hr[kColShift] = static_cast<T>(c0);
       if (own == kSlotGlobal) Hg[0] = c0;
-      Cr[pitch - 1] = kCodeVert | static_cast<uint16_t>(k0);
+      Cr[pitch - 1] = kCodeVert | kCodeViaK | static_cast<uint16_t>(k0);
     }
     int32_t carry = c0;
     int32_t h_last = kNegInf;
@@ -465,7 +500,7 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
       bool in[kStrip];
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
-        sc[u] = ((w4 >> (8 * u)) & 0xffu) == b ? P.match : P.mismatch;
+        sc[u] = ((w4 >> (8 * u)) & 0xffu) == b ? match : mismatch;
         in[u] = j0 + u <= L;
       }
       int32_t bd[kStrip], bv[kStrip];
@@ -528,8 +563,8 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
 #pragma unroll
       for (int u = 0; u < kStrip; ++u) {
         h[u] = max(pre[u], in_carry) + (j0 + u) * gap;
-        const uint32_t c_diag = kCodeDiag | (kk[u] & 0xffffu);
-        const uint32_t c_vert = kCodeVert | (kk[u] >> 16);
+        const uint32_t c_diag = kCodeDiag | kCodeViaK | (kk[u] & 0xffffu);
+        const uint32_t c_vert = kCodeVert | kCodeViaK | (kk[u] >> 16);
         cd[u] = bd[u] == h[u] ? c_diag : (bv[u] == h[u] ? c_vert : kCodeHorz);
       }
       if (live) {
@@ -583,11 +618,294 @@ __device__ int dp_layer(const PoaParams& P, const Slot& s, uint8_t* lds, int V, 
   return best_row;
 }
 
+// Predecessor row p (entry e) at the lane's strip: hp[u] = H[p][j0 - 1 + u],
+// u = 0..W. Row 0 is the virtual start row (j * gap).
+template <int W>
+__device__ __forceinline__ void load_strip(const int16_t* ring, int pitch, const Slot& s, int e,
+                                           int j0, int jr, int L, int32_t gap, int32_t* hp) {
+  const int p = e & kPredRowMask;
+  const int sl = static_cast<unsigned>(e) >> kPredRowBits;
+  if (p == 0) {
+#pragma unroll
+    for (int u = 0; u <= W; ++u) hp[u] = (j0 - 1 + u) * gap;
+  } else if (sl < kSlotGlobal) {
+    const int16_t* row = ring + sl * pitch + kColShift;
+    hp[0] = row[jr - 1];
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) Quad<int16_t>::load(row + jr + 4 * q, hp + 1 + 4 * q);
+  } else {
+    const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+#pragma unroll
+    for (int u = 0; u <= W; ++u) {
+      const int j = j0 - 1 + u;
+      hp[u] = j <= L ? row[j] : kNegInf;
+    }
+    wait_stores();
+  }
+}
+
+// Rest of one DP row once the best diagonal (bd) and vertical (bv) moves of
+// every strip column are known: horizontal chain, move codes, row storage.
+// With kKeys the moves are (score << 8 | 255 - k) keys of predecessor k,
+// else plain scores from predecessor 0. Lane 0's chain starts at column 0
+// (key or score c0, code vertical). Returns H[row][L].
+template <int W, bool kKeys>
+__device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* bv, int32_t c0,
+                                              uint32_t pay0, uint32_t pay1, uint32_t pay2,
+                                              uint32_t pay3, bool by_table, int lane,
+                                              int j0, bool live, int L, int32_t gap, int own,
+                                              int pitch, int16_t* ring, int32_t* Hg,
+                                              uint16_t* Cr, int end_lane, int end_u) {
+  auto score = [](int32_t x) { return kKeys ? (x >> 8) : x; };
+  // code payload of the move's predecessor: payk (row delta) for the
+  // first four predecessors when by_table, else kCodeViaK | k
+  auto pred = [&](int32_t x) -> uint32_t {
+    if (!kKeys) return pay0;
+    const uint32_t k = 255u - (static_cast<uint32_t>(x) & 255u);
+    if (!by_table) return kCodeViaK | k;
+    const uint32_t v01 = k == 1u ? pay1 : pay0;
+    const uint32_t v23 = k == 3u ? pay3 : pay2;
+    return k >= 2u ? v23 : v01;
+  };
+  const int32_t h0 = score(c0);
+  int32_t pm = lane == 0 ? h0 : kNegInf;
+  int32_t pre[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    pm = max(pm, score(max(bd[u], bv[u])) - (j0 + u) * gap);
+    pre[u] = pm;
+  }
+  const int32_t excl = wave_shift_right(wave_prefix_max(pm), kNegInf);
+  int32_t h[W];
+  uint32_t cd[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    h[u] = max(pre[u], excl) + (j0 + u) * gap;
+    const uint32_t c = score(bv[u]) == h[u] ? (kCodeVert | pred(bv[u])) : kCodeHorz;
+    cd[u] = score(bd[u]) == h[u] ? (kCodeDiag | pred(bd[u])) : c;
+  }
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      uint2 v;
+      v.x = cd[4 * q] | (cd[4 * q + 1] << 16);
+      v.y = cd[4 * q + 2] | (cd[4 * q + 3] << 16);
+      *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = v;
+    }
+    if (own < kSlotGlobal) {
+      int16_t* row = ring + own * pitch + kColShift;
+#pragma unroll
+      for (int q = 0; q < W / 4; ++q) Quad<int16_t>::store(row + j0 + 4 * q, h + 4 * q);
+    }
+  }
+  if (own == kSlotGlobal) {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      if (j0 + u <= L) Hg[j0 + u] = h[u];
+    }
+  }
+  if (lane == 0) {
+    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(h0);
+    if (own == kSlotGlobal) Hg[0] = h0;
+    Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | pred(c0));
+  }
+  int32_t last = h[0];
+#pragma unroll
+  for (int u = 1; u < W; ++u) {
+    if (u == end_u) last = h[u];
+  }
+  return __builtin_amdgcn_readlane(last, end_lane);
+}
+
+// DP rows of one layer with 16-bit scores, every row in one pass of the
+// wave: lane k owns columns 1 + kW .. (k + 1)W (W = 4, 8 or 12, so that
+// 64W >= L), the layer bytes of its strip stay in registers, and column 0
+// heads lane 0's horizontal chain. Rows with one predecessor (most of them)
+// take no argmax; rows with several keep (score, 255 - k) in one integer so
+// that one max keeps the first predecessor with the best score (in-edge
+// order, as spoa's traceback prefers). Same moves, codes and end cell as
+// dp_layer. Requires in-degree <= 255, 64W + pad <= pitch and 16-bit scores
+// over the whole strip.
+template <int W>
+__device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t* lds, int V, int L,
+                              int* best_score, bool* tied) {
+  static_assert(W % 4 == 0 && W >= 4 && W <= 12, "strip width");
+  const int lane = lane_id();
+  const int pitch = static_cast<int>(P.caps.cols);
+  // scores as uniform registers (a select between two parameter fields
+  // would otherwise become a per-lane load from the selected address)
+  const int32_t gap = uniform(P.gap);
+  const int32_t match = uniform(P.match);
+  const int32_t mismatch = uniform(P.mismatch);
+  int16_t* ring = reinterpret_cast<int16_t*>(lds);
+  const uint8_t* seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
+  const int j0 = 1 + lane * W;
+  const bool live = j0 <= L;
+  const int jr = live ? j0 : 1;  // idle lanes read columns 0..W (in bounds)
+  uint32_t sw[W / 4];            // layer bytes of the strip
+#pragma unroll
+  for (int q = 0; q < W / 4; ++q) sw[q] = *reinterpret_cast<const uint32_t*>(seq + (jr - 1) + 4 * q);
+  const int end_lane = (L - 1) / W;
+  const int end_u = (L - 1) % W;
+  int32_t best = kNegInf;
+  int best_row = 0;
+  int ties = 0;
+  const int total_preds = uniform(s.pred_off()[V]);
+  for (int r0 = 1; r0 <= V; r0 += kWave) {
+    const int rk = r0 + lane;
+    int off_v = 0, end_v = 0, meta_v = 0;
+    if (rk <= V) {
+      off_v = s.pred_off()[rk - 1];
+      end_v = s.pred_off()[rk];
+      meta_v = s.row_meta()[rk - 1];
+    }
+    const int base = __builtin_amdgcn_readlane(off_v, 0);
+    const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
+    const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+    // complete the chunk's loads here: otherwise the row loop would wait for
+    // them (and for every code store issued before them) on each row
+    wait_stores();
+    const int chunk = min(kWave, V - r0 + 1);
+    for (int rr = 0; rr < chunk; ++rr) {
+      const int r = r0 + rr;
+      const int off = __builtin_amdgcn_readlane(off_v, rr);
+      const int np = __builtin_amdgcn_readlane(end_v, rr) - off;
+      const int meta = __builtin_amdgcn_readlane(meta_v, rr);
+      auto pred_at = [&](int k) -> int {
+        const int idx = off + k - base;
+        if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
+        if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
+        const int e = uniform(s.pred_rows()[off + k]);
+        wait_stores();
+        return e;
+      };
+      const uint32_t b = static_cast<uint32_t>(meta & 0xff);
+      const int own = (meta >> kRowMetaSlotShift) & 31;
+      int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+      uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
+      // match score of strip column u against the row's base
+      auto sc = [&](int u) -> int32_t {
+        return ((sw[u / 4] >> (8 * (u % 4))) & 0xffu) == b ? match : mismatch;
+      };
+      int32_t bd[W], bv[W];
+      int32_t h_last;
+      // predecessor payloads of the move codes: row deltas of the first
+      // four predecessors (when every row delta fits the payload)
+      auto delta = [&](int k) -> uint32_t {
+        return k < np ? static_cast<uint32_t>(r - (pred_at(k) & kPredRowMask)) : 0u;
+      };
+      uint32_t pay0 = delta(0), pay1 = delta(1), pay2 = delta(2), pay3 = delta(3);
+      const bool by_table = np <= 4 && max(max(pay0, pay1), max(pay2, pay3)) <= kCodePayload;
+      if (np == 1 && !by_table) {
+        pay0 = kCodeViaK;
+      }
+      if (np == 1) {
+        int32_t hp[W + 1];
+        load_strip<W>(ring, pitch, s, pred_at(0), j0, jr, L, gap, hp);
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          bd[u] = hp[u] + sc(u);
+          bv[u] = hp[u + 1] + gap;
+        }
+        h_last = finish_row<W, false>(bd, bv, hp[0] + gap, pay0, pay1, pay2, pay3, true, lane,
+                                      j0, live, L, gap, own, pitch, ring, Hg, Cr, end_lane,
+                                      end_u);
+      } else {
+        int32_t c0k = kNegKey;
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          bd[u] = kNegKey;
+          bv[u] = kNegKey;
+        }
+        for (int k = 0; k < np; ++k) {
+          int32_t hp[W + 1];
+          load_strip<W>(ring, pitch, s, pred_at(k), j0, jr, L, gap, hp);
+          const int32_t tag = 255 - k;
+#pragma unroll
+          for (int u = 0; u < W; ++u) {
+            bd[u] = max(bd[u], (hp[u] + sc(u)) * 256 | tag);
+            bv[u] = max(bv[u], (hp[u + 1] + gap) * 256 | tag);
+          }
+          c0k = max(c0k, (hp[0] + gap) * 256 | tag);
+        }
+        h_last = finish_row<W, true>(bd, bv, c0k, pay0, pay1, pay2, pay3, by_table, lane, j0,
+                                     live, L, gap, own, pitch, ring, Hg, Cr, end_lane, end_u);
+      }
+      if ((meta & kRowMetaSink) != 0) {
+        if (lane == 0) {
+          s.sink_score()[r] = h_last;
+        }
+        if (h_last > best) {
+          best = h_last;
+          best_row = r;
+          ties = 0;
+        } else if (h_last == best) {
+          ++ties;
+        }
+      }
+      if (own == kSlotGlobal) {
+        wait_stores();
+      }
+      wave_fence();
+    }
+  }
+  *best_score = best;
+  *tied = ties > 0;
+  return best_row;
+}
+
+// Uniform copy of a pointer (both halves read from the first active lane).
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (T*)((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+enum DpKind : int { kDpStrip4, kDpStrip8, kDpStrip12, kDpGeneric16, kDpGeneric32 };
+
+// The DP of one layer as an out-of-line call: its loops get a register
+// allocation of their own instead of sharing one with the rest of the window
+// loop. Arguments arrive in vector registers and are made uniform again on
+// entry. Returns the end-cell row, its score and whether sinks tied.
+typedef __attribute__((address_space(3))) uint8_t LByte;
+
+template <int kKind>
+__device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLayout* layout,
+                                               uint32_t lds_addr, int V, int L) {
+  CParams& P = *uniform_ptr(params);
+  Slot s = carve(uniform_ptr(base), *uniform_ptr(layout));
+  V = uniform(V);
+  L = uniform(L);
+  // the wave's LDS as a local-memory pointer from the caller (no lookup of
+  // the dynamic LDS offset inside the loops)
+  uint8_t* lds = (uint8_t*)(LByte*)(size_t)__builtin_amdgcn_readfirstlane(lds_addr);
+  int best_score = 0;
+  bool tied = false;
+  int row;
+  if (kKind == kDpStrip4) {
+    row = dp_layer_strip<4>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpStrip8) {
+    row = dp_layer_strip<8>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpStrip12) {
+    row = dp_layer_strip<12>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpGeneric16) {
+    row = dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied);
+  } else {
+    row = dp_layer<int32_t>(P, s, lds, V, L, &best_score, &tied);
+  }
+  return (static_cast<uint64_t>(static_cast<uint32_t>(best_score)) << 32) |
+         (static_cast<uint32_t>(row) & 0x7fffffffu) | (tied ? 0x80000000u : 0u);
+}
+
 // Graph insertion of one layer (spoa add_alignment), position by position,
 // lane-uniform. The alignment pairs are stored backwards (aln_*[0] is the
 // last pair). New nodes record their anchor rank for the incremental order.
-__device__ void insert_sequential(Slot& s, const PoaSlotCaps& caps, int L, int na, bool* order_ok_out,
-                                  int* n_path_out) {
+// (Rare: two path nodes share an aligned set. Out of line, returns the slot
+// with its updated counters.)
+__device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, int na,
+                                              bool* order_ok_out, int* n_path_out) {
   bool order_ok = true;
   int n_path = 0;
   {
@@ -700,6 +1018,7 @@ __device__ void insert_sequential(Slot& s, const PoaSlotCaps& caps, int L, int n
   }
   *order_ok_out = order_ok;
   *n_path_out = n_path;
+  return s;
 }
 
 // Graph insertion of one layer, one lane per layer position. A global
@@ -712,7 +1031,7 @@ __device__ void insert_sequential(Slot& s, const PoaSlotCaps& caps, int L, int n
 // (the rank bound of the last non-insertion position before it) is a wave
 // prefix scan. Returns 1 when two path nodes share an aligned set (nothing
 // in the graph has changed then), 0 when inserted, or a negative status.
-__device__ int insert_parallel(Slot& s, const PoaSlotCaps& caps, const uint8_t* lseq, int L, int na,
+__device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8_t* lseq, int L, int na,
                                bool* order_ok) {
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -888,13 +1207,14 @@ __device__ int insert_parallel(Slot& s, const PoaSlotCaps& caps, const uint8_t* 
 
 // ---- the window ----
 
-__device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const PoaWindow& W,
+__device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds, CWindow& W,
                                PoaResult& R) {
   const int lane = lane_id();
-  const PoaSlotCaps& caps = P.caps;
+  CCaps& caps = P.caps;
   const int pitch = static_cast<int>(caps.cols);
-  const uint8_t* bb = P.arena + W.backbone_off;
-  const uint8_t* bq = P.arena + W.backbone_qual;
+  const GByte* arena = global(P.arena);
+  const GByte* bb = arena + W.backbone_off;
+  const GByte* bq = arena + W.backbone_qual;
   const int n = static_cast<int>(W.backbone_len);
   uint64_t cells = 0;
   int max_rows = 0;
@@ -931,21 +1251,21 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
   clk.lap(R, kPhaseSetup);
 
   for (uint32_t li = 0; li < W.n_layers; ++li) {
-    const PoaLayer& Lr = P.layers[W.layer_first + li];
+    CLayer& Lr = ((CLayer*)P.layers)[W.layer_first + li];
     const int L = static_cast<int>(Lr.len);
-    if (L + 1 > pitch) {
+    if (L + static_cast<int>(kPoaColPad) > pitch) {
       R.status = kPoaRowOverflow;
       return;
     }
     // layer bytes and weights into the slot
     uint8_t* lds_seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
     for (int j = lane; j < L; j += kWave) {
-      const uint8_t c = P.arena[Lr.seq_off + j];
+      const uint8_t c = arena[Lr.seq_off + j];
       s.seq()[j] = c;
       lds_seq[j] = c;
       s.qual()[j] = Lr.qual_off == kNoQuality
                       ? 1
-                      : static_cast<int>(P.arena[Lr.qual_off + j]) - 33;
+                      : static_cast<int>(arena[Lr.qual_off + j]) - 33;
     }
     const bool full = Lr.begin < W.offset && Lr.end > W.backbone_len - W.offset;
     if (static_cast<int>(Lr.end) >= s.n_nodes || Lr.begin > Lr.end) {
@@ -1022,6 +1342,8 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     const int n_slots = min(kMaxLdsSlots, narrow ? static_cast<int>(P.ring_rows)
                                                  : max(1, static_cast<int>(P.ring_rows) / 2));
 
+    bool strip_ok = false;  // the one-pass DP applies (dp_layer_strip)
+
     // DP row of every included node; predecessor rows, sink flags, slots
     for (int r = lane; r < V; r += kWave) {
       s.row_of()[s.sub_order()[r]] = r + 1;
@@ -1032,6 +1354,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     // longer needed (lane k < n_slots holds the last row reading slot k)
     {
       int run = 0;
+      int max_np = 0;
       int busy = lane < n_slots ? 0 : 0x7fffffff;
       for (int r0 = 0; r0 < V; r0 += kWave) {
         const int r = r0 + lane;
@@ -1059,6 +1382,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
           sink = out_cnt == 0 ? kRowMetaSink : 0;
           sink |= static_cast<int>(s.base()[v]);
         }
+        max_np = max(max_np, cnt);
         int my_slot = kSlotNone;
         const int n_chunk = min(kWave, V - r0);
         for (int rr = 0; rr < n_chunk; ++rr) {
@@ -1091,6 +1415,8 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
       if (lane == 0) {
         s.pred_off()[V] = run;
       }
+      strip_ok = narrow && L >= 1 && max_abs <= 4096 &&
+                 __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1) <= 255;
       wave_sync();
       for (int r = lane; r < V; r += kWave) {
         const int v = s.sub_order()[r];
@@ -1117,8 +1443,27 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     clk.lap(R, kPhaseColumn0);
     int best_score = 0;
     bool tied = false;
-    int best_row = uniform(narrow ? dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied)
-                                  : dp_layer<int32_t>(P, s, lds, V, L, &best_score, &tied));
+    uint64_t dp;
+    CParams* pp = &P;
+    CLayout* ly = s.L;
+    uint32_t lds_addr = static_cast<uint32_t>((size_t)(LByte*)lds);
+    // opaque to the optimizer: otherwise the callee would rebuild the address
+    // from the dynamic LDS symbol (a table load) inside its loops
+    asm volatile("" : "+v"(lds_addr));
+    if (strip_ok && L <= 4 * kWave) {
+      dp = dp_layer_call<kDpStrip4>(pp, s.p, ly, lds_addr, V, L);
+    } else if (strip_ok && L <= 8 * kWave) {
+      dp = dp_layer_call<kDpStrip8>(pp, s.p, ly, lds_addr, V, L);
+    } else if (strip_ok && L <= 12 * kWave) {
+      dp = dp_layer_call<kDpStrip12>(pp, s.p, ly, lds_addr, V, L);
+    } else if (narrow) {
+      dp = dp_layer_call<kDpGeneric16>(pp, s.p, ly, lds_addr, V, L);
+    } else {
+      dp = dp_layer_call<kDpGeneric32>(pp, s.p, ly, lds_addr, V, L);
+    }
+    int best_row = uniform(static_cast<int>(static_cast<uint32_t>(dp) & 0x7fffffffu));
+    best_score = uniform(static_cast<int>(dp >> 32));
+    tied = uniform(static_cast<int>(static_cast<uint32_t>(dp) >> 31)) != 0;
     wave_sync();
     if (tied) {
       // several sinks share the best score: spoa takes the first one in its
@@ -1149,44 +1494,51 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(L);
     clk.lap(R, kPhaseDp);
 
-    // 4. traceback (lane-uniform), pairs collected backwards
+    // 4. traceback (lane-uniform), pairs collected backwards; a step loads
+    // the cell's code and the row's node together, and a predecessor given
+    // by index costs one more dependent load (pred_off and pred_rows)
     int na = 0;
     {
+      const uint16_t* code = s.code();
+      const int32_t* sub_order = s.sub_order();
+      const int32_t* pred_off = s.pred_off();
+      const int32_t* pred_rows = s.pred_rows();
+      int32_t* aln_node = s.aln_node();
+      int32_t* aln_pos = s.aln_pos();
       int i = best_row, j = L;
       const int aln_cap = static_cast<int>(caps.nodes + caps.cols);
-      while (!(i == 0 && j == 0)) {
+      while (i != 0 || j != 0) {
         if (na >= aln_cap || j < 0 || i < 0) {
           R.status = kPoaBadInput;
           return;
         }
         if (i == 0) {
-          s.aln_node()[na] = -1;
-          s.aln_pos()[na] = j - 1;
+          aln_node[na] = -1;
+          aln_pos[na] = j - 1;
           ++na;
           --j;
           continue;
         }
-        const int c = uniform(s.code()[static_cast<int64_t>(i) * pitch + code_index(j, pitch)]);
+        const int c = uniform(code[static_cast<int64_t>(i) * pitch + code_index(j, pitch)]);
+        const int node = uniform(sub_order[i - 1]);
         const int kind = c & 0xc000;
-        const int k = c & 0x3fff;
-        const int node = uniform(s.sub_order()[i - 1]);
-        if (kind == kCodeDiag) {
-          s.aln_node()[na] = node;
-          s.aln_pos()[na] = j - 1;
-          ++na;
-          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]) & kPredRowMask;
-          --j;
-        } else if (kind == kCodeVert) {
-          s.aln_node()[na] = node;
-          s.aln_pos()[na] = -1;
-          ++na;
-          i = uniform(s.pred_rows()[s.pred_off()[i - 1] + k]) & kPredRowMask;
-        } else {
-          s.aln_node()[na] = -1;
-          s.aln_pos()[na] = j - 1;
+        if (kind == kCodeHorz) {
+          aln_node[na] = -1;
+          aln_pos[na] = j - 1;
           ++na;
           --j;
+          continue;
         }
+        aln_node[na] = node;
+        aln_pos[na] = kind == kCodeDiag ? j - 1 : -1;
+        ++na;
+        const int pay = c & static_cast<int>(kCodePayload);
+        if ((c & kCodeViaK) != 0) {
+          i = uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask;
+        } else {
+          i -= pay;
+        }
+        if (kind == kCodeDiag) --j;
       }
     }
 
@@ -1205,7 +1557,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
       if (rc == 0) {
         n_path = L;
       } else {
-        insert_sequential(s, caps, L, na, &order_ok, &n_path);
+        s = insert_sequential(s, caps, L, na, &order_ok, &n_path);
       }
     }
     if (s.overflow) {
@@ -1404,7 +1756,7 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
     return;
   }
   for (int i = lane; i < out_len; i += kWave) {
-    P.out[W.out_off + i] = s.base()[s.path()[plen - 1 - (begin + i)]];
+    global(P.out)[W.out_off + i] = s.base()[s.path()[plen - 1 - (begin + i)]];
   }
   clk.lap(R, kPhaseConsensus);
   R.length = static_cast<uint32_t>(out_len);
@@ -1417,9 +1769,15 @@ __device__ void process_window(const PoaParams& P, Slot& s, uint8_t* lds, const 
 }  // namespace
 
 // Persistent batch kernel: each wave takes windows from the shared queue.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void poa_batch_kernel(PoaParams P) {
+// The parameters stay in device memory (read-only for the whole launch) and
+// are read through a restrict pointer: uniform scalar loads, and no private
+// copy of the struct (a by-value kernel argument whose address is taken is
+// copied to scratch, with its fields held in vector registers).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_batch_kernel(
+    const PoaParams* __restrict__ params) {
+  CParams& P = *(CParams*)params;
   extern __shared__ uint8_t lds[];
-  uint8_t* slot_mem = P.scratch + static_cast<uint64_t>(blockIdx.x) * P.slot_bytes;
+  GByte* slot_mem = global(P.scratch) + static_cast<uint64_t>(blockIdx.x) * P.slot_bytes;
   __shared__ uint32_t next;
   while (true) {
     if (threadIdx.x == 0) {
@@ -1431,11 +1789,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void po
     if (q >= P.n_order) {
       break;
     }
-    const uint32_t w = P.order[q];
+    const uint32_t w = global(P.order)[q];
     Slot s = carve(slot_mem, P.layout);
-    PoaResult res{};
-    res.status = kPoaNotRun;
-    process_window(P, s, lds, P.windows[w], res);
+    // the window's result accumulates in LDS (phase ticks are read-modify-
+    // written by every lane with the same value), not in registers
+    __shared__ PoaResult res;
+    if (threadIdx.x == 0) {
+      res = PoaResult{};
+      res.status = kPoaNotRun;
+    }
+    __syncthreads();
+    process_window(P, s, lds, ((CWindow*)P.windows)[w], res);
     __syncthreads();
     if (threadIdx.x == 0) {
       P.results[w] = res;
@@ -1445,13 +1809,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void po
 }
 
 // Plain kernel: one launch per window (single wave, window index given).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void poa_single_kernel(PoaParams P, uint32_t q) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_single_kernel(
+    const PoaParams* __restrict__ params, uint32_t q) {
+  CParams& P = *(CParams*)params;
   extern __shared__ uint8_t lds[];
-  const uint32_t w = P.order[q];
-  Slot s = carve(P.scratch, P.layout);
-  PoaResult res{};
-  res.status = kPoaNotRun;
-  process_window(P, s, lds, P.windows[w], res);
+  const uint32_t w = global(P.order)[q];
+  Slot s = carve(global(P.scratch), P.layout);
+  __shared__ PoaResult res;
+  if (threadIdx.x == 0) {
+    res = PoaResult{};
+    res.status = kPoaNotRun;
+  }
+  __syncthreads();
+  process_window(P, s, lds, ((CWindow*)P.windows)[w], res);
   __syncthreads();
   if (threadIdx.x == 0) {
     P.results[w] = res;
@@ -1463,13 +1833,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void po
 namespace rh {
 
 // Host launchers (kept in this translation unit with the kernels).
-hipError_t launch_poa_batch(const PoaParams& params, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL(poa_batch_kernel, dim3(grid), dim3(64), params.lds_bytes, stream, params);
+hipError_t launch_poa_batch(const PoaParams* d_params, uint32_t lds_bytes, uint32_t grid,
+                            hipStream_t stream) {
+  hipLaunchKernelGGL(poa_batch_kernel, dim3(grid), dim3(64), lds_bytes, stream, d_params);
   return hipGetLastError();
 }
 
-hipError_t launch_poa_single(const PoaParams& params, uint32_t q, hipStream_t stream) {
-  hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), params.lds_bytes, stream, params, q);
+hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), lds_bytes, stream, d_params, q);
   return hipGetLastError();
 }
 

@@ -82,6 +82,10 @@ struct PoaSlotCaps {
 
 constexpr uint32_t kMaxAligned = 8;  // aligned-set entries kept per node
 
+// DP row pitch >= layer length + kPoaColPad: strips that end past column L
+// (up to 12 columns, stored from column + 3) stay inside their row.
+constexpr uint32_t kPoaColPad = 24;
+
 // Byte offsets of every array inside one slot.
 struct PoaSlotLayout {
   uint64_t base, in_head, out_head, in_tail, out_tail, n_aligned, aligned, cov,

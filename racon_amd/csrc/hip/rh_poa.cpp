@@ -19,8 +19,10 @@
 
 namespace rh {
 
-hipError_t launch_poa_batch(const PoaParams& params, uint32_t grid, hipStream_t stream);
-hipError_t launch_poa_single(const PoaParams& params, uint32_t q, hipStream_t stream);
+hipError_t launch_poa_batch(const PoaParams* d_params, uint32_t lds_bytes, uint32_t grid,
+                            hipStream_t stream);
+hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
+                             hipStream_t stream);
 hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes);
 
 namespace {
@@ -53,7 +55,7 @@ struct rh_poa_batch {
   uint64_t out_bytes = 0;
 
   rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results, d_order, d_queue,
-      d_scratch;
+      d_scratch, d_params;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool uploaded = false;
@@ -150,14 +152,20 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   p.ring_rows = ring_rows_for(caps.cols, b->tune_ring);
   p.lds_bytes = rh::poa_lds_bytes(caps.cols, p.ring_rows);
 
+  RH_HIP_CHECK(b->d_params.reserve(sizeof(PoaParams)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(hipMemcpyAsync(b->d_params.ptr, &p, sizeof(PoaParams), hipMemcpyHostToDevice,
+                              b->stream),
+               RH_ERR_DEVICE);
+  const PoaParams* dp = b->d_params.as<PoaParams>();
+
   RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
   if (mode == RH_POA_PLAIN) {
     for (uint32_t q = 0; q < order.size(); ++q) {
-      RH_HIP_CHECK(rh::launch_poa_single(p, q, b->stream), RH_ERR_KERNEL);
+      RH_HIP_CHECK(rh::launch_poa_single(dp, p.lds_bytes, q, b->stream), RH_ERR_KERNEL);
       b->stats.launches++;
     }
   } else {
-    RH_HIP_CHECK(rh::launch_poa_batch(p, static_cast<uint32_t>(slots), b->stream),
+    RH_HIP_CHECK(rh::launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(slots), b->stream),
                  RH_ERR_KERNEL);
     b->stats.launches++;
   }
@@ -344,9 +352,9 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
     const uint64_t node_bound = w.backbone_length + sum_len;
     const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
     // DP rows are packed in 24 bits of a predecessor entry and the in-edge
-    // index (at most one in-edge per node and layer) in 14 bits of a move code
+    // index (at most one in-edge per node and layer) in 13 bits of a move code
     if (node_bound >= (1ull << 24) - 1 || edge_bound > 0x7fffffffull ||
-        w.n_layers >= (1u << 14) - 1) {
+        w.n_layers >= (1u << 13) - 2) {
       rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is too large");
       return RH_ERR_ARGUMENT;
     }
@@ -436,7 +444,7 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   }
   caps.nodes = max_nodes;
   caps.edges = max_edges;
-  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols + 8, 16));
+  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols - 1 + rh::kPoaColPad, 16));
   caps.rows = std::min<uint32_t>(max_nodes + 1, b->tune_rows ? b->tune_rows : 4096);
   caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
 
