@@ -45,6 +45,57 @@ CONFIGS = {
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+def queue_report(windows, batch, slots, kernel_ms):
+    """How well the work queue's cost order balances the resident waves.
+
+    Replays the queue (windows in the library's order: estimated cost
+    (backbone + layer bytes) x (longest layer + 1), largest first, each to the
+    first free wave) with the measured per-window device times, and compares
+    its makespan with the same replay in measured-time order and with the
+    perfect split (total / waves)."""
+    import heapq
+    cells, ticks = batch.window_costs()
+    rate = batch.tick_rate()
+    est = []
+    for w in windows:
+        n_bound = len(w.backbone) + sum(len(l.sequence) for l in w.layers)
+        longest = max((len(l.sequence) for l in w.layers), default=0)
+        est.append(n_bound * (longest + 1))
+    ran = [i for i, t in enumerate(ticks) if t > 0]
+    if not ran or not slots:
+        return {}
+    secs = [ticks[i] / rate for i in range(len(ticks))]
+
+    def replay(order):
+        heap = [0.0] * slots
+        for i in order:
+            t = heapq.heappop(heap)
+            heapq.heappush(heap, t + secs[i])
+        return max(heap)
+
+    by_est = sorted(ran, key=lambda i: -est[i])
+    by_time = sorted(ran, key=lambda i: -secs[i])
+    total = sum(secs[i] for i in ran)
+    srt = sorted(secs[i] for i in ran)
+    rep = {
+        "kernel_s": round(kernel_ms / 1e3, 4),
+        "replay_cost_order_s": round(replay(by_est), 4),
+        "replay_time_order_s": round(replay(by_time), 4),
+        "perfect_split_s": round(total / slots, 4),
+        "window_s_mean": round(total / len(ran), 4),
+        "window_s_p99": round(srt[int(0.99 * (len(srt) - 1))], 4),
+        "window_s_max": round(srt[-1], 4),
+    }
+    log("queue: " + ", ".join(f"{k} {v}" for k, v in rep.items()))
+    for i in by_time[:5]:
+        w = windows[i]
+        lens = [len(l.sequence) for l in w.layers]
+        log(f"  heavy window {i}: {secs[i]:.3f} s, {cells[i] / 1e6:.1f} M cells, "
+            f"{len(lens)} layers, backbone {len(w.backbone)}, layer mean "
+            f"{sum(lens) / max(1, len(lens)):.0f} max {max(lens, default=0)}")
+    return rep
+
+
 
 def prepare(cfg_id: int, workdir: str, threads: int):
     """Generates the config's inputs and dumps windows/overlaps with the CLI."""
@@ -163,6 +214,7 @@ def main() -> int:
     # end of the queue (load imbalance) or between windows
     busy_frac = total_phase / (st.slots * (st.kernel_ms / 1e3)) if st.slots else 0.0
     log(f"wave busy fraction {busy_frac:.3f} ({st.slots} resident waves)")
+    sched = queue_report(windows, batch, st.slots, st.kernel_ms)
     gpu_out, gpu_status = batch.download()
     n_windows = len(windows)
     cells = st.cells
@@ -258,6 +310,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "poa_phase_share": {k: round(v / total_phase, 4) for k, v in phases.items()},
         "poa_wave_busy_frac": round(busy_frac, 4),
+        "poa_queue": sched,
         "poa_resident_waves": st.slots,
         "autotune_ms": {str(k): round(v, 1) for k, v in tuned.items()},
         "ring_rows": best_ring,

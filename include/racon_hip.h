@@ -141,6 +141,13 @@ int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows,
 int32_t rh_poa_phase_ticks(const rh_poa_batch* batch, uint64_t* ticks, uint32_t n,
                            uint64_t* ticks_per_second);
 
+/* Profiling: per window of the last run, in upload order, the DP cells and
+ * the device clock ticks its wave spent on it (both 0 for windows that need
+ * no kernel: fewer than 3 sequences); n <= the uploaded window count. For
+ * studying the work queue's cost order. No reference counterpart. */
+int32_t rh_poa_window_costs(const rh_poa_batch* batch, uint64_t* cells, uint64_t* ticks,
+                            uint64_t n);
+
 /* ---- (a) overlap alignment ---- */
 
 /* One pair in edlib's orientation: query = read span (reverse-complemented

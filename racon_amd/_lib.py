@@ -104,6 +104,8 @@ _SIGNATURES = {
                                            ctypes.c_uint32, ctypes.c_uint32]),
     "rh_poa_phase_ticks": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    "rh_poa_window_costs": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
+                                             ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]),
     "rh_align_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "rh_align_destroy": (None, [ctypes.c_void_p]),
     "rh_align_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
@@ -234,6 +236,22 @@ class PoaBatch:
         _check(self.lib.rh_poa_phase_ticks(self.handle, ticks, len(self.PHASES),
                                            ctypes.byref(rate)), "rh_poa_phase_ticks")
         return {k: ticks[i] / max(1, rate.value) for i, k in enumerate(self.PHASES)}
+
+    def tick_rate(self) -> int:
+        """Device constant-clock ticks per second."""
+        ticks = (ctypes.c_uint64 * len(self.PHASES))()
+        rate = ctypes.c_uint64()
+        _check(self.lib.rh_poa_phase_ticks(self.handle, ticks, len(self.PHASES),
+                                           ctypes.byref(rate)), "rh_poa_phase_ticks")
+        return max(1, rate.value)
+
+    def window_costs(self):
+        """Per uploaded window of the last run: (DP cells, device ticks)."""
+        cells = (ctypes.c_uint64 * self.n)()
+        ticks = (ctypes.c_uint64 * self.n)()
+        _check(self.lib.rh_poa_window_costs(self.handle, cells, ticks, self.n),
+               "rh_poa_window_costs")
+        return list(cells), list(ticks)
 
     def stats(self) -> RhPoaStats:
         st = RhPoaStats()

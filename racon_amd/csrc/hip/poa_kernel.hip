@@ -122,8 +122,6 @@ struct Slot {
   __device__ int32_t* row_meta() const { return (int32_t*)(p + L->row_meta); }
   __device__ int32_t* H() const { return (int32_t*)(p + L->H); }
   __device__ uint16_t* code() const { return (uint16_t*)(p + L->code); }
-  __device__ int32_t* aln_node() const { return (int32_t*)(p + L->aln_node); }
-  __device__ int32_t* aln_pos() const { return (int32_t*)(p + L->aln_pos); }
   __device__ int64_t* scores() const { return (int64_t*)(p + L->scores); }
   __device__ int32_t* preds() const { return (int32_t*)(p + L->preds); }
   __device__ int32_t* path() const { return (int32_t*)(p + L->path); }
@@ -132,6 +130,7 @@ struct Slot {
   __device__ int32_t* anchor() const { return (int32_t*)(p + L->anchor); }
   __device__ int32_t* lpath() const { return (int32_t*)(p + L->lpath); }
   __device__ int32_t* sink_score() const { return (int32_t*)(p + L->sink_score); }
+  __device__ int32_t* tie_mark() const { return (int32_t*)(p + L->tie_mark); }
 
   int32_t n_nodes;
   int32_t n_edges;
@@ -209,19 +208,6 @@ __device__ __forceinline__ void add_edge(Slot& s, CCaps& caps, int tail, int hea
   s.in_tail()[head] = e;
   touch(s, tail, s.n_seqs);
   touch(s, head, s.n_seqs);
-}
-
-// Chain of new nodes for seq[begin, end); returns the first node or -1.
-__device__ __forceinline__ int add_chain(Slot& s, CCaps& caps, int begin, int end) {
-  if (begin >= end) {
-    return -1;
-  }
-  const int first = add_node(s, caps, s.seq()[begin]);
-  for (int i = begin + 1; i < end && !s.overflow; ++i) {
-    const int v = add_node(s, caps, s.seq()[i]);
-    add_edge(s, caps, v - 1, v, s.qual()[i - 1] + s.qual()[i]);
-  }
-  return first;
 }
 
 // DFS topological order over nodes with keep[v] (all nodes when keep is
@@ -651,21 +637,16 @@ __device__ __forceinline__ void load_strip(const int16_t* ring, int pitch, const
 // (key or score c0, code vertical). Returns H[row][L].
 template <int W, bool kKeys>
 __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* bv, int32_t c0,
-                                              uint32_t pay0, uint32_t pay1, uint32_t pay2,
-                                              uint32_t pay3, bool by_table, int lane,
-                                              int j0, bool live, int L, int32_t gap, int own,
-                                              int pitch, int16_t* ring, int32_t* Hg,
-                                              uint16_t* Cr, int end_lane, int end_u) {
+                                              uint32_t pay0, int lane, int j0, bool live,
+                                              int L, int32_t gap, int own, int pitch,
+                                              int16_t* ring, int32_t* Hg, uint16_t* Cr,
+                                              int end_lane, int end_u) {
   auto score = [](int32_t x) { return kKeys ? (x >> 8) : x; };
-  // code payload of the move's predecessor: payk (row delta) for the
-  // first four predecessors when by_table, else kCodeViaK | k
+  // code payload of the move's predecessor: the row's single predecessor
+  // (pay0: row delta or kCodeViaK), else kCodeViaK | k from the key
   auto pred = [&](int32_t x) -> uint32_t {
     if (!kKeys) return pay0;
-    const uint32_t k = 255u - (static_cast<uint32_t>(x) & 255u);
-    if (!by_table) return kCodeViaK | k;
-    const uint32_t v01 = k == 1u ? pay1 : pay0;
-    const uint32_t v23 = k == 3u ? pay3 : pay2;
-    return k >= 2u ? v23 : v01;
+    return kCodeViaK | (255u - (static_cast<uint32_t>(x) & 255u));
   };
   const int32_t h0 = score(c0);
   int32_t pm = lane == 0 ? h0 : kNegInf;
@@ -789,17 +770,10 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
       };
       int32_t bd[W], bv[W];
       int32_t h_last;
-      // predecessor payloads of the move codes: row deltas of the first
-      // four predecessors (when every row delta fits the payload)
-      auto delta = [&](int k) -> uint32_t {
-        return k < np ? static_cast<uint32_t>(r - (pred_at(k) & kPredRowMask)) : 0u;
-      };
-      uint32_t pay0 = delta(0), pay1 = delta(1), pay2 = delta(2), pay3 = delta(3);
-      const bool by_table = np <= 4 && max(max(pay0, pay1), max(pay2, pay3)) <= kCodePayload;
-      if (np == 1 && !by_table) {
-        pay0 = kCodeViaK;
-      }
       if (np == 1) {
+        // move-code payload of the single predecessor: its row delta
+        const uint32_t delta = static_cast<uint32_t>(r - (pred_at(0) & kPredRowMask));
+        const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
         int32_t hp[W + 1];
         load_strip<W>(ring, pitch, s, pred_at(0), j0, jr, L, gap, hp);
 #pragma unroll
@@ -807,9 +781,8 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
           bd[u] = hp[u] + sc(u);
           bv[u] = hp[u + 1] + gap;
         }
-        h_last = finish_row<W, false>(bd, bv, hp[0] + gap, pay0, pay1, pay2, pay3, true, lane,
-                                      j0, live, L, gap, own, pitch, ring, Hg, Cr, end_lane,
-                                      end_u);
+        h_last = finish_row<W, false>(bd, bv, hp[0] + gap, pay0, lane, j0, live, L, gap, own,
+                                      pitch, ring, Hg, Cr, end_lane, end_u);
       } else {
         int32_t c0k = kNegKey;
 #pragma unroll
@@ -828,8 +801,8 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
           }
           c0k = max(c0k, (hp[0] + gap) * 256 | tag);
         }
-        h_last = finish_row<W, true>(bd, bv, c0k, pay0, pay1, pay2, pay3, by_table, lane, j0,
-                                     live, L, gap, own, pitch, ring, Hg, Cr, end_lane, end_u);
+        h_last = finish_row<W, true>(bd, bv, c0k, 0u, lane, j0, live, L, gap, own, pitch, ring,
+                                     Hg, Cr, end_lane, end_u);
       }
       if ((meta & kRowMetaSink) != 0) {
         if (lane == 0) {
@@ -900,122 +873,96 @@ __device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLa
 }
 
 // Graph insertion of one layer (spoa add_alignment), position by position,
-// lane-uniform. The alignment pairs are stored backwards (aln_*[0] is the
-// last pair). New nodes record their anchor rank for the incremental order.
-// (Rare: two path nodes share an aligned set. Out of line, returns the slot
-// with its updated counters.)
-__device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, int na,
-                                              bool* order_ok_out, int* n_path_out) {
+// lane-uniform, from the aligned node of every layer position (pos_node,
+// -1 for an insertion). New nodes record their anchor rank for the
+// incremental order. (Rare: two path nodes share an aligned set. Out of
+// line; returns the slot with its updated counters.)
+__device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool* order_ok_out,
+                                              int* n_path_out) {
   bool order_ok = true;
   int n_path = 0;
-  {
-    int first_pos = -1, last_pos = -1;
-    for (int q = na - 1; q >= 0; --q) {
-      if (s.aln_pos()[q] != -1) {
-        first_pos = s.aln_pos()[q];
-        break;
+  const int32_t* pos_node = s.path();
+  const int before = s.n_nodes;
+  int head = -1;
+  int prev_w = 0;
+  int head_anchor = -1;  // anchor of the previous path node when it is new
+  for (int pos = 0; pos < L && !s.overflow; ++pos) {
+    const int node = uniform(pos_node[pos]);
+    const uint8_t letter = s.seq()[pos];
+    int nv;
+    if (node == -1) {
+      nv = add_node(s, caps, letter);
+    } else if (s.base()[node] == letter) {
+      nv = node;
+    } else {
+      int same = -1;
+      const int cnt = s.n_aligned()[node];
+      for (int k = 0; k < cnt; ++k) {
+        const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+        if (s.base()[a] == letter) {
+          same = a;
+          break;
+        }
       }
-    }
-    for (int q = 0; q < na; ++q) {
-      if (s.aln_pos()[q] != -1) {
-        last_pos = s.aln_pos()[q];
-        break;
-      }
-    }
-    const int before = s.n_nodes;
-    add_chain(s, caps, 0, first_pos);
-    int head = before == s.n_nodes ? -1 : s.n_nodes - 1;
-    const int tail = add_chain(s, caps, last_pos + 1, L);
-    if (s.n_nodes != before) {
-      order_ok = false;  // chains outside the alignment (not produced by NW)
-    }
-    int prev_w = head == -1 ? 0 : s.qual()[first_pos - 1];
-    int head_anchor = -1;  // anchor of the previous path node when it is new
-    for (int q = na - 1; q >= 0 && !s.overflow; --q) {
-      const int pos = uniform(s.aln_pos()[q]);
-      if (pos == -1) {
-        continue;
-      }
-      const int node = uniform(s.aln_node()[q]);
-      const uint8_t letter = s.seq()[pos];
-      int nv;
-      if (node == -1) {
+      if (same == -1) {
         nv = add_node(s, caps, letter);
-      } else if (s.base()[node] == letter) {
-        nv = node;
+        if (s.overflow || cnt + 1 > static_cast<int>(kMaxAligned)) {
+          s.overflow = true;
+          break;
+        }
+        for (int k = 0; k < cnt; ++k) {
+          const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+          const int ca = s.n_aligned()[a];
+          s.aligned()[nv * kMaxAligned + k] = a;
+          s.aligned()[a * kMaxAligned + ca] = nv;
+          s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
+        }
+        s.aligned()[nv * kMaxAligned + cnt] = node;
+        s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
+        s.aligned()[node * kMaxAligned + cnt] = nv;
+        s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
       } else {
-        int same = -1;
+        nv = same;
+      }
+    }
+    if (nv >= before) {
+      // a new node: placed after its path predecessor's aligned set
+      // (mismatch nodes: after the aligned set they join)
+      int anchor;
+      if (node != -1) {
+        anchor = s.node_rank()[node];
         const int cnt = s.n_aligned()[node];
         for (int k = 0; k < cnt; ++k) {
           const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-          if (s.base()[a] == letter) {
-            same = a;
-            break;
-          }
+          if (a < before) anchor = max(anchor, s.node_rank()[a]);
         }
-        if (same == -1) {
-          nv = add_node(s, caps, letter);
-          if (s.overflow || cnt + 1 > static_cast<int>(kMaxAligned)) {
-            s.overflow = true;
-            break;
-          }
-          for (int k = 0; k < cnt; ++k) {
-            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-            const int ca = s.n_aligned()[a];
-            s.aligned()[nv * kMaxAligned + k] = a;
-            s.aligned()[a * kMaxAligned + ca] = nv;
-            s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
-          }
-          s.aligned()[nv * kMaxAligned + cnt] = node;
-          s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
-          s.aligned()[node * kMaxAligned + cnt] = nv;
-          s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
-        } else {
-          nv = same;
+      } else if (head == -1) {
+        anchor = -1;
+      } else if (head >= before) {
+        anchor = head_anchor;
+      } else {
+        anchor = s.node_rank()[head];
+        const int cnt = s.n_aligned()[head];
+        for (int k = 0; k < cnt; ++k) {
+          const int a = uniform(s.aligned()[head * kMaxAligned + k]);
+          if (a < before) anchor = max(anchor, s.node_rank()[a]);
         }
       }
-      if (nv >= before) {
-        // a new node: placed after its path predecessor's aligned set
-        // (mismatch nodes: after the aligned set they join)
-        int anchor;
-        if (node != -1) {
-          anchor = s.node_rank()[node];
-          const int cnt = s.n_aligned()[node];
-          for (int k = 0; k < cnt; ++k) {
-            const int a = uniform(s.aligned()[node * kMaxAligned + k]);
-            if (a < before) anchor = max(anchor, s.node_rank()[a]);
-          }
-        } else if (head == -1) {
-          anchor = -1;
-        } else if (head >= before) {
-          anchor = head_anchor;
-        } else {
-          anchor = s.node_rank()[head];
-          const int cnt = s.n_aligned()[head];
-          for (int k = 0; k < cnt; ++k) {
-            const int a = uniform(s.aligned()[head * kMaxAligned + k]);
-            if (a < before) anchor = max(anchor, s.node_rank()[a]);
-          }
-        }
-        const int k_new = nv - before;
-        if (k_new > 0 && anchor < s.anchor()[k_new - 1]) {
-          order_ok = false;
-        }
-        s.anchor()[k_new] = anchor;
-        head_anchor = anchor;
+      const int k_new = nv - before;
+      if (k_new > 0 && anchor < s.anchor()[k_new - 1]) {
+        order_ok = false;
       }
-      s.lpath()[n_path++] = nv;
-      if (head != -1) {
-        add_edge(s, caps, head, nv, prev_w + s.qual()[pos]);
-      }
-      head = nv;
-      prev_w = s.qual()[pos];
+      s.anchor()[k_new] = anchor;
+      head_anchor = anchor;
     }
-    if (tail != -1 && !s.overflow) {
-      add_edge(s, caps, head, tail, prev_w + s.qual()[last_pos + 1]);
+    s.lpath()[n_path++] = nv;
+    if (head != -1) {
+      add_edge(s, caps, head, nv, prev_w + s.qual()[pos]);
     }
-    s.n_seqs = s.n_seqs + 1;
+    head = nv;
+    prev_w = s.qual()[pos];
   }
+  s.n_seqs = s.n_seqs + 1;
   *order_ok_out = order_ok;
   *n_path_out = n_path;
   return s;
@@ -1031,18 +978,13 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, int n
 // (the rank bound of the last non-insertion position before it) is a wave
 // prefix scan. Returns 1 when two path nodes share an aligned set (nothing
 // in the graph has changed then), 0 when inserted, or a negative status.
-__device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8_t* lseq, int L, int na,
-                               bool* order_ok) {
+__device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8_t* lseq, int L,
+                                               bool* order_ok) {
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  int32_t* pnode = s.path();   // aligned graph node of each layer position (-1: none)
-  int32_t* pnv = s.lpath();    // graph node of each layer position after insertion
-  uint8_t* stamp = s.in_sub(); // 2 on the path's nodes (the DP no longer needs it)
-  for (int q = lane; q < na; q += kWave) {
-    const int pos = s.aln_pos()[q];
-    if (pos != -1) pnode[pos] = s.aln_node()[q];
-  }
-  wave_sync();
+  const int32_t* pnode = s.path();  // aligned graph node of each layer position (-1: none)
+  int32_t* pnv = s.lpath();         // graph node of each layer position after insertion
+  uint8_t* stamp = s.in_sub();      // 2 on the path's nodes (the DP no longer needs it)
   for (int p = lane; p < L; p += kWave) {
     const int v = pnode[p];
     if (v != -1) stamp[v] = 2;
@@ -1205,6 +1147,207 @@ __device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8
   return 0;
 }
 
+// DP rows of a layer (row r + 1 is node sub_order[r]): predecessor rows in
+// in-edge order (row 0, the virtual start, for rows without one), sink flags,
+// and an LDS slot for every row that a later row reads: the lowest slot whose
+// previous row is no longer needed (lane k < n_slots holds the last row that
+// reads slot k). With kFull every node is in the layer's subgraph and the
+// membership checks are skipped. Returns the largest predecessor count.
+// Out of line, like the DP, for a register allocation of its own.
+template <bool kFull>
+__device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V, int n_slots) {
+  const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
+  V = uniform(V);
+  n_slots = uniform(n_slots);
+  const int lane = lane_id();
+  const int32_t* sub_order = s.sub_order();
+  const int32_t* in_head = s.in_head();
+  const int32_t* out_head = s.out_head();
+  const int32_t* e_next_in = s.e_next_in();
+  const int32_t* e_next_out = s.e_next_out();
+  const int32_t* e_tail = s.e_tail();
+  const int32_t* e_head = s.e_head();
+  const uint8_t* in_sub = s.in_sub();
+  const uint8_t* base = s.base();
+  int32_t* row_of = s.row_of();
+  int32_t* row_meta = s.row_meta();
+  int32_t* pred_off = s.pred_off();
+  int32_t* pred_rows = s.pred_rows();
+  auto included = [&](int v) { return kFull || in_sub[v] != 0; };
+  int run = 0;
+  int max_np = 0;
+  int busy = lane < n_slots ? 0 : 0x7fffffff;
+  for (int r0 = 0; r0 < V; r0 += kWave) {
+    const int r = r0 + lane;
+    int cnt = 0;
+    int last_use = 0;
+    int meta = 0;
+    if (r < V) {
+      const int v = sub_order[r];
+      for (int e = in_head[v]; e != -1; e = e_next_in[e]) {
+        if (included(e_tail[e])) ++cnt;
+      }
+      cnt = max(cnt, 1);  // the virtual start row
+      int out_cnt = 0;
+      for (int e = out_head[v]; e != -1; e = e_next_out[e]) {
+        const int h = e_head[e];
+        if (included(h)) {
+          ++out_cnt;
+          last_use = max(last_use, row_of[h]);
+        }
+      }
+      meta = (out_cnt == 0 ? kRowMetaSink : 0) | static_cast<int>(base[v]);
+    }
+    max_np = max(max_np, cnt);
+    int my_slot = kSlotNone;
+    const int n_chunk = min(kWave, V - r0);
+    for (int rr = 0; rr < n_chunk; ++rr) {
+      const int lu = __builtin_amdgcn_readlane(last_use, rr);
+      if (lu > 0) {
+        const int row = r0 + rr + 1;
+        const unsigned long long free_mask = __ballot(busy < row);
+        int sl = kSlotGlobal;
+        if (free_mask != 0) {
+          sl = __ffsll(static_cast<long long>(free_mask)) - 1;
+          if (lane == sl) busy = lu;
+        }
+        if (lane == rr) my_slot = sl;
+      }
+    }
+    if (r < V) {
+      row_meta[r] = meta | (my_slot << kRowMetaSlotShift);
+    }
+    // inclusive scan of cnt
+    int x = cnt;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int y = __shfl_up(x, d, kWave);
+      if (lane >= d) x += y;
+    }
+    if (r < V) {
+      pred_off[r] = run + x - cnt;
+    }
+    run += __shfl(x, kWave - 1, kWave);
+  }
+  if (lane == 0) {
+    pred_off[V] = run;
+  }
+  wave_sync();
+  for (int r = lane; r < V; r += kWave) {
+    const int v = sub_order[r];
+    int o = pred_off[r];
+    bool any = false;
+    for (int e = in_head[v]; e != -1; e = e_next_in[e]) {
+      const int u = e_tail[e];
+      if (included(u)) {
+        const int pr = row_of[u];
+        const int sl = (row_meta[pr - 1] >> kRowMetaSlotShift) & 31;
+        pred_rows[o++] = pr | (sl << kPredRowBits);
+        any = true;
+      }
+    }
+    if (!any) {
+      pred_rows[o] = 0;
+    }
+  }
+  return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
+}
+
+// End cell among tied sinks: spoa takes the first tied sink in its DFS order
+// of the (sub)graph (roots in id order; a node's predecessors first; a lead
+// node pushes its aligned set and emits it after itself, in aligned-list
+// order). A root's DFS reaches exactly the nodes from which it can be
+// reached along edges and aligned sets, so the aligned set G of a tied sink
+// is first reached in the root iteration r = the smallest id among G and
+// everything reachable from G (out-edges and aligned sets). When r is itself
+// in G, r leads G and G is emitted as r followed by r's aligned list. Tied
+// sinks are then ordered by (r, position after r) without the DFS: this is
+// exact for sets of sinks (then r is the smallest member) and for sets whose
+// descendants all have larger ids. Returns the first tied sink's row, or -1
+// when some tied sink's r lies outside its set or its search outgrows the
+// bound (the caller then runs the DFS).
+__device__ __forceinline__ int sink_tie_by_sets(const Slot& s, int V, bool full, int best,
+                                                int stack_cap) {
+  constexpr int kReachCap = 256;  // nodes one search may visit
+  const int lane = lane_id();
+  const uint8_t* in_sub = s.in_sub();
+  const int32_t* sub_order = s.sub_order();
+  const int32_t* row_meta = s.row_meta();
+  const int32_t* sink_score = s.sink_score();
+  const uint8_t* n_aligned = s.n_aligned();
+  const int32_t* aligned = s.aligned();
+  const int32_t* out_head = s.out_head();
+  const int32_t* e_next_out = s.e_next_out();
+  const int32_t* e_head = s.e_head();
+  int32_t* mark = s.tie_mark();
+  int32_t* stack = s.stack();
+  if (stack_cap < kReachCap) {
+    return -1;
+  }
+  auto kept = [&](int v) { return full || in_sub[v] != 0; };
+  int best_key = 0x7fffffff, best_at = -1, searches = 0;
+  for (int r0 = 1; r0 <= V; r0 += kWave) {
+    const int row = r0 + lane;
+    unsigned long long cand = __ballot(row <= V && (row_meta[row - 1] & kRowMetaSink) != 0 &&
+                                       sink_score[row] == best);
+    while (cand != 0) {
+      // one tied sink at a time (the searches share the visit marks)
+      const int l = __ffsll(static_cast<long long>(cand)) - 1;
+      cand &= cand - 1;
+      const int crow = r0 + l;
+      const int v = uniform(sub_order[crow - 1]);
+      if (++searches >= 256) return -1;
+      const int tag = s.n_seqs << 8 | searches;  // unique per layer and search
+      // smallest id reachable from v's aligned set (the set included)
+      int top = 0, visited = 0, r = v;
+      mark[v] = tag;
+      stack[top++] = v;
+      while (top > 0) {
+        const int x = uniform(stack[--top]);
+        r = min(r, x);
+        if (++visited > kReachCap) return -1;
+        for (int e = uniform(out_head[x]); e != -1; e = uniform(e_next_out[e])) {
+          const int h = uniform(e_head[e]);
+          if (kept(h) && uniform(mark[h]) != tag) {
+            if (top >= kReachCap) return -1;
+            mark[h] = tag;
+            stack[top++] = h;
+          }
+        }
+        const int na = n_aligned[x];
+        for (int k = 0; k < na; ++k) {
+          const int a = uniform(aligned[x * kMaxAligned + k]);
+          if (kept(a) && uniform(mark[a]) != tag) {
+            if (top >= kReachCap) return -1;
+            mark[a] = tag;
+            stack[top++] = a;
+          }
+        }
+      }
+      // r must lead v's set: r == v or r in v's aligned list
+      int pos = 0;
+      if (r != v) {
+        bool member = false;
+        const int nv = n_aligned[v];
+        for (int k = 0; k < nv; ++k) member |= aligned[v * kMaxAligned + k] == r;
+        if (!member) return -1;
+        const int nr = n_aligned[r];
+        for (int k = 0; k < nr; ++k) {
+          const int a = uniform(aligned[r * kMaxAligned + k]);
+          if (!kept(a)) continue;
+          ++pos;
+          if (a == v) break;
+        }
+      }
+      const int key = r * 16 + pos;
+      if (key < best_key) {
+        best_key = key;
+        best_at = crow;
+      }
+    }
+  }
+  return best_at;
+}
+
 // ---- the window ----
 
 __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds, CWindow& W,
@@ -1247,6 +1390,10 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
   s.n_nodes = n;
   s.n_edges = n > 0 ? n - 1 : 0;
   s.n_seqs = 1;
+  // visit marks of the tie resolution: (layer, lane) tags, none set yet
+  for (int v = lane; v < static_cast<int>(W.node_bound); v += kWave) {
+    s.tie_mark()[v] = 0;
+  }
   wave_sync();
   clk.lap(R, kPhaseSetup);
 
@@ -1349,93 +1496,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       s.row_of()[s.sub_order()[r]] = r + 1;
     }
     wave_sync();
-    // predecessor counts per row (lane-parallel), then offsets; each row that
-    // a later row reads gets the lowest LDS slot whose previous row is no
-    // longer needed (lane k < n_slots holds the last row reading slot k)
-    {
-      int run = 0;
-      int max_np = 0;
-      int busy = lane < n_slots ? 0 : 0x7fffffff;
-      for (int r0 = 0; r0 < V; r0 += kWave) {
-        const int r = r0 + lane;
-        int cnt = 0;
-        int last_use = 0;
-        int sink = 0;
-        if (r < V) {
-          const int v = s.sub_order()[r];
-          for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
-            if (s.in_sub()[s.e_tail()[e]] != 0) {
-              ++cnt;
-            }
-          }
-          if (cnt == 0) {
-            cnt = 1;  // the virtual start row
-          }
-          int out_cnt = 0;
-          for (int e = s.out_head()[v]; e != -1; e = s.e_next_out()[e]) {
-            const int h = s.e_head()[e];
-            if (s.in_sub()[h] != 0) {
-              ++out_cnt;
-              last_use = max(last_use, s.row_of()[h]);
-            }
-          }
-          sink = out_cnt == 0 ? kRowMetaSink : 0;
-          sink |= static_cast<int>(s.base()[v]);
-        }
-        max_np = max(max_np, cnt);
-        int my_slot = kSlotNone;
-        const int n_chunk = min(kWave, V - r0);
-        for (int rr = 0; rr < n_chunk; ++rr) {
-          const int lu = __builtin_amdgcn_readlane(last_use, rr);
-          if (lu > 0) {
-            const int row = r0 + rr + 1;
-            const unsigned long long free_mask = __ballot(busy < row);
-            int sl = kSlotGlobal;
-            if (free_mask != 0) {
-              sl = __ffsll(static_cast<long long>(free_mask)) - 1;
-              if (lane == sl) busy = lu;
-            }
-            if (lane == rr) my_slot = sl;
-          }
-        }
-        if (r < V) {
-          s.row_meta()[r] = sink | (my_slot << kRowMetaSlotShift);
-        }
-        // inclusive scan of cnt
-        int x = cnt;
-        for (int d = 1; d < kWave; d <<= 1) {
-          int y = __shfl_up(x, d, kWave);
-          if (lane >= d) x += y;
-        }
-        if (r < V) {
-          s.pred_off()[r] = run + x - cnt;
-        }
-        run += __shfl(x, kWave - 1, kWave);
-      }
-      if (lane == 0) {
-        s.pred_off()[V] = run;
-      }
-      strip_ok = narrow && L >= 1 && max_abs <= 4096 &&
-                 __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1) <= 255;
-      wave_sync();
-      for (int r = lane; r < V; r += kWave) {
-        const int v = s.sub_order()[r];
-        int o = s.pred_off()[r];
-        bool any = false;
-        for (int e = s.in_head()[v]; e != -1; e = s.e_next_in()[e]) {
-          const int u = s.e_tail()[e];
-          if (s.in_sub()[u] != 0) {
-            const int pr = s.row_of()[u];
-            const int sl = (s.row_meta()[pr - 1] >> kRowMetaSlotShift) & 31;
-            s.pred_rows()[o++] = pr | (sl << kPredRowBits);
-            any = true;
-          }
-        }
-        if (!any) {
-          s.pred_rows()[o] = 0;
-        }
-      }
-    }
+    const int max_np = uniform(full ? build_rows<true>(s.p, s.L, V, n_slots)
+                                    : build_rows<false>(s.p, s.L, V, n_slots));
+    strip_ok = narrow && L >= 1 && max_abs <= 4096 && max_np <= 255;
     wave_sync();
 
     clk.lap(R, kPhaseRows);
@@ -1466,6 +1529,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     tied = uniform(static_cast<int>(static_cast<uint32_t>(dp) >> 31)) != 0;
     wave_sync();
     if (tied) {
+      best_row = uniform(sink_tie_by_sets(s, V, full, best_score, static_cast<int>(caps.stack)));
+    }
+    if (tied && best_row < 0) {
       // several sinks share the best score: spoa takes the first one in its
       // DFS order of the (sub)graph
       const int cnt = topo_sort(s, full ? nullptr : s.in_sub(), s.path(),
@@ -1494,53 +1560,76 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(L);
     clk.lap(R, kPhaseDp);
 
-    // 4. traceback (lane-uniform), pairs collected backwards; a step loads
-    // the cell's code and the row's node together, and a predecessor given
-    // by index costs one more dependent load (pred_off and pred_rows)
-    int na = 0;
+    // 4. traceback, from the end cell to (0, 0). The alignment of layer
+    // position p (the graph node of a diagonal move into column p + 1, or -1
+    // for a horizontal move) goes to pos_node[p]; vertical moves align no
+    // layer position. The wave loads the 8 x 8 block of codes at and before
+    // the current cell (and the nodes of its 8 rows) in one round trip and
+    // follows the path through it with lane reads until the path leaves it;
+    // the block's positions are collected in one register (lane t: position
+    // j0 - 1 - t) and stored together.
     {
       const uint16_t* code = s.code();
       const int32_t* sub_order = s.sub_order();
       const int32_t* pred_off = s.pred_off();
       const int32_t* pred_rows = s.pred_rows();
-      int32_t* aln_node = s.aln_node();
-      int32_t* aln_pos = s.aln_pos();
+      int32_t* pos_node = s.path();
       int i = best_row, j = L;
-      const int aln_cap = static_cast<int>(caps.nodes + caps.cols);
-      while (i != 0 || j != 0) {
-        if (na >= aln_cap || j < 0 || i < 0) {
-          R.status = kPoaBadInput;
-          return;
+      int steps = 0;
+      const int step_cap = uniform(V + L + 1);  // every move lowers i or j
+      bool bad = false;
+      while (i > 0 && !bad) {
+        i = uniform(i);
+        j = uniform(j);
+        const int i0 = i, j0 = j;
+        const int ri = i0 - (lane >> 3), cj = j0 - (lane & 7);
+        const int cw = ri >= 1 && cj >= 0
+                           ? code[static_cast<int64_t>(ri) * pitch + code_index(cj, pitch)]
+                           : 0;
+        const int nw = lane < 8 && i0 - lane >= 1 ? sub_order[i0 - lane - 1] : -1;
+        wait_stores();  // the block is in registers before the walk starts
+        int out = -1;   // lane t: the node of position j0 - 1 - t
+        while (i > 0 && i0 - i < 8 && j0 - j < 8) {
+          i = uniform(i);  // keeps the walk in scalar registers and branches
+          j = uniform(j);
+          steps = uniform(steps);
+          const int c = __builtin_amdgcn_readlane(cw, (i0 - i) * 8 + (j0 - j));
+          const int kind = c & 0xc000;
+          if (++steps > step_cap || (kind != kCodeVert && j == 0)) {
+            bad = true;  // only vertical moves leave column 0
+            break;
+          }
+          if (kind == kCodeHorz) {
+            --j;
+            continue;
+          }
+          if (kind == kCodeDiag) {
+            const int node = __builtin_amdgcn_readlane(nw, i0 - i);
+            out = lane == j0 - j ? node : out;
+            --j;
+          }
+          const int pay = c & static_cast<int>(kCodePayload);
+          if ((c & kCodeViaK) != 0) {
+            i = uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask;
+          } else {
+            i -= pay;
+          }
+          bad = i < 0;
         }
-        if (i == 0) {
-          aln_node[na] = -1;
-          aln_pos[na] = j - 1;
-          ++na;
-          --j;
-          continue;
+        if (lane < j0 - j) {
+          pos_node[j0 - 1 - lane] = out;
         }
-        const int c = uniform(code[static_cast<int64_t>(i) * pitch + code_index(j, pitch)]);
-        const int node = uniform(sub_order[i - 1]);
-        const int kind = c & 0xc000;
-        if (kind == kCodeHorz) {
-          aln_node[na] = -1;
-          aln_pos[na] = j - 1;
-          ++na;
-          --j;
-          continue;
-        }
-        aln_node[na] = node;
-        aln_pos[na] = kind == kCodeDiag ? j - 1 : -1;
-        ++na;
-        const int pay = c & static_cast<int>(kCodePayload);
-        if ((c & kCodeViaK) != 0) {
-          i = uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask;
-        } else {
-          i -= pay;
-        }
-        if (kind == kCodeDiag) --j;
+      }
+      if (bad) {
+        R.status = kPoaBadInput;
+        return;
+      }
+      // from the virtual start row only horizontal moves remain
+      for (int p = lane; p < j; p += kWave) {
+        pos_node[p] = -1;
       }
     }
+    wave_sync();
 
     clk.lap(R, kPhaseTraceback);
     // 5. insert the layer: lane-parallel over layer positions unless two
@@ -1549,7 +1638,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     int n_path = 0;
     const int nodes_before = s.n_nodes;
     {
-      const int rc = insert_parallel(s, caps, lds_seq, L, na, &order_ok);
+      const int rc = insert_parallel(s, caps, lds_seq, L, &order_ok);
       if (rc < 0) {
         R.status = rc;
         return;
@@ -1557,7 +1646,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       if (rc == 0) {
         n_path = L;
       } else {
-        s = insert_sequential(s, caps, L, na, &order_ok, &n_path);
+        s = insert_sequential(s, caps, L, &order_ok, &n_path);
       }
     }
     if (s.overflow) {
@@ -1790,6 +1879,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES
       break;
     }
     const uint32_t w = global(P.order)[q];
+    // the heaviest windows (front of the cost-ordered queue) bound the
+    // kernel's length: their waves issue ahead of the other waves of their
+    // SIMD while both are ready
+    if (q < P.n_priority) {
+      __builtin_amdgcn_s_setprio(3);
+    } else {
+      __builtin_amdgcn_s_setprio(0);
+    }
     Slot s = carve(slot_mem, P.layout);
     // the window's result accumulates in LDS (phase ticks are read-modify-
     // written by every lane with the same value), not in registers

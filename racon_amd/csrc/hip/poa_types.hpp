@@ -91,8 +91,8 @@ struct PoaSlotLayout {
   uint64_t base, in_head, out_head, in_tail, out_tail, n_aligned, aligned, cov,
       last_label, e_tail, e_head, e_weight, e_next_in, e_next_out, rank_to_node,
       node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
-      row_meta, H, code, aln_node, aln_pos, scores, preds, path, seq, qual, anchor, lpath,
-      sink_score, total;
+      row_meta, H, code, scores, preds, path, seq, qual, anchor, lpath,
+      sink_score, tie_mark, total;
 };
 
 constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -133,8 +133,6 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.row_meta = take(4 * n);
   l.H = take(4 * uint64_t(c.rows) * c.cols);
   l.code = take(2 * uint64_t(c.rows) * c.cols);
-  l.aln_node = take(4 * (n + c.cols));
-  l.aln_pos = take(4 * (n + c.cols));
   l.scores = take(8 * n);
   l.preds = take(4 * n);
   l.path = take(4 * n);
@@ -143,6 +141,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.anchor = take(4 * uint64_t(c.cols));
   l.lpath = take(4 * uint64_t(c.cols));
   l.sink_score = take(4 * (n + 1));
+  l.tie_mark = take(4 * n);
   l.total = o;
   return l;
 }
@@ -166,6 +165,8 @@ struct PoaParams {
   uint32_t n_slots;
   uint32_t ring_rows;  // DP rows of 16-bit scores kept in LDS (int32 mode: half)
   uint32_t lds_bytes;  // dynamic LDS per wave: ring + layer bytes
+  uint32_t n_priority; // queue positions below this run at raised wave priority
+  uint32_t pad;
 };
 
 // Dynamic LDS of one wave: the DP row ring (ring_rows x cols x 2 bytes)

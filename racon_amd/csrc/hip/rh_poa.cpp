@@ -106,6 +106,24 @@ uint32_t ring_rows_for(uint32_t cols, uint32_t tuned) {
   return 2;
 }
 
+// Windows at the front of the cost-ordered queue whose estimated cost is at
+// least twice the mean: the ones that can outlast the rest of the batch on a
+// single wave.
+uint32_t priority_windows(const rh_poa_batch* b, const std::vector<uint32_t>& order) {
+#ifdef RH_POA_NO_PRIORITY
+  return 0;
+#endif
+  if (order.empty()) {
+    return 0;
+  }
+  long double total = 0;
+  for (uint32_t w : order) total += static_cast<long double>(window_cost(b->windows[w]));
+  const long double mean = total / order.size();
+  uint32_t n = 0;
+  while (n < order.size() && window_cost(b->windows[order[n]]) >= 2 * mean) ++n;
+  return n;
+}
+
 int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaSlotCaps& caps,
                  uint32_t max_slots, int32_t mode, double* kernel_ms) {
   if (order.empty()) {
@@ -151,6 +169,7 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   p.n_slots = static_cast<uint32_t>(slots);
   p.ring_rows = ring_rows_for(caps.cols, b->tune_ring);
   p.lds_bytes = rh::poa_lds_bytes(caps.cols, p.ring_rows);
+  p.n_priority = priority_windows(b, order);
 
   RH_HIP_CHECK(b->d_params.reserve(sizeof(PoaParams)), RH_ERR_MEMORY);
   RH_HIP_CHECK(hipMemcpyAsync(b->d_params.ptr, &p, sizeof(PoaParams), hipMemcpyHostToDevice,
@@ -602,6 +621,27 @@ int32_t rh_poa_phase_ticks(const rh_poa_batch* b, uint64_t* ticks, uint32_t n,
     for (uint32_t k = 0; k < n && k < static_cast<uint32_t>(rh::kPoaPhases); ++k) {
       ticks[k] += b->results[w].phase[k];
     }
+  }
+  return RH_OK;
+}
+
+int32_t rh_poa_window_costs(const rh_poa_batch* b, uint64_t* cells, uint64_t* ticks,
+                            uint64_t n) {
+  if (b == nullptr || cells == nullptr || ticks == nullptr) {
+    rh::set_error("rh_poa_window_costs: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  if (n > b->results.size()) {
+    rh::set_error("rh_poa_window_costs: more windows asked for than uploaded");
+    return RH_ERR_ARGUMENT;
+  }
+  for (uint64_t w = 0; w < n; ++w) {
+    const rh::PoaResult& r = b->results[w];
+    const bool ran = r.status != rh::kPoaNotRun;
+    cells[w] = ran ? r.cells : 0;
+    uint64_t t = 0;
+    for (int k = 0; k < rh::kPoaPhases; ++k) t += r.phase[k];
+    ticks[w] = ran ? t : 0;
   }
   return RH_OK;
 }
