@@ -131,6 +131,7 @@ struct Slot {
   __device__ int32_t* lpath() const { return (int32_t*)(p + L->lpath); }
   __device__ int32_t* sink_score() const { return (int32_t*)(p + L->sink_score); }
   __device__ int32_t* tie_mark() const { return (int32_t*)(p + L->tie_mark); }
+  __device__ uint64_t* row_pred4() const { return (uint64_t*)(p + L->row_pred4); }
 
   int32_t n_nodes;
   int32_t n_edges;
@@ -1232,22 +1233,28 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
     pred_off[V] = run;
   }
   wave_sync();
+  uint64_t* row_pred4 = s.row_pred4();
   for (int r = lane; r < V; r += kWave) {
     const int v = sub_order[r];
     int o = pred_off[r];
-    bool any = false;
+    int k = 0;
+    uint64_t deltas = 0;  // row deltas of the first four predecessors (0: none or too far)
     for (int e = in_head[v]; e != -1; e = e_next_in[e]) {
       const int u = e_tail[e];
       if (included(u)) {
         const int pr = row_of[u];
         const int sl = (row_meta[pr - 1] >> kRowMetaSlotShift) & 31;
         pred_rows[o++] = pr | (sl << kPredRowBits);
-        any = true;
+        const int d = r + 1 - pr;
+        if (k < 4 && d <= 0xffff) deltas |= static_cast<uint64_t>(d) << (16 * k);
+        ++k;
       }
     }
-    if (!any) {
+    if (k == 0) {
       pred_rows[o] = 0;
+      deltas = static_cast<uint64_t>(r + 1 <= 0xffff ? r + 1 : 0);
     }
+    row_pred4[r] = deltas;
   }
   return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
 }
@@ -1573,6 +1580,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       const int32_t* sub_order = s.sub_order();
       const int32_t* pred_off = s.pred_off();
       const int32_t* pred_rows = s.pred_rows();
+      const uint64_t* row_pred4 = s.row_pred4();
       int32_t* pos_node = s.path();
       int i = best_row, j = L;
       int steps = 0;
@@ -1587,6 +1595,10 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
                            ? code[static_cast<int64_t>(ri) * pitch + code_index(cj, pitch)]
                            : 0;
         const int nw = lane < 8 && i0 - lane >= 1 ? sub_order[i0 - lane - 1] : -1;
+        // lanes 8..15: the first four predecessor row deltas of the block's rows
+        uint64_t d4 = 0;
+        if (lane >= 8 && lane < 16 && i0 - (lane - 8) >= 1) d4 = row_pred4[i0 - (lane - 8) - 1];
+        const uint32_t d4lo = static_cast<uint32_t>(d4), d4hi = static_cast<uint32_t>(d4 >> 32);
         wait_stores();  // the block is in registers before the walk starts
         int out = -1;   // lane t: the node of position j0 - 1 - t
         while (i > 0 && i0 - i < 8 && j0 - j < 8) {
@@ -1609,11 +1621,19 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
             --j;
           }
           const int pay = c & static_cast<int>(kCodePayload);
+          int delta = pay;
           if ((c & kCodeViaK) != 0) {
-            i = uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask;
-          } else {
-            i -= pay;
+            // predecessor by in-edge index: the block's delta table, else the lists
+            delta = 0;
+            if (pay < 4) {
+              const uint32_t w = __builtin_amdgcn_readlane(pay < 2 ? d4lo : d4hi, 8 + (i0 - i));
+              delta = static_cast<int>((w >> (16 * (pay & 1))) & 0xffffu);
+            }
+            if (delta == 0) {
+              delta = i - (uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask);
+            }
           }
+          i -= delta;
           bad = i < 0;
         }
         if (lane < j0 - j) {

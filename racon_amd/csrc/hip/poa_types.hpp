@@ -92,7 +92,7 @@ struct PoaSlotLayout {
       last_label, e_tail, e_head, e_weight, e_next_in, e_next_out, rank_to_node,
       node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
       row_meta, H, code, scores, preds, path, seq, qual, anchor, lpath,
-      sink_score, tie_mark, total;
+      sink_score, tie_mark, row_pred4, total;
 };
 
 constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -142,6 +142,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.lpath = take(4 * uint64_t(c.cols));
   l.sink_score = take(4 * (n + 1));
   l.tie_mark = take(4 * n);
+  l.row_pred4 = take(8 * uint64_t(c.rows));
   l.total = o;
   return l;
 }
