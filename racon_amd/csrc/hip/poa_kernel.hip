@@ -301,14 +301,16 @@ __device__ __forceinline__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* 
 
 __device__ __forceinline__ int32_t wave_prefix_max(int32_t x) {
   // within rows of 16 lanes: shifts 1, 2, 4, 8 (lanes shifted in from
-  // outside the row keep the identity)
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x111, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x112, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x114, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x118, 0xf, 0xf, false));
+  // outside the row keep the identity). INT32_MIN, the identity of max, lets
+  // each move fold into one v_max_i32 with a DPP source.
+  constexpr int32_t kId = INT32_MIN;
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x118, 0xf, 0xf, false));
   // across rows: lane 15 into rows 1 and 3, lane 31 into rows 2 and 3
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x142, 0xa, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(kNegInf, x, 0x143, 0xc, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(kId, x, 0x143, 0xc, 0xf, false));
   return x;
 }
 
@@ -699,6 +701,199 @@ __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* 
   return __builtin_amdgcn_readlane(last, end_lane);
 }
 
+// ---- packed 16-bit pairs: two DP columns per 32-bit register ----
+
+typedef short __attribute__((ext_vector_type(2))) I16x2;
+typedef unsigned short __attribute__((ext_vector_type(2))) U16x2;
+
+__device__ __forceinline__ U16x2 as_u16x2(uint32_t x) { return __builtin_bit_cast(U16x2, x); }
+__device__ __forceinline__ I16x2 as_i16x2(uint32_t x) { return __builtin_bit_cast(I16x2, x); }
+__device__ __forceinline__ uint32_t pk_bits(U16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ uint32_t pk_bits(I16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+// per-half wrapping add / subtract, signed max, unsigned min, multiply-add
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+  return pk_bits(as_u16x2(a) + as_u16x2(b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+  return pk_bits(as_u16x2(a) - as_u16x2(b));
+}
+__device__ __forceinline__ uint32_t pk_max(uint32_t a, uint32_t b) {
+  return pk_bits(__builtin_elementwise_max(as_i16x2(a), as_i16x2(b)));
+}
+__device__ __forceinline__ uint32_t pk_min_u(uint32_t a, uint32_t b) {
+  return pk_bits(__builtin_elementwise_min(as_u16x2(a), as_u16x2(b)));
+}
+__device__ __forceinline__ uint32_t pk_mad(uint32_t a, uint32_t b, uint32_t c) {
+  return pk_bits(as_u16x2(a) * as_u16x2(b) + as_u16x2(c));
+}
+// max of a with the low (resp. high) half of b in both halves
+__device__ __forceinline__ uint32_t pk_max_lo(uint32_t a, uint32_t b) {
+  const I16x2 v = as_i16x2(b);
+  return pk_bits(__builtin_elementwise_max(as_i16x2(a), __builtin_shufflevector(v, v, 0, 0)));
+}
+__device__ __forceinline__ uint32_t pk_max_hi(uint32_t a, uint32_t b) {
+  const I16x2 v = as_i16x2(b);
+  return pk_bits(__builtin_elementwise_max(as_i16x2(a), __builtin_shufflevector(v, v, 1, 1)));
+}
+// (lo, max(lo, hi)): inclusive prefix max inside the pair
+__device__ __forceinline__ uint32_t pk_scan(uint32_t a) {
+  const I16x2 v = as_i16x2(a);
+  return pk_bits(__builtin_elementwise_max(v, __builtin_shufflevector(v, v, 0, 0)));
+}
+__device__ __forceinline__ uint32_t pk_splat(int32_t v) {
+  return (static_cast<uint32_t>(v) & 0xffffu) * 0x10001u;
+}
+__device__ __forceinline__ int32_t pk_half(uint32_t x, int hi) {
+  return hi ? static_cast<int32_t>(x) >> 16 : static_cast<int32_t>(static_cast<int16_t>(x & 0xffffu));
+}
+
+// Per-lane constants of the packed rows: pair q holds strip columns
+// j0 + 2q (low half) and j0 + 2q + 1 (high half).
+template <int W>
+struct PackedStrip {
+  uint32_t base[W / 2];  // layer bytes of the pair, one per half
+  uint32_t cg[W / 2];    // j * gap
+  uint32_t match;        // match and gap in both halves
+  uint32_t gap;
+  uint32_t one;          // 1 in both halves, opaque to the optimizer (a
+  uint32_t c4000;        // visible 1 turns min(x, 1) into compares and selects)
+  uint32_t diff;         // mismatch - match in both halves
+};
+
+template <int W>
+__device__ __forceinline__ void packed_strip_init(PackedStrip<W>& ps, const uint32_t* sw, int j0,
+                                                  int32_t match, int32_t mismatch, int32_t gap) {
+#pragma unroll
+  for (int q = 0; q < W / 2; ++q) {
+    ps.base[q] = __builtin_amdgcn_perm(sw[q / 2], sw[q / 2], (q & 1) ? 0x0c030c02u : 0x0c010c00u);
+    const int32_t j = j0 + 2 * q;
+    ps.cg[q] = (static_cast<uint32_t>(j * gap) & 0xffffu) | (static_cast<uint32_t>((j + 1) * gap) << 16);
+  }
+  ps.match = pk_splat(match);
+  ps.gap = pk_splat(gap);
+  ps.one = 0x00010001u;
+  asm volatile("" : "+s"(ps.one));
+  ps.c4000 = 0x40004000u;
+  asm volatile("" : "+v"(ps.c4000));
+  ps.diff = pk_splat(mismatch - match);
+}
+
+// Predecessor row p (entry e) at the lane's strip as pairs:
+// pp[q] = (H[p][j0 + 2q], H[p][j0 + 2q + 1]); returns H[p][j0 - 1] in the
+// low half. Row 0 is the virtual start row (j * gap).
+template <int W>
+__device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, const Slot& s, int e,
+                                               int j0, int jr, int L, int32_t gap,
+                                               const PackedStrip<W>& ps, uint32_t* pp) {
+  const int p = e & kPredRowMask;
+  const int sl = static_cast<unsigned>(e) >> kPredRowBits;
+  if (p == 0) {
+#pragma unroll
+    for (int q = 0; q < W / 2; ++q) pp[q] = ps.cg[q];
+    return static_cast<uint32_t>((j0 - 1) * gap);
+  } else if (sl < kSlotGlobal) {
+    const int16_t* row = ring + sl * pitch + kColShift;
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      const uint2 v = *reinterpret_cast<const uint2*>(row + jr + 4 * q);
+      pp[2 * q] = v.x;
+      pp[2 * q + 1] = v.y;
+    }
+    return static_cast<uint16_t>(row[jr - 1]);
+  } else {
+    const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+    int32_t hv[W + 1];
+#pragma unroll
+    for (int u = 0; u <= W; ++u) {
+      const int j = j0 - 1 + u;
+      hv[u] = j <= L ? row[j] : 0;
+    }
+    wait_stores();
+#pragma unroll
+    for (int q = 0; q < W / 2; ++q) {
+      pp[q] = (static_cast<uint32_t>(hv[2 * q + 1]) & 0xffffu) |
+              (static_cast<uint32_t>(hv[2 * q + 2]) << 16);
+    }
+    return static_cast<uint32_t>(hv[0]);
+  }
+}
+
+// One DP row with a single predecessor in packed pairs: the moves, codes,
+// stores and end-column value of finish_row<W, false>, two columns per
+// instruction. Columns are kept as H - j * gap through the horizontal
+// chain, so every value of a live column lies within
+// max_abs * (V + 2L + 2) of zero; the caller takes this path only when that
+// fits 16 bits. Columns past L compute unused values (they flow only into
+// later columns).
+template <int W>
+__device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const uint32_t* pp,
+                                              uint32_t hp0, uint32_t b, uint32_t pay0, int lane,
+                                              int j0, bool live, int L, int32_t gap, int own,
+                                              int pitch, int16_t* ring, int32_t* Hg, uint16_t* Cr,
+                                              int end_lane, int end_u) {
+  constexpr int H2 = W / 2;
+  const uint32_t bb = b * 0x10001u;
+  // column 0 (hp0 + gap) heads lane 0's chain
+  const int32_t h0 = pk_half(hp0 + static_cast<uint32_t>(gap), 0);
+  uint32_t carry = lane == 0 ? pk_splat(h0) : 0x80008000u;
+  uint32_t bd[H2], bv[H2], pre[H2];
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    // diagonal sources H[p][j - 1] of the pair
+    const uint32_t d = q == 0 ? ((pp[0] << 16) | (hp0 & 0xffffu))
+                              : __builtin_amdgcn_alignbit(pp[q], pp[q - 1], 16);
+    const uint32_t mis = pk_min_u(ps.base[q] ^ bb, ps.one);  // 1 where the bases differ
+    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match));
+    bv[q] = pk_add(pp[q], ps.gap);
+    carry = pk_max_hi(pk_scan(pk_sub(pk_max(bd[q], bv[q]), ps.cg[q])), carry);
+    pre[q] = carry;
+  }
+  const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
+  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
+  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
+  uint32_t h[H2], cd[H2];
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    h[q] = pk_add(pk_max_lo(pre[q], static_cast<uint32_t>(excl)), ps.cg[q]);
+    // diagonal first, then vertical, else horizontal: with 0/1 flags
+    // fd (not diagonal) and fv (not vertical) the code is
+    // pay + fd * (0x4000 + fv * (0x4000 - pay))
+    const uint32_t fd = pk_min_u(pk_sub(h[q], bd[q]), ps.one);
+    const uint32_t fv = pk_min_u(pk_sub(h[q], bv[q]), ps.one);
+    cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+  }
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(cd[2 * q], cd[2 * q + 1]);
+    }
+    if (own < kSlotGlobal) {
+      int16_t* row = ring + own * pitch + kColShift;
+#pragma unroll
+      for (int q = 0; q < W / 4; ++q) {
+        *reinterpret_cast<uint2*>(row + j0 + 4 * q) = make_uint2(h[2 * q], h[2 * q + 1]);
+      }
+    }
+  }
+  if (own == kSlotGlobal) {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      if (j0 + u <= L) Hg[j0 + u] = pk_half(h[u / 2], u & 1);
+    }
+  }
+  if (lane == 0) {
+    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(h0);
+    if (own == kSlotGlobal) Hg[0] = h0;
+    Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | pay0);
+  }
+  uint32_t lastp = h[0];
+#pragma unroll
+  for (int q = 1; q < H2; ++q) {
+    if (q == (end_u >> 1)) lastp = h[q];
+  }
+  return __builtin_amdgcn_readlane(pk_half(lastp, end_u & 1), end_lane);
+}
+
 // DP rows of one layer with 16-bit scores, every row in one pass of the
 // wave: lane k owns columns 1 + kW .. (k + 1)W (W = 4, 8 or 12, so that
 // 64W >= L), the layer bytes of its strip stay in registers, and column 0
@@ -707,8 +902,9 @@ __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* 
 // that one max keeps the first predecessor with the best score (in-edge
 // order, as spoa's traceback prefers). Same moves, codes and end cell as
 // dp_layer. Requires in-degree <= 255, 64W + pad <= pitch and 16-bit scores
-// over the whole strip.
-template <int W>
+// over the whole strip. With kPacked, single-predecessor rows run in packed
+// 16-bit pairs (packed_row), which needs max_abs * (V + 2L + 2) in 16 bits.
+template <int W, bool kPacked>
 __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t* lds, int V, int L,
                               int* best_score, bool* tied) {
   static_assert(W % 4 == 0 && W >= 4 && W <= 12, "strip width");
@@ -727,6 +923,8 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
   uint32_t sw[W / 4];            // layer bytes of the strip
 #pragma unroll
   for (int q = 0; q < W / 4; ++q) sw[q] = *reinterpret_cast<const uint32_t*>(seq + (jr - 1) + 4 * q);
+  PackedStrip<W> ps;
+  if (kPacked) packed_strip_init<W>(ps, sw, j0, match, mismatch, gap);
   const int end_lane = (L - 1) / W;
   const int end_u = (L - 1) % W;
   int32_t best = kNegInf;
@@ -771,7 +969,15 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
       };
       int32_t bd[W], bv[W];
       int32_t h_last;
-      if (np == 1) {
+      if (kPacked && np == 1) {
+        const int e = pred_at(0);
+        const uint32_t delta = static_cast<uint32_t>(r - (e & kPredRowMask));
+        const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
+        uint32_t pp[W / 2];
+        const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pp);
+        h_last = packed_row<W>(ps, pp, hp0, b, pay0, lane, j0, live, L, gap, own, pitch, ring, Hg,
+                               Cr, end_lane, end_u);
+      } else if (np == 1) {
         // move-code payload of the single predecessor: its row delta
         const uint32_t delta = static_cast<uint32_t>(r - (pred_at(0) & kPredRowMask));
         const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
@@ -837,7 +1043,16 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
   return (T*)((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-enum DpKind : int { kDpStrip4, kDpStrip8, kDpStrip12, kDpGeneric16, kDpGeneric32 };
+enum DpKind : int {
+  kDpStrip4,
+  kDpStrip8,
+  kDpStrip12,
+  kDpPacked4,
+  kDpPacked8,
+  kDpPacked12,
+  kDpGeneric16,
+  kDpGeneric32
+};
 
 // The DP of one layer as an out-of-line call: its loops get a register
 // allocation of their own instead of sharing one with the rest of the window
@@ -859,11 +1074,17 @@ __device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLa
   bool tied = false;
   int row;
   if (kKind == kDpStrip4) {
-    row = dp_layer_strip<4>(P, s, lds, V, L, &best_score, &tied);
+    row = dp_layer_strip<4, false>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpStrip8) {
-    row = dp_layer_strip<8>(P, s, lds, V, L, &best_score, &tied);
+    row = dp_layer_strip<8, false>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpStrip12) {
-    row = dp_layer_strip<12>(P, s, lds, V, L, &best_score, &tied);
+    row = dp_layer_strip<12, false>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpPacked4) {
+    row = dp_layer_strip<4, true>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpPacked8) {
+    row = dp_layer_strip<8, true>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpPacked12) {
+    row = dp_layer_strip<12, true>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpGeneric16) {
     row = dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied);
   } else {
@@ -1259,6 +1480,121 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
 }
 
+typedef __attribute__((address_space(3))) uint32_t LWord;
+
+// Included nodes of a layer that does not span the window (spoa
+// Graph::subgraph's walk): the nodes >= begin that reach `end` backwards
+// along in-edges and aligned sets through nodes >= begin. The set is a
+// reachability closure and the stack bound (every in-edge and aligned entry
+// once, plus one) is within stack_cap, so the order of the walk changes
+// nothing: here every lane pops one stack entry per round, a visit bit in
+// LDS (claimed with an atomic OR) lets each node expand once, and the stack
+// sits in LDS above the bits, continuing into the slot's global stack.
+// Writes in_sub[0..nn). Returns 0, kPoaNodeOverflow, or 1 when the LDS
+// (lds_bytes at lds_addr) cannot hold the bits and 64 stack entries.
+__device__ __noinline__ int include_walk(GByte* slot_base, CLayout* layout, uint32_t lds_addr,
+                                         int lds_bytes, int begin, int end, int nn,
+                                         int stack_cap) {
+  const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
+  lds_addr = uniform(static_cast<int>(lds_addr));
+  lds_bytes = uniform(lds_bytes);
+  begin = uniform(begin);
+  end = uniform(end);
+  nn = uniform(nn);
+  stack_cap = uniform(stack_cap);
+  const int lane = lane_id();
+  const int n_words = (nn + 31) >> 5;
+  const int bits_bytes = static_cast<int>(align_up(static_cast<uint64_t>(n_words) * 4, 16));
+  const int lcap = (lds_bytes - bits_bytes) / 4;
+  if (lcap < kWave) return 1;
+  LWord* bits = (LWord*)(size_t)lds_addr;
+  LWord* lstack = (LWord*)(size_t)(lds_addr + bits_bytes);
+  int32_t* gstack = s.stack();
+  const int32_t* in_head = s.in_head();
+  const int32_t* e_next_in = s.e_next_in();
+  const int32_t* e_tail = s.e_tail();
+  const int32_t* aligned = s.aligned();
+  const uint8_t* n_aligned_of = s.n_aligned();
+  for (int w = lane; w < n_words; w += kWave) bits[w] = 0u;
+  auto st_load = [&](int i) -> int {
+    return i < lcap ? static_cast<int>(lstack[i]) : gstack[i];
+  };
+  auto st_store = [&](int i, int x) {
+    if (i < lcap) {
+      lstack[i] = static_cast<uint32_t>(x);
+    } else {
+      gstack[i] = x;
+    }
+  };
+  auto unseen = [&](int u) {
+    return u >= begin && ((bits[u >> 5] >> (u & 31)) & 1u) == 0u;
+  };
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (lane == 0) st_store(0, end);
+  int top = 1;
+  wave_fence();
+  // candidates of one round, pushed in lane order; false on overflow
+  auto push = [&](bool c, int x) -> bool {
+    const uint64_t m = __ballot(c);
+    const int cnt = __popcll(m);
+    if (top + cnt > stack_cap) return false;
+    if (c) st_store(top + __popcll(m & below), x);
+    top += cnt;
+    return true;
+  };
+  // every claim sets a bit that stays set, so more than nn claims cannot
+  // happen; the count bounds the rounds even so (rounds without a claim only
+  // pop)
+  int claims = 0;
+  while (top > 0) {
+    const int nb = min(top, kWave);
+    const int first = top - nb;
+    int v = -1;
+    if (lane < nb) v = st_load(first + lane);
+    top = first;
+    bool mine = false;
+    if (v >= begin) {
+      const uint32_t m = 1u << (v & 31);
+      mine = (__atomic_fetch_or(&bits[v >> 5], m, __ATOMIC_RELAXED) & m) == 0u;
+    }
+    claims += __popcll(__ballot(mine));
+    if (claims > nn) return kPoaNodeOverflow;
+    int e = -1;
+    int na = 0;
+    int al[kMaxAligned];
+    if (mine) {
+      e = in_head[v];
+      na = n_aligned_of[v];
+#pragma unroll
+      for (int k = 0; k < static_cast<int>(kMaxAligned); ++k) al[k] = aligned[v * kMaxAligned + k];
+    }
+    wave_fence();  // the claims before the visit-bit reads below
+    const int max_na = uniform(__builtin_amdgcn_readlane(wave_prefix_max(na), kWave - 1));
+#pragma unroll
+    for (int k = 0; k < static_cast<int>(kMaxAligned); ++k) {
+      if (k >= max_na) break;
+      const bool c = k < na && unseen(al[k]);
+      if (!push(c, al[k])) return kPoaNodeOverflow;
+    }
+    while (__ballot(e != -1) != 0) {
+      int t = -1;
+      int nx = -1;
+      if (e != -1) {
+        t = e_tail[e];
+        nx = e_next_in[e];
+      }
+      if (!push(e != -1 && unseen(t), t)) return kPoaNodeOverflow;
+      e = nx;
+    }
+    wave_fence();
+  }
+  uint8_t* in_sub = s.in_sub();
+  for (int v = lane; v < nn; v += kWave) {
+    in_sub[v] = static_cast<uint8_t>((bits[v >> 5] >> (v & 31)) & 1u);
+  }
+  return 0;
+}
+
 // End cell among tied sinks: spoa takes the first tied sink in its DFS order
 // of the (sub)graph (roots in id order; a node's predecessors first; a lead
 // node pushes its aligned set and emits it after itself, in aligned-list
@@ -1427,37 +1763,52 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       return;
     }
 
-    // 1. included node set
+    // 1. included node set (all nodes for a layer spanning the window)
     const int nn = s.n_nodes;
-    for (int v = lane; v < nn; v += kWave) {
-      s.in_sub()[v] = full ? 1 : 0;
-    }
-    wave_sync();
+    int walk_rc = 1;
     if (!full) {
-      int top = 0;
-      s.stack()[top++] = static_cast<int>(Lr.end);
-      while (top > 0) {
-        const int v = uniform(s.stack()[--top]);
-        if (s.in_sub()[v] == 0 && v >= static_cast<int>(Lr.begin)) {
-          const int na = s.n_aligned()[v];
-          int deg = na;
-          for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
-            ++deg;
+      uint32_t walk_lds = static_cast<uint32_t>((size_t)(LByte*)lds);
+      asm volatile("" : "+v"(walk_lds));
+      walk_rc = uniform(include_walk(s.p, s.L, walk_lds, static_cast<int>(P.ring_rows) * pitch * 2,
+                                     static_cast<int>(Lr.begin), static_cast<int>(Lr.end), nn,
+                                     static_cast<int>(caps.stack)));
+      if (walk_rc < 0) {
+        R.status = walk_rc;
+        return;
+      }
+    }
+    if (walk_rc != 0) {
+      for (int v = lane; v < nn; v += kWave) {
+        s.in_sub()[v] = full ? 1 : 0;
+      }
+      wave_sync();
+      if (!full) {
+        int top = 0;
+        s.stack()[top++] = static_cast<int>(Lr.end);
+        while (top > 0) {
+          const int v = uniform(s.stack()[--top]);
+          if (s.in_sub()[v] == 0 && v >= static_cast<int>(Lr.begin)) {
+            const int na = s.n_aligned()[v];
+            int deg = na;
+            for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+              ++deg;
+            }
+            if (top + deg > static_cast<int>(caps.stack)) {
+              R.status = kPoaNodeOverflow;
+              return;
+            }
+            for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+              s.stack()[top++] = s.e_tail()[e];
+            }
+            for (int k = 0; k < na; ++k) {
+              s.stack()[top++] = s.aligned()[v * kMaxAligned + k];
+            }
+            s.in_sub()[v] = 1;
           }
-          if (top + deg > static_cast<int>(caps.stack)) {
-            R.status = kPoaNodeOverflow;
-            return;
-          }
-          for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
-            s.stack()[top++] = s.e_tail()[e];
-          }
-          for (int k = 0; k < na; ++k) {
-            s.stack()[top++] = s.aligned()[v * kMaxAligned + k];
-          }
-          s.in_sub()[v] = 1;
         }
       }
     }
+    wave_sync();
 
     clk.lap(R, kPhaseInclude);
     // 2. DP row order: the maintained topological order restricted to the
@@ -1520,7 +1871,15 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     // opaque to the optimizer: otherwise the callee would rebuild the address
     // from the dynamic LDS symbol (a table load) inside its loops
     asm volatile("" : "+v"(lds_addr));
-    if (strip_ok && L <= 4 * kWave) {
+    // packed pairs keep H - j * gap in 16 bits
+    const bool packed = strip_ok && static_cast<int64_t>(max_abs) * (V + 2 * L + 2) < 32000;
+    if (packed && L <= 4 * kWave) {
+      dp = dp_layer_call<kDpPacked4>(pp, s.p, ly, lds_addr, V, L);
+    } else if (packed && L <= 8 * kWave) {
+      dp = dp_layer_call<kDpPacked8>(pp, s.p, ly, lds_addr, V, L);
+    } else if (packed && L <= 12 * kWave) {
+      dp = dp_layer_call<kDpPacked12>(pp, s.p, ly, lds_addr, V, L);
+    } else if (strip_ok && L <= 4 * kWave) {
       dp = dp_layer_call<kDpStrip4>(pp, s.p, ly, lds_addr, V, L);
     } else if (strip_ok && L <= 8 * kWave) {
       dp = dp_layer_call<kDpStrip8>(pp, s.p, ly, lds_addr, V, L);
