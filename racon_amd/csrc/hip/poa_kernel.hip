@@ -1369,15 +1369,31 @@ __device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8
   return 0;
 }
 
+// Inclusive prefix sum over the 64 lanes (DPP, as wave_prefix_max; 0 is
+// the identity, so each move folds into one v_add_u32 with a DPP source).
+__device__ __forceinline__ int32_t wave_prefix_sum(int32_t x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+  return x;
+}
+
 // DP rows of a layer (row r + 1 is node sub_order[r]): predecessor rows in
 // in-edge order (row 0, the virtual start, for rows without one), sink flags,
 // and an LDS slot for every row that a later row reads: the lowest slot whose
 // previous row is no longer needed (lane k < n_slots holds the last row that
 // reads slot k). With kFull every node is in the layer's subgraph and the
 // membership checks are skipped. Returns the largest predecessor count.
-// Out of line, like the DP, for a register allocation of its own.
+// Each lane takes kU rows per round (64 kU rows) and walks their edge lists
+// in lockstep, one edge of every list per step, so that the loads of the
+// kU lists are in flight together. Out of line, like the DP, for a
+// register allocation of its own.
 template <bool kFull>
 __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V, int n_slots) {
+  constexpr int kU = 4;
   const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
   V = uniform(V);
   n_slots = uniform(n_slots);
@@ -1399,83 +1415,152 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   int run = 0;
   int max_np = 0;
   int busy = lane < n_slots ? 0 : 0x7fffffff;
-  for (int r0 = 0; r0 < V; r0 += kWave) {
-    const int r = r0 + lane;
-    int cnt = 0;
-    int last_use = 0;
-    int meta = 0;
-    if (r < V) {
-      const int v = sub_order[r];
-      for (int e = in_head[v]; e != -1; e = e_next_in[e]) {
-        if (included(e_tail[e])) ++cnt;
-      }
-      cnt = max(cnt, 1);  // the virtual start row
-      int out_cnt = 0;
-      for (int e = out_head[v]; e != -1; e = e_next_out[e]) {
-        const int h = e_head[e];
-        if (included(h)) {
-          ++out_cnt;
-          last_use = max(last_use, row_of[h]);
+  for (int r0 = 0; r0 < V; r0 += kU * kWave) {
+    int v[kU], ei[kU], eo[kU], cnt[kU], oc[kU], lu[kU], meta[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int r = r0 + i * kWave + lane;
+      v[i] = r < V ? sub_order[r] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      ei[i] = v[i] >= 0 ? in_head[v[i]] : -1;
+      eo[i] = v[i] >= 0 ? out_head[v[i]] : -1;
+      meta[i] = v[i] >= 0 ? static_cast<int>(base[v[i]]) : 0;
+      cnt[i] = 0;
+      oc[i] = 0;
+      lu[i] = 0;
+    }
+    for (;;) {
+      bool act = false;
+#pragma unroll
+      for (int i = 0; i < kU; ++i) act = act || ei[i] != -1 || eo[i] != -1;
+      if (__ballot(act) == 0) break;
+      int t[kU], h[kU], ni[kU], no[kU];
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        t[i] = -1;
+        ni[i] = -1;
+        h[i] = -1;
+        no[i] = -1;
+        if (ei[i] != -1) {
+          t[i] = e_tail[ei[i]];
+          ni[i] = e_next_in[ei[i]];
+        }
+        if (eo[i] != -1) {
+          h[i] = e_head[eo[i]];
+          no[i] = e_next_out[eo[i]];
         }
       }
-      meta = (out_cnt == 0 ? kRowMetaSink : 0) | static_cast<int>(base[v]);
-    }
-    max_np = max(max_np, cnt);
-    int my_slot = kSlotNone;
-    const int n_chunk = min(kWave, V - r0);
-    for (int rr = 0; rr < n_chunk; ++rr) {
-      const int lu = __builtin_amdgcn_readlane(last_use, rr);
-      if (lu > 0) {
-        const int row = r0 + rr + 1;
-        const unsigned long long free_mask = __ballot(busy < row);
-        int sl = kSlotGlobal;
-        if (free_mask != 0) {
-          sl = __ffsll(static_cast<long long>(free_mask)) - 1;
-          if (lane == sl) busy = lu;
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        if (t[i] != -1 && included(t[i])) ++cnt[i];
+        if (h[i] != -1 && included(h[i])) {
+          ++oc[i];
+          lu[i] = max(lu[i], row_of[h[i]]);
         }
-        if (lane == rr) my_slot = sl;
+        ei[i] = ni[i];
+        eo[i] = no[i];
       }
     }
-    if (r < V) {
-      row_meta[r] = meta | (my_slot << kRowMetaSlotShift);
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      if (v[i] >= 0) {
+        cnt[i] = max(cnt[i], 1);  // the virtual start row
+        meta[i] |= oc[i] == 0 ? kRowMetaSink : 0;
+      }
+      max_np = max(max_np, cnt[i]);
     }
-    // inclusive scan of cnt
-    int x = cnt;
-    for (int d = 1; d < kWave; d <<= 1) {
-      const int y = __shfl_up(x, d, kWave);
-      if (lane >= d) x += y;
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int c0 = r0 + i * kWave;
+      if (c0 >= V) break;
+      int my_slot = kSlotNone;
+      const int n_chunk = min(kWave, V - c0);
+      for (int rr = 0; rr < n_chunk; ++rr) {
+        const int l = __builtin_amdgcn_readlane(lu[i], rr);
+        if (l > 0) {
+          const int row = c0 + rr + 1;
+          const unsigned long long free_mask = __ballot(busy < row);
+          int sl = kSlotGlobal;
+          if (free_mask != 0) {
+            sl = __ffsll(static_cast<long long>(free_mask)) - 1;
+            if (lane == sl) busy = l;
+          }
+          if (lane == rr) my_slot = sl;
+        }
+      }
+      const int r = c0 + lane;
+      const int x = wave_prefix_sum(cnt[i]);
+      if (r < V) {
+        row_meta[r] = meta[i] | (my_slot << kRowMetaSlotShift);
+        pred_off[r] = run + x - cnt[i];
+      }
+      run += __builtin_amdgcn_readlane(x, kWave - 1);
     }
-    if (r < V) {
-      pred_off[r] = run + x - cnt;
-    }
-    run += __shfl(x, kWave - 1, kWave);
   }
   if (lane == 0) {
     pred_off[V] = run;
   }
   wave_sync();
   uint64_t* row_pred4 = s.row_pred4();
-  for (int r = lane; r < V; r += kWave) {
-    const int v = sub_order[r];
-    int o = pred_off[r];
-    int k = 0;
-    uint64_t deltas = 0;  // row deltas of the first four predecessors (0: none or too far)
-    for (int e = in_head[v]; e != -1; e = e_next_in[e]) {
-      const int u = e_tail[e];
-      if (included(u)) {
-        const int pr = row_of[u];
-        const int sl = (row_meta[pr - 1] >> kRowMetaSlotShift) & 31;
-        pred_rows[o++] = pr | (sl << kPredRowBits);
-        const int d = r + 1 - pr;
-        if (k < 4 && d <= 0xffff) deltas |= static_cast<uint64_t>(d) << (16 * k);
-        ++k;
+  for (int r0 = 0; r0 < V; r0 += kU * kWave) {
+    int v[kU], e[kU], o[kU], k[kU];
+    uint64_t deltas[kU];  // row deltas of the first four predecessors (0: none or too far)
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int r = r0 + i * kWave + lane;
+      v[i] = r < V ? sub_order[r] : -1;
+      o[i] = r < V ? pred_off[r] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      e[i] = v[i] >= 0 ? in_head[v[i]] : -1;
+      k[i] = 0;
+      deltas[i] = 0;
+    }
+    for (;;) {
+      bool act = false;
+#pragma unroll
+      for (int i = 0; i < kU; ++i) act = act || e[i] != -1;
+      if (__ballot(act) == 0) break;
+      int t[kU], nx[kU], pr[kU];
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        t[i] = -1;
+        nx[i] = -1;
+        if (e[i] != -1) {
+          t[i] = e_tail[e[i]];
+          nx[i] = e_next_in[e[i]];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        pr[i] = t[i] != -1 && included(t[i]) ? row_of[t[i]] : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        if (pr[i] > 0) {
+          const int sl = (row_meta[pr[i] - 1] >> kRowMetaSlotShift) & 31;
+          pred_rows[o[i]++] = pr[i] | (sl << kPredRowBits);
+          const int d = r0 + i * kWave + lane + 1 - pr[i];
+          if (k[i] < 4 && d <= 0xffff) deltas[i] |= static_cast<uint64_t>(d) << (16 * k[i]);
+          ++k[i];
+        }
+        e[i] = nx[i];
       }
     }
-    if (k == 0) {
-      pred_rows[o] = 0;
-      deltas = static_cast<uint64_t>(r + 1 <= 0xffff ? r + 1 : 0);
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int r = r0 + i * kWave + lane;
+      if (v[i] >= 0) {
+        if (k[i] == 0) {
+          pred_rows[o[i]] = 0;
+          deltas[i] = static_cast<uint64_t>(r + 1 <= 0xffff ? r + 1 : 0);
+        }
+        row_pred4[r] = deltas[i];
+      }
     }
-    row_pred4[r] = deltas;
   }
   return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
 }
@@ -2041,23 +2126,41 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       const int M = s.n_nodes - nodes_before;
       const int total = s.n_nodes;
       if (order_ok && M > 0) {
-        for (int r = lane; r < nodes_before; r += kWave) {
-          // number of anchors < r (anchors are non-decreasing)
-          int lo = 0, hi = M;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (s.anchor()[mid] < r) lo = mid + 1; else hi = mid;
-          }
-          s.sub_order()[r + lo] = s.rank_to_node()[r];
+        // an old node of rank r moves up by the number of anchors below r
+        // (anchors are non-decreasing; searched in LDS when they fit), so a
+        // sweep from the top rank down reads every entry before it is
+        // overwritten; ranks up to the first anchor stay
+        const int32_t* anchor = s.anchor();
+        LWord* la = (LWord*)(size_t)static_cast<uint32_t>((size_t)(LByte*)lds);
+        const bool anchors_in_lds = M * 4 <= static_cast<int>(P.ring_rows) * pitch * 2;
+        if (anchors_in_lds) {
+          for (int k = lane; k < M; k += kWave) la[k] = static_cast<uint32_t>(anchor[k]);
+          wave_fence();
         }
-        for (int k = lane; k < M; k += kWave) {
-          s.sub_order()[s.anchor()[k] + 1 + k] = nodes_before + k;
+        const int first_moved = uniform(anchor[0]) + 1;
+        for (int r0 = ((nodes_before - 1) / kWave) * kWave; r0 + kWave > first_moved; r0 -= kWave) {
+          const int r = r0 + lane;
+          const bool moves = r >= first_moved && r < nodes_before;
+          int lo = 1, hi = M;  // anchor[0] < r for every moved rank
+          while (__ballot(moves && lo < hi) != 0) {
+            if (moves && lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              const int a = anchors_in_lds ? static_cast<int>(la[mid]) : anchor[mid];
+              if (a < r) lo = mid + 1; else hi = mid;
+            }
+          }
+          if (moves) {
+            const int v = s.rank_to_node()[r];
+            s.rank_to_node()[r + lo] = v;
+            s.node_rank()[v] = r + lo;
+          }
+          wave_fence();
         }
         wave_sync();
-        for (int r = lane; r < total; r += kWave) {
-          const int v = s.sub_order()[r];
-          s.rank_to_node()[r] = v;
-          s.node_rank()[v] = r;
+        for (int k = lane; k < M; k += kWave) {
+          const int r = anchor[k] + 1 + k;
+          s.rank_to_node()[r] = nodes_before + k;
+          s.node_rank()[nodes_before + k] = r;
         }
         wave_sync();
       }
@@ -2084,99 +2187,116 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     clk.lap(R, kPhaseSort);
   }
 
-  // spoa's exact DFS order of the final graph for the heaviest bundle
-  if (W.n_layers > 0) {
-    topo_sort(s, nullptr, s.rank_to_node(), static_cast<int>(caps.stack));
-    if (s.overflow) {
-      R.status = kPoaNodeOverflow;
-      return;
-    }
-    wave_sync();
-  }
   clk.lap(R, kPhaseSort);
 
-  // heaviest bundle (lane-uniform)
+  // heaviest bundle (lane-uniform). Node scores and predecessors do not
+  // depend on which topological order visits the nodes; spoa's DFS order
+  // decides only which of several top-scoring nodes comes first and which
+  // nodes branch completion revisits. The first pass takes the maintained
+  // order; when the top score is shared or its node has out-edges, spoa's
+  // exact DFS order of the final graph is computed and the bundle taken
+  // again. (With no layers the backbone chain is already in DFS order.)
   const int N = s.n_nodes;
-  for (int v = lane; v < N; v += kWave) {
-    s.scores()[v] = -1;
-    s.preds()[v] = -1;
-  }
-  wave_sync();
   int max_node = 0;
-  for (int r = 0; r < N; ++r) {
-    const int v = uniform(s.rank_to_node()[r]);
-    for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
-      const int t = uniform(s.e_tail()[e]);
-      const int64_t w = s.e_weight()[e];
-      const int64_t sv = s.scores()[v];
-      const int pv = s.preds()[v];
-      if (sv < w || (sv == w && pv != -1 && s.scores()[pv] <= s.scores()[t])) {
-        s.scores()[v] = w;
-        s.preds()[v] = t;
-      }
-    }
-    const int pv = s.preds()[v];
-    if (pv != -1) {
-      s.scores()[v] = s.scores()[v] + s.scores()[pv];
-    }
-    if (s.scores()[max_node] < s.scores()[v]) {
-      max_node = v;
-    }
-  }
-  if (s.out_head()[max_node] != -1) {
-    for (int r = lane; r < N; r += kWave) {
-      s.node_rank()[s.rank_to_node()[r]] = r;
-    }
-    wave_sync();
-    int guard = 0;
-    while (s.out_head()[max_node] != -1) {
-      if (++guard > N + 1) {
-        R.status = kPoaLoopLimit;
+  for (int pass = 0;; ++pass) {
+    const bool exact = pass > 0 || W.n_layers == 0;
+    if (pass > 0) {
+      clk.lap(R, kPhaseConsensus);
+      topo_sort(s, nullptr, s.rank_to_node(), static_cast<int>(caps.stack));
+      if (s.overflow) {
+        R.status = kPoaNodeOverflow;
         return;
       }
-      // branch completion from the rank of max_node
-      const int rank = s.node_rank()[max_node];
-      const int v0 = max_node;
-      for (int e = uniform(s.out_head()[v0]); e != -1; e = uniform(s.e_next_out()[e])) {
-        const int h = uniform(s.e_head()[e]);
-        for (int f = uniform(s.in_head()[h]); f != -1; f = uniform(s.e_next_in()[f])) {
-          const int t = uniform(s.e_tail()[f]);
-          if (t != v0) {
-            s.scores()[t] = -1;
-          }
-        }
-      }
-      int64_t best_score = 0;
-      int best_node = 0;
-      for (int r = rank + 1; r < N; ++r) {
-        const int u = uniform(s.rank_to_node()[r]);
-        s.scores()[u] = -1;
-        s.preds()[u] = -1;
-        for (int e = uniform(s.in_head()[u]); e != -1; e = uniform(s.e_next_in()[e])) {
-          const int t = uniform(s.e_tail()[e]);
-          const int64_t st = s.scores()[t];
-          if (st == -1) {
-            continue;
-          }
-          const int64_t w = s.e_weight()[e];
-          const int64_t su = s.scores()[u];
-          const int pu = s.preds()[u];
-          if (su < w || (su == w && pu != -1 && s.scores()[pu] <= st)) {
-            s.scores()[u] = w;
-            s.preds()[u] = t;
-          }
-        }
-        const int pu = s.preds()[u];
-        if (pu != -1) {
-          s.scores()[u] = s.scores()[u] + s.scores()[pu];
-        }
-        if (best_score < s.scores()[u]) {
-          best_score = s.scores()[u];
-          best_node = u;
-        }
-      }
-      max_node = best_node;
+      wave_sync();
+      clk.lap(R, kPhaseSort);
     }
+    for (int v = lane; v < N; v += kWave) {
+      s.scores()[v] = -1;
+      s.preds()[v] = -1;
+    }
+    wave_sync();
+    max_node = 0;
+    bool top_tied = false;
+    for (int r = 0; r < N; ++r) {
+      const int v = uniform(s.rank_to_node()[r]);
+      for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
+        const int t = uniform(s.e_tail()[e]);
+        const int64_t w = s.e_weight()[e];
+        const int64_t sv = s.scores()[v];
+        const int pv = s.preds()[v];
+        if (sv < w || (sv == w && pv != -1 && s.scores()[pv] <= s.scores()[t])) {
+          s.scores()[v] = w;
+          s.preds()[v] = t;
+        }
+      }
+      const int pv = s.preds()[v];
+      if (pv != -1) {
+        s.scores()[v] = s.scores()[v] + s.scores()[pv];
+      }
+      if (s.scores()[max_node] < s.scores()[v]) {
+        max_node = v;
+        top_tied = false;
+      } else if (v != max_node && s.scores()[max_node] == s.scores()[v]) {
+        top_tied = true;
+      }
+    }
+    if (!exact && (top_tied || s.out_head()[max_node] != -1)) continue;
+    if (s.out_head()[max_node] != -1) {
+      for (int r = lane; r < N; r += kWave) {
+        s.node_rank()[s.rank_to_node()[r]] = r;
+      }
+      wave_sync();
+      int guard = 0;
+      while (s.out_head()[max_node] != -1) {
+        if (++guard > N + 1) {
+          R.status = kPoaLoopLimit;
+          return;
+        }
+        // branch completion from the rank of max_node
+        const int rank = s.node_rank()[max_node];
+        const int v0 = max_node;
+        for (int e = uniform(s.out_head()[v0]); e != -1; e = uniform(s.e_next_out()[e])) {
+          const int h = uniform(s.e_head()[e]);
+          for (int f = uniform(s.in_head()[h]); f != -1; f = uniform(s.e_next_in()[f])) {
+            const int t = uniform(s.e_tail()[f]);
+            if (t != v0) {
+              s.scores()[t] = -1;
+            }
+          }
+        }
+        int64_t best_score = 0;
+        int best_node = 0;
+        for (int r = rank + 1; r < N; ++r) {
+          const int u = uniform(s.rank_to_node()[r]);
+          s.scores()[u] = -1;
+          s.preds()[u] = -1;
+          for (int e = uniform(s.in_head()[u]); e != -1; e = uniform(s.e_next_in()[e])) {
+            const int t = uniform(s.e_tail()[e]);
+            const int64_t st = s.scores()[t];
+            if (st == -1) {
+              continue;
+            }
+            const int64_t w = s.e_weight()[e];
+            const int64_t su = s.scores()[u];
+            const int pu = s.preds()[u];
+            if (su < w || (su == w && pu != -1 && s.scores()[pu] <= st)) {
+              s.scores()[u] = w;
+              s.preds()[u] = t;
+            }
+          }
+          const int pu = s.preds()[u];
+          if (pu != -1) {
+            s.scores()[u] = s.scores()[u] + s.scores()[pu];
+          }
+          if (best_score < s.scores()[u]) {
+            best_score = s.scores()[u];
+            best_node = u;
+          }
+        }
+        max_node = best_node;
+      }
+    }
+    break;
   }
   // path back from max_node
   int plen = 0;
