@@ -818,16 +818,23 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
   }
 }
 
-// One DP row with a single predecessor in packed pairs: the moves, codes,
-// stores and end-column value of finish_row<W, false>, two columns per
-// instruction. Columns are kept as H - j * gap through the horizontal
-// chain, so every value of a live column lies within
+// One DP row in packed pairs: the moves, codes, stores and end-column value
+// of finish_row, two columns per instruction. pp holds the predecessor
+// scores at the strip's columns and hp0 (low half) the one at column j0 - 1.
+// With one predecessor (kMulti false) every diagonal and vertical move names
+// it through pay0. With several, pp and hp0 hold the best predecessor score
+// of each column and kk, k0 the first predecessor k (in-edge order) that
+// reaches it: the diagonal into column j and the vertical into column j - 1
+// both take the first predecessor with the best H[p][j - 1], which is the
+// predecessor spoa's traceback prefers. Columns are kept as H - j * gap
+// through the horizontal chain, so every value of a live column lies within
 // max_abs * (V + 2L + 2) of zero; the caller takes this path only when that
 // fits 16 bits. Columns past L compute unused values (they flow only into
 // later columns).
-template <int W>
+template <int W, bool kMulti>
 __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const uint32_t* pp,
-                                              uint32_t hp0, uint32_t b, uint32_t pay0, int lane,
+                                              uint32_t hp0, const uint32_t* kk, uint32_t k0,
+                                              uint32_t b, uint32_t pay0, int lane,
                                               int j0, bool live, int L, int32_t gap, int own,
                                               int pitch, int16_t* ring, int32_t* Hg, uint16_t* Cr,
                                               int end_lane, int end_u) {
@@ -855,12 +862,22 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
 #pragma unroll
   for (int q = 0; q < H2; ++q) {
     h[q] = pk_add(pk_max_lo(pre[q], static_cast<uint32_t>(excl)), ps.cg[q]);
-    // diagonal first, then vertical, else horizontal: with 0/1 flags
-    // fd (not diagonal) and fv (not vertical) the code is
-    // pay + fd * (0x4000 + fv * (0x4000 - pay))
+    // diagonal first, then vertical, else horizontal, from 0/1 flags
+    // fd (not diagonal) and fv (not vertical)
     const uint32_t fd = pk_min_u(pk_sub(h[q], bd[q]), ps.one);
     const uint32_t fv = pk_min_u(pk_sub(h[q], bv[q]), ps.one);
-    cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+    if (kMulti) {
+      // codes of the first best predecessor: kCodeViaK | k
+      const uint32_t kd = q == 0 ? ((kk[0] << 16) | (k0 & 0xffffu))
+                                 : __builtin_amdgcn_alignbit(kk[q], kk[q - 1], 16);
+      const uint32_t dc = pk_add(kd, 0x20002000u);     // diagonal | kCodeViaK | k
+      const uint32_t vc = pk_add(kk[q], 0x60006000u);  // vertical | kCodeViaK | k
+      const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
+      cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
+    } else {
+      // pay + fd * (0x4000 + fv * (0x4000 - pay))
+      cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+    }
   }
   if (live) {
 #pragma unroll
@@ -884,7 +901,8 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
   if (lane == 0) {
     if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(h0);
     if (own == kSlotGlobal) Hg[0] = h0;
-    Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | pay0);
+    Cr[pitch - 1] = static_cast<uint16_t>(
+        kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu)) : (kCodeVert | pay0));
   }
   uint32_t lastp = h[0];
 #pragma unroll
@@ -975,8 +993,32 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
         uint32_t pp[W / 2];
         const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pp);
-        h_last = packed_row<W>(ps, pp, hp0, b, pay0, lane, j0, live, L, gap, own, pitch, ring, Hg,
-                               Cr, end_lane, end_u);
+        h_last = packed_row<W, false>(ps, pp, hp0, pp, 0u, b, pay0, lane, j0, live, L, gap, own,
+                                      pitch, ring, Hg, Cr, end_lane, end_u);
+      } else if (kPacked) {
+        // best predecessor score of every column and the first predecessor
+        // reaching it (a later one replaces it only when strictly better)
+        uint32_t mx[W / 2], kx[W / 2];
+        uint32_t m0 = load_pairs<W>(ring, pitch, s, pred_at(0), j0, jr, L, gap, ps, mx);
+        uint32_t k0 = 0;
+#pragma unroll
+        for (int q = 0; q < W / 2; ++q) kx[q] = 0;
+        for (int k = 1; k < np; ++k) {
+          uint32_t pp[W / 2];
+          const uint32_t hk = load_pairs<W>(ring, pitch, s, pred_at(k), j0, jr, L, gap, ps, pp);
+          const uint32_t ks = static_cast<uint32_t>(k) * 0x10001u;
+#pragma unroll
+          for (int q = 0; q < W / 2; ++q) {
+            const uint32_t m = pk_max(mx[q], pp[q]);
+            kx[q] = pk_mad(pk_min_u(pk_sub(m, mx[q]), ps.one), pk_sub(ks, kx[q]), kx[q]);
+            mx[q] = m;
+          }
+          const uint32_t m = pk_max(m0, hk);
+          k0 = pk_mad(pk_min_u(pk_sub(m, m0), ps.one), pk_sub(ks, k0), k0);
+          m0 = m;
+        }
+        h_last = packed_row<W, true>(ps, mx, m0, kx, k0, b, 0u, lane, j0, live, L, gap, own, pitch,
+                                     ring, Hg, Cr, end_lane, end_u);
       } else if (np == 1) {
         // move-code payload of the single predecessor: its row delta
         const uint32_t delta = static_cast<uint32_t>(r - (pred_at(0) & kPredRowMask));
@@ -1389,11 +1431,13 @@ __device__ __forceinline__ int32_t wave_prefix_sum(int32_t x) {
 // membership checks are skipped. Returns the largest predecessor count.
 // Each lane takes kU rows per round (64 kU rows) and walks their edge lists
 // in lockstep, one edge of every list per step, so that the loads of the
-// kU lists are in flight together. Out of line, like the DP, for a
-// register allocation of its own.
+// kU lists are in flight together; the walk keeps the first kP predecessor
+// rows of each row, so only rows with more walk their in-edges again. Out
+// of line, like the DP, for a register allocation of its own.
 template <bool kFull>
 __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V, int n_slots) {
   constexpr int kU = 4;
+  constexpr int kP = 4;
   const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
   V = uniform(V);
   n_slots = uniform(n_slots);
@@ -1411,12 +1455,18 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   int32_t* row_meta = s.row_meta();
   int32_t* pred_off = s.pred_off();
   int32_t* pred_rows = s.pred_rows();
+  uint64_t* row_pred4 = s.row_pred4();
   auto included = [&](int v) { return kFull || in_sub[v] != 0; };
+  // predecessor entry of DP row pr: the row and the slot it lives in
+  auto pred_entry = [&](int pr) {
+    return pr | (((row_meta[pr - 1] >> kRowMetaSlotShift) & 31) << kPredRowBits);
+  };
   int run = 0;
   int max_np = 0;
   int busy = lane < n_slots ? 0 : 0x7fffffff;
   for (int r0 = 0; r0 < V; r0 += kU * kWave) {
-    int v[kU], ei[kU], eo[kU], cnt[kU], oc[kU], lu[kU], meta[kU];
+    int v[kU], ei[kU], eo[kU], np[kU], oc[kU], lu[kU], meta[kU], off[kU];
+    int pr[kU][kP];
 #pragma unroll
     for (int i = 0; i < kU; ++i) {
       const int r = r0 + i * kWave + lane;
@@ -1427,9 +1477,11 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
       ei[i] = v[i] >= 0 ? in_head[v[i]] : -1;
       eo[i] = v[i] >= 0 ? out_head[v[i]] : -1;
       meta[i] = v[i] >= 0 ? static_cast<int>(base[v[i]]) : 0;
-      cnt[i] = 0;
+      np[i] = 0;
       oc[i] = 0;
       lu[i] = 0;
+#pragma unroll
+      for (int j = 0; j < kP; ++j) pr[i][j] = 0;
     }
     for (;;) {
       bool act = false;
@@ -1452,12 +1504,22 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
           no[i] = e_next_out[eo[i]];
         }
       }
+      int rt[kU], rh[kU];
 #pragma unroll
       for (int i = 0; i < kU; ++i) {
-        if (t[i] != -1 && included(t[i])) ++cnt[i];
-        if (h[i] != -1 && included(h[i])) {
+        rt[i] = t[i] != -1 && included(t[i]) ? row_of[t[i]] : 0;
+        rh[i] = h[i] != -1 && included(h[i]) ? row_of[h[i]] : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        if (rt[i] > 0) {
+#pragma unroll
+          for (int j = 0; j < kP; ++j) pr[i][j] = np[i] == j ? rt[i] : pr[i][j];
+          ++np[i];
+        }
+        if (rh[i] > 0) {
           ++oc[i];
-          lu[i] = max(lu[i], row_of[h[i]]);
+          lu[i] = max(lu[i], rh[i]);
         }
         ei[i] = ni[i];
         eo[i] = no[i];
@@ -1465,11 +1527,8 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
     }
 #pragma unroll
     for (int i = 0; i < kU; ++i) {
-      if (v[i] >= 0) {
-        cnt[i] = max(cnt[i], 1);  // the virtual start row
-        meta[i] |= oc[i] == 0 ? kRowMetaSink : 0;
-      }
-      max_np = max(max_np, cnt[i]);
+      if (v[i] >= 0) meta[i] |= oc[i] == 0 ? kRowMetaSink : 0;
+      max_np = max(max_np, v[i] >= 0 ? max(np[i], 1) : 0);  // row 0 stands in for none
     }
 #pragma unroll
     for (int i = 0; i < kU; ++i) {
@@ -1491,76 +1550,53 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
         }
       }
       const int r = c0 + lane;
-      const int x = wave_prefix_sum(cnt[i]);
+      const int cnt = v[i] >= 0 ? max(np[i], 1) : 0;
+      const int x = wave_prefix_sum(cnt);
+      off[i] = run + x - cnt;
       if (r < V) {
         row_meta[r] = meta[i] | (my_slot << kRowMetaSlotShift);
-        pred_off[r] = run + x - cnt[i];
+        pred_off[r] = off[i];
       }
       run += __builtin_amdgcn_readlane(x, kWave - 1);
+    }
+    // predecessor entries (their slots are all assigned by now: predecessor
+    // rows come first)
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      if (v[i] < 0) continue;
+      const int r = r0 + i * kWave + lane;
+      uint64_t deltas = 0;  // row deltas of the first four predecessors (0: none or too far)
+      if (np[i] == 0) {
+        pred_rows[off[i]] = 0;
+        deltas = static_cast<uint64_t>(r + 1 <= 0xffff ? r + 1 : 0);
+      } else if (np[i] <= kP) {
+#pragma unroll
+        for (int j = 0; j < kP; ++j) {
+          if (j < np[i]) {
+            pred_rows[off[i] + j] = pred_entry(pr[i][j]);
+            const int d = r + 1 - pr[i][j];
+            if (d <= 0xffff) deltas |= static_cast<uint64_t>(d) << (16 * j);
+          }
+        }
+      } else {
+        int o = off[i];
+        int k = 0;
+        for (int e = in_head[v[i]]; e != -1; e = e_next_in[e]) {
+          const int u = e_tail[e];
+          if (included(u)) {
+            const int p = row_of[u];
+            pred_rows[o++] = pred_entry(p);
+            const int d = r + 1 - p;
+            if (k < 4 && d <= 0xffff) deltas |= static_cast<uint64_t>(d) << (16 * k);
+            ++k;
+          }
+        }
+      }
+      row_pred4[r] = deltas;
     }
   }
   if (lane == 0) {
     pred_off[V] = run;
-  }
-  wave_sync();
-  uint64_t* row_pred4 = s.row_pred4();
-  for (int r0 = 0; r0 < V; r0 += kU * kWave) {
-    int v[kU], e[kU], o[kU], k[kU];
-    uint64_t deltas[kU];  // row deltas of the first four predecessors (0: none or too far)
-#pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      const int r = r0 + i * kWave + lane;
-      v[i] = r < V ? sub_order[r] : -1;
-      o[i] = r < V ? pred_off[r] : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      e[i] = v[i] >= 0 ? in_head[v[i]] : -1;
-      k[i] = 0;
-      deltas[i] = 0;
-    }
-    for (;;) {
-      bool act = false;
-#pragma unroll
-      for (int i = 0; i < kU; ++i) act = act || e[i] != -1;
-      if (__ballot(act) == 0) break;
-      int t[kU], nx[kU], pr[kU];
-#pragma unroll
-      for (int i = 0; i < kU; ++i) {
-        t[i] = -1;
-        nx[i] = -1;
-        if (e[i] != -1) {
-          t[i] = e_tail[e[i]];
-          nx[i] = e_next_in[e[i]];
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < kU; ++i) {
-        pr[i] = t[i] != -1 && included(t[i]) ? row_of[t[i]] : 0;
-      }
-#pragma unroll
-      for (int i = 0; i < kU; ++i) {
-        if (pr[i] > 0) {
-          const int sl = (row_meta[pr[i] - 1] >> kRowMetaSlotShift) & 31;
-          pred_rows[o[i]++] = pr[i] | (sl << kPredRowBits);
-          const int d = r0 + i * kWave + lane + 1 - pr[i];
-          if (k[i] < 4 && d <= 0xffff) deltas[i] |= static_cast<uint64_t>(d) << (16 * k[i]);
-          ++k[i];
-        }
-        e[i] = nx[i];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kU; ++i) {
-      const int r = r0 + i * kWave + lane;
-      if (v[i] >= 0) {
-        if (k[i] == 0) {
-          pred_rows[o[i]] = 0;
-          deltas[i] = static_cast<uint64_t>(r + 1 <= 0xffff ? r + 1 : 0);
-        }
-        row_pred4[r] = deltas[i];
-      }
-    }
   }
   return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
 }
