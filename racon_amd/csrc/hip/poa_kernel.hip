@@ -76,6 +76,14 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// v_writelane: `old` with lane `lane` set to the uniform `value` (the
+// intrinsic has no clang builtin here, so it is bound by name)
+extern "C" __device__ int32_t rh_amdgcn_writelane(int32_t value, int32_t lane, int32_t old) __asm(
+    "llvm.amdgcn.writelane.i32");
+__device__ __forceinline__ int lane_write(int value, int lane, int old) {
+  return rh_amdgcn_writelane(value, lane, old);
+}
+
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Phase clock: accumulates device constant-clock ticks into R.phase[k].
@@ -1535,18 +1543,18 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
       const int c0 = r0 + i * kWave;
       if (c0 >= V) break;
       int my_slot = kSlotNone;
-      const int n_chunk = min(kWave, V - c0);
+      const int n_chunk = uniform(min(kWave, V - c0));
       for (int rr = 0; rr < n_chunk; ++rr) {
         const int l = __builtin_amdgcn_readlane(lu[i], rr);
         if (l > 0) {
           const int row = c0 + rr + 1;
           const unsigned long long free_mask = __ballot(busy < row);
-          int sl = kSlotGlobal;
-          if (free_mask != 0) {
-            sl = __ffsll(static_cast<long long>(free_mask)) - 1;
-            if (lane == sl) busy = l;
-          }
-          if (lane == rr) my_slot = sl;
+          // the lowest free slot, else the global matrix (then the write
+          // goes to lane 63, which holds no slot and stays never free)
+          const bool none = free_mask == 0;
+          const int sl = none ? kSlotGlobal : __ffsll(static_cast<long long>(free_mask)) - 1;
+          busy = lane_write(none ? 0x7fffffff : l, none ? kWave - 1 : sl, busy);
+          my_slot = lane_write(sl, rr, my_slot);
         }
       }
       const int r = c0 + lane;
