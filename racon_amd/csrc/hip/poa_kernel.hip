@@ -1248,15 +1248,48 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
 // the position-by-position insertion: node ids and edge ids are handed out
 // in position order with wave prefix counts, and the anchor of an insertion
 // (the rank bound of the last non-insertion position before it) is a wave
-// prefix scan. Returns 1 when two path nodes share an aligned set (nothing
-// in the graph has changed then), 0 when inserted, or a negative status.
-__device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8_t* lseq, int L,
-                                               bool* order_ok) {
+// prefix scan. The same independence lets each pass load what kU chunks of
+// 64 positions read before any of their writes, so those loads are in
+// flight together. Status 1 when two path nodes share an aligned set
+// (nothing in the graph has changed then), 0 when inserted (with the new
+// node and edge counts), or a negative status. Out of line, for a register
+// allocation of its own.
+struct InsertResult {
+  int32_t status;
+  int32_t n_nodes;
+  int32_t n_edges;
+  int32_t order_ok;
+};
+
+__device__ __noinline__ InsertResult insert_parallel(GByte* slot_base, CLayout* layout,
+                                                     uint32_t lseq_addr, int L, int n_nodes,
+                                                     int n_edges, int n_seqs, int cap_nodes,
+                                                     int cap_edges) {
+  constexpr int kU = 4;   // chunks per round in the edge and label passes
+  constexpr int kUA = 2;  // in the passes that hold aligned sets
+  constexpr int kA = static_cast<int>(kMaxAligned);
+  const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
+  const uint8_t* lseq = (const uint8_t*)(LByte*)(size_t)uniform(static_cast<int>(lseq_addr));
+  L = uniform(L);
+  n_nodes = uniform(n_nodes);
+  n_edges = uniform(n_edges);
+  n_seqs = uniform(n_seqs);
+  cap_nodes = uniform(cap_nodes);
+  cap_edges = uniform(cap_edges);
+  InsertResult res;
+  res.status = 0;
+  res.n_nodes = n_nodes;
+  res.n_edges = n_edges;
+  res.order_ok = 1;
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   const int32_t* pnode = s.path();  // aligned graph node of each layer position (-1: none)
   int32_t* pnv = s.lpath();         // graph node of each layer position after insertion
   uint8_t* stamp = s.in_sub();      // 2 on the path's nodes (the DP no longer needs it)
+  uint8_t* base = s.base();
+  uint8_t* n_aligned = s.n_aligned();
+  int32_t* aligned = s.aligned();
+  int32_t* node_rank = s.node_rank();
   for (int p = lane; p < L; p += kWave) {
     const int v = pnode[p];
     if (v != -1) stamp[v] = 2;
@@ -1265,158 +1298,249 @@ __device__ __forceinline__ int insert_parallel(Slot& s, CCaps& caps, const uint8
   // existing node per position (match, or an aligned node with the letter)
   bool conflict = false, full = false;
   int n_new = 0;
-  for (int p0 = 0; p0 < L; p0 += kWave) {
-    const int p = p0 + lane;
-    bool isnew = false;
-    if (p < L) {
-      const int v = pnode[p];
-      int nv = -1;
-      if (v != -1) {
-        const uint8_t letter = lseq[p];
-        const int cnt = s.n_aligned()[v];
-        nv = s.base()[v] == letter ? v : -1;
-        for (int k = 0; k < cnt; ++k) {
-          const int a = s.aligned()[v * kMaxAligned + k];
-          conflict |= stamp[a] == 2;
-          if (nv == -1 && s.base()[a] == letter) nv = a;
-        }
-        full |= nv == -1 && cnt + 1 > static_cast<int>(kMaxAligned);
-      }
-      isnew = nv == -1;
-      pnv[p] = nv;
+  for (int p0 = 0; p0 < L; p0 += kUA * kWave) {
+    int v[kUA], cnt[kUA], nv[kUA], letter[kUA], al[kUA][kA];
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      const int p = p0 + i * kWave + lane;
+      v[i] = p < L ? pnode[p] : -1;
+      letter[i] = p < L ? static_cast<int>(lseq[p]) : 0;
     }
-    n_new += __popcll(__ballot(isnew));
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      cnt[i] = v[i] != -1 ? static_cast<int>(n_aligned[v[i]]) : 0;
+      nv[i] = v[i] != -1 && static_cast<int>(base[v[i]]) == letter[i] ? v[i] : -1;
+#pragma unroll
+      for (int k = 0; k < kA; ++k) al[i][k] = v[i] != -1 ? aligned[v[i] * kA + k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      int st[kA], ba[kA];
+#pragma unroll
+      for (int k = 0; k < kA; ++k) {
+        st[k] = k < cnt[i] ? static_cast<int>(stamp[al[i][k]]) : 0;
+        ba[k] = k < cnt[i] ? static_cast<int>(base[al[i][k]]) : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < kA; ++k) {
+        conflict |= st[k] == 2;
+        if (nv[i] == -1 && ba[k] == letter[i]) nv[i] = al[i][k];
+      }
+      full |= v[i] != -1 && nv[i] == -1 && cnt[i] + 1 > kA;
+      const int p = p0 + i * kWave + lane;
+      if (p < L) pnv[p] = nv[i];
+      n_new += __popcll(__ballot(p < L && nv[i] == -1));
+    }
   }
   if (__ballot(conflict) != 0) {
-    return 1;
+    res.status = 1;
+    return res;
   }
-  if (__ballot(full) != 0 || s.n_nodes + n_new > static_cast<int>(caps.nodes)) {
-    return kPoaNodeOverflow;
+  if (__ballot(full) != 0 || n_nodes + n_new > cap_nodes) {
+    res.status = kPoaNodeOverflow;
+    return res;
   }
   wave_sync();
   // new nodes (ids in position order), aligned sets, anchors
-  const int before = s.n_nodes;
+  const int before = n_nodes;
   int next_id = before;
   int carry = -1;  // rank bound of the last non-insertion position so far
-  for (int p0 = 0; p0 < L; p0 += kWave) {
-    const int p = p0 + lane;
-    const bool in = p < L;
-    const int v = in ? pnode[p] : -1;
-    int nv = in ? pnv[p] : 0;
-    const bool isnew = in && nv == -1;
-    const unsigned long long m = __ballot(isnew);
-    if (isnew) nv = next_id + __popcll(m & below);
-    next_id += __popcll(m);
-    int bound = -1;  // highest rank in v's aligned set, v included
-    if (v != -1) {
-      bound = s.node_rank()[v];
-      const int cnt = s.n_aligned()[v];
-      for (int k = 0; k < cnt; ++k) {
-        bound = max(bound, s.node_rank()[s.aligned()[v * kMaxAligned + k]]);
-      }
+  for (int p0 = 0; p0 < L; p0 += kUA * kWave) {
+    int v[kUA], nv[kUA], cnt[kUA], bound[kUA], al[kUA][kA];
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      const int p = p0 + i * kWave + lane;
+      v[i] = p < L ? pnode[p] : -1;
+      nv[i] = p < L ? pnv[p] : 0;
     }
-    const int incl = wave_prefix_max(v != -1 ? lane : -1);
-    const int excl = wave_shift_right(incl, -1);
-    const int from_lane = __shfl(bound, max(excl, 0), kWave);
-    const int prev_bound = excl >= 0 ? from_lane : carry;
-    if (isnew) {
-      s.anchor()[nv - before] = v != -1 ? bound : prev_bound;
-      s.base()[nv] = lseq[p];
-      s.in_head()[nv] = -1;
-      s.out_head()[nv] = -1;
-      s.in_tail()[nv] = -1;
-      s.out_tail()[nv] = -1;
-      s.n_aligned()[nv] = 0;
-      s.cov()[nv] = 0;
-      s.last_label()[nv] = -1;
-      if (v != -1) {
-        // the mismatch node joins v's aligned set
-        const int cnt = s.n_aligned()[v];
-        for (int k = 0; k < cnt; ++k) {
-          const int a = s.aligned()[v * kMaxAligned + k];
-          const int ca = s.n_aligned()[a];
-          s.aligned()[nv * kMaxAligned + k] = a;
-          s.aligned()[a * kMaxAligned + ca] = nv;
-          s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      cnt[i] = v[i] != -1 ? static_cast<int>(n_aligned[v[i]]) : 0;
+      bound[i] = v[i] != -1 ? node_rank[v[i]] : -1;
+#pragma unroll
+      for (int k = 0; k < kA; ++k) al[i][k] = v[i] != -1 ? aligned[v[i] * kA + k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      int rk[kA];
+#pragma unroll
+      for (int k = 0; k < kA; ++k) rk[k] = k < cnt[i] ? node_rank[al[i][k]] : -1;
+#pragma unroll
+      for (int k = 0; k < kA; ++k) bound[i] = max(bound[i], rk[k]);  // highest rank in v's set
+    }
+#pragma unroll
+    for (int i = 0; i < kUA; ++i) {
+      const int p = p0 + i * kWave + lane;
+      const bool in = p < L;
+      const bool isnew = in && nv[i] == -1;
+      const unsigned long long m = __ballot(isnew);
+      if (isnew) nv[i] = next_id + __popcll(m & below);
+      next_id += __popcll(m);
+      const int incl = wave_prefix_max(v[i] != -1 ? lane : -1);
+      const int excl = wave_shift_right(incl, -1);
+      const int from_lane = __shfl(bound[i], max(excl, 0), kWave);
+      const int prev_bound = excl >= 0 ? from_lane : carry;
+      if (isnew) {
+        const int id = nv[i];
+        s.anchor()[id - before] = v[i] != -1 ? bound[i] : prev_bound;
+        base[id] = lseq[p];
+        s.in_head()[id] = -1;
+        s.out_head()[id] = -1;
+        s.in_tail()[id] = -1;
+        s.out_tail()[id] = -1;
+        n_aligned[id] = 0;
+        s.cov()[id] = 0;
+        s.last_label()[id] = -1;
+        if (v[i] != -1) {
+          // the mismatch node joins v's aligned set
+          const int c = cnt[i];
+          for (int k = 0; k < c; ++k) {
+            const int a = al[i][k];
+            const int ca = n_aligned[a];
+            aligned[id * kA + k] = a;
+            aligned[a * kA + ca] = id;
+            n_aligned[a] = static_cast<uint8_t>(ca + 1);
+          }
+          aligned[id * kA + c] = v[i];
+          n_aligned[id] = static_cast<uint8_t>(c + 1);
+          aligned[v[i] * kA + c] = id;
+          n_aligned[v[i]] = static_cast<uint8_t>(c + 1);
         }
-        s.aligned()[nv * kMaxAligned + cnt] = v;
-        s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
-        s.aligned()[v * kMaxAligned + cnt] = nv;
-        s.n_aligned()[v] = static_cast<uint8_t>(cnt + 1);
       }
+      if (in) pnv[p] = nv[i];
+      const int last = __builtin_amdgcn_readlane(incl, kWave - 1);
+      if (last >= 0) carry = __builtin_amdgcn_readlane(bound[i], last);
     }
-    if (in) pnv[p] = nv;
-    const int last = __builtin_amdgcn_readlane(incl, kWave - 1);
-    if (last >= 0) carry = __builtin_amdgcn_readlane(bound, last);
   }
-  s.n_nodes = next_id;
+  res.n_nodes = next_id;
   wave_sync();
   // anchors of the new nodes must be non-decreasing for the merged order
   bool bad = false;
   for (int k = 1 + lane; k < n_new; k += kWave) {
     bad |= s.anchor()[k] < s.anchor()[k - 1];
   }
-  *order_ok = __ballot(bad) == 0;
+  res.order_ok = __ballot(bad) == 0 ? 1 : 0;
   // edges between consecutive positions: weight w[p-1] + w[p]
-  for (int p0 = 1; p0 < L; p0 += kWave) {
-    const int p = p0 + lane;
-    const bool in = p < L;
-    int tail = 0, head = 0, w = 0, found = -1;
-    if (in) {
-      tail = pnv[p - 1];
-      head = pnv[p];
-      w = s.qual()[p - 1] + s.qual()[p];
-      for (int e = s.out_head()[tail]; e != -1; e = s.e_next_out()[e]) {
-        if (s.e_head()[e] == head) {
-          found = e;
-          break;
+  int32_t* e_tail = s.e_tail();
+  int32_t* e_head = s.e_head();
+  int32_t* e_weight = s.e_weight();
+  int32_t* e_next_in = s.e_next_in();
+  int32_t* e_next_out = s.e_next_out();
+  int32_t* out_head = s.out_head();
+  int32_t* out_tail = s.out_tail();
+  int32_t* in_head = s.in_head();
+  int32_t* in_tail = s.in_tail();
+  for (int p0 = 1; p0 < L; p0 += kU * kWave) {
+    int tail[kU], head[kU], w[kU], found[kU], e[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int p = p0 + i * kWave + lane;
+      const bool in = p < L;
+      tail[i] = in ? pnv[p - 1] : 0;
+      head[i] = in ? pnv[p] : 0;
+      w[i] = in ? s.qual()[p - 1] + s.qual()[p] : 0;
+      found[i] = -1;
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int p = p0 + i * kWave + lane;
+      e[i] = p < L ? out_head[tail[i]] : -1;
+    }
+    // out-edge lists of the kU tails in lockstep
+    for (;;) {
+      bool act = false;
+#pragma unroll
+      for (int i = 0; i < kU; ++i) act = act || e[i] != -1;
+      if (__ballot(act) == 0) break;
+      int h[kU], nx[kU];
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        h[i] = -1;
+        nx[i] = -1;
+        if (e[i] != -1) {
+          h[i] = e_head[e[i]];
+          nx[i] = e_next_out[e[i]];
         }
       }
-      if (found != -1) s.e_weight()[found] = s.e_weight()[found] + w;
-    }
-    const bool need = in && found == -1;
-    const unsigned long long m = __ballot(need);
-    if (s.n_edges + __popcll(m) > static_cast<int>(caps.edges)) {
-      return kPoaNodeOverflow;
-    }
-    if (need) {
-      const int e = s.n_edges + __popcll(m & below);
-      s.e_tail()[e] = tail;
-      s.e_head()[e] = head;
-      s.e_weight()[e] = w;
-      s.e_next_in()[e] = -1;
-      s.e_next_out()[e] = -1;
-      const int ot = s.out_tail()[tail];
-      if (ot == -1) {
-        s.out_head()[tail] = e;
-      } else {
-        s.e_next_out()[ot] = e;
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        if (e[i] != -1 && h[i] == head[i]) {
+          found[i] = e[i];
+          nx[i] = -1;
+        }
+        e[i] = nx[i];
       }
-      s.out_tail()[tail] = e;
-      const int it = s.in_tail()[head];
-      if (it == -1) {
-        s.in_head()[head] = e;
-      } else {
-        s.e_next_in()[it] = e;
-      }
-      s.in_tail()[head] = e;
     }
-    s.n_edges += __popcll(m);
+    int ot[kU], it[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int p = p0 + i * kWave + lane;
+      const bool need = p < L && found[i] == -1;
+      ot[i] = need ? out_tail[tail[i]] : -1;
+      it[i] = need ? in_tail[head[i]] : -1;
+      if (found[i] != -1) e_weight[found[i]] = e_weight[found[i]] + w[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      const int p = p0 + i * kWave + lane;
+      const bool need = p < L && found[i] == -1;
+      const unsigned long long m = __ballot(need);
+      if (n_edges + __popcll(m) > cap_edges) {
+        res.status = kPoaNodeOverflow;
+        return res;
+      }
+      if (need) {
+        const int ne = n_edges + __popcll(m & below);
+        e_tail[ne] = tail[i];
+        e_head[ne] = head[i];
+        e_weight[ne] = w[i];
+        e_next_in[ne] = -1;
+        e_next_out[ne] = -1;
+        if (ot[i] == -1) {
+          out_head[tail[i]] = ne;
+        } else {
+          e_next_out[ot[i]] = ne;
+        }
+        out_tail[tail[i]] = ne;
+        if (it[i] == -1) {
+          in_head[head[i]] = ne;
+        } else {
+          e_next_in[it[i]] = ne;
+        }
+        in_tail[head[i]] = ne;
+      }
+      n_edges += __popcll(m);
+    }
   }
+  res.n_edges = n_edges;
   // every node of the layer has an incident edge of this sequence when L > 1
   if (L > 1) {
-    const int label = s.n_seqs;
-    for (int p = lane; p < L; p += kWave) {
-      const int v = pnv[p];
-      if (s.last_label()[v] != label) {
-        s.last_label()[v] = label;
-        s.cov()[v] = s.cov()[v] + 1;
+    const int label = n_seqs;
+    int32_t* last_label = s.last_label();
+    int32_t* cov = s.cov();
+    for (int p0 = 0; p0 < L; p0 += kU * kWave) {
+      int v[kU], ll[kU], cv[kU];
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        const int p = p0 + i * kWave + lane;
+        v[i] = p < L ? pnv[p] : -1;
+      }
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        ll[i] = v[i] != -1 ? last_label[v[i]] : label;
+        cv[i] = v[i] != -1 ? cov[v[i]] : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kU; ++i) {
+        if (ll[i] != label) {
+          last_label[v[i]] = label;
+          cov[v[i]] = cv[i] + 1;
+        }
       }
     }
   }
-  s.n_seqs = s.n_seqs + 1;
   wave_sync();
-  return 0;
+  return res;
 }
 
 // Inclusive prefix sum over the 64 lanes (DPP, as wave_prefix_max; 0 is
@@ -2146,12 +2270,21 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     int n_path = 0;
     const int nodes_before = s.n_nodes;
     {
-      const int rc = insert_parallel(s, caps, lds_seq, L, &order_ok);
+      uint32_t lseq_addr = static_cast<uint32_t>((size_t)(LByte*)lds_seq);
+      asm volatile("" : "+v"(lseq_addr));
+      const InsertResult ir =
+          insert_parallel(s.p, s.L, lseq_addr, L, s.n_nodes, s.n_edges, s.n_seqs,
+                          static_cast<int>(caps.nodes), static_cast<int>(caps.edges));
+      const int rc = uniform(ir.status);
       if (rc < 0) {
         R.status = rc;
         return;
       }
       if (rc == 0) {
+        s.n_nodes = uniform(ir.n_nodes);
+        s.n_edges = uniform(ir.n_edges);
+        s.n_seqs = s.n_seqs + 1;
+        order_ok = uniform(ir.order_ok) != 0;
         n_path = L;
       } else {
         s = insert_sequential(s, caps, L, &order_ok, &n_path);
