@@ -760,8 +760,7 @@ __device__ __forceinline__ int32_t pk_half(uint32_t x, int hi) {
 template <int W>
 struct PackedStrip {
   uint32_t base[W / 2];  // layer bytes of the pair, one per half
-  uint32_t cg[W / 2];    // j * gap
-  uint32_t match;        // match and gap in both halves
+  uint32_t match_gap;    // match - gap and gap in both halves
   uint32_t gap;
   uint32_t one;          // 1 in both halves, opaque to the optimizer (a
   uint32_t c4000;        // visible 1 turns min(x, 1) into compares and selects)
@@ -774,10 +773,8 @@ __device__ __forceinline__ void packed_strip_init(PackedStrip<W>& ps, const uint
 #pragma unroll
   for (int q = 0; q < W / 2; ++q) {
     ps.base[q] = __builtin_amdgcn_perm(sw[q / 2], sw[q / 2], (q & 1) ? 0x0c030c02u : 0x0c010c00u);
-    const int32_t j = j0 + 2 * q;
-    ps.cg[q] = (static_cast<uint32_t>(j * gap) & 0xffffu) | (static_cast<uint32_t>((j + 1) * gap) << 16);
   }
-  ps.match = pk_splat(match);
+  ps.match_gap = pk_splat(match - gap);
   ps.gap = pk_splat(gap);
   ps.one = 0x00010001u;
   asm volatile("" : "+s"(ps.one));
@@ -786,9 +783,10 @@ __device__ __forceinline__ void packed_strip_init(PackedStrip<W>& ps, const uint
   ps.diff = pk_splat(mismatch - match);
 }
 
-// Predecessor row p (entry e) at the lane's strip as pairs:
-// pp[q] = (H[p][j0 + 2q], H[p][j0 + 2q + 1]); returns H[p][j0 - 1] in the
-// low half. Row 0 is the virtual start row (j * gap).
+// Predecessor row p (entry e) at the lane's strip as pairs of the row's
+// stored values G[p][j] = H[p][j] - j * gap: pp[q] = (G[p][j0 + 2q],
+// G[p][j0 + 2q + 1]); returns G[p][j0 - 1] in the low half. Row 0 is the
+// virtual start row (H = j * gap, so G = 0).
 template <int W>
 __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, const Slot& s, int e,
                                                int j0, int jr, int L, int32_t gap,
@@ -797,8 +795,8 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
   const int sl = static_cast<unsigned>(e) >> kPredRowBits;
   if (p == 0) {
 #pragma unroll
-    for (int q = 0; q < W / 2; ++q) pp[q] = ps.cg[q];
-    return static_cast<uint32_t>((j0 - 1) * gap);
+    for (int q = 0; q < W / 2; ++q) pp[q] = 0u;
+    return 0u;
   } else if (sl < kSlotGlobal) {
     const int16_t* row = ring + sl * pitch + kColShift;
 #pragma unroll
@@ -834,11 +832,13 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
 // of each column and kk, k0 the first predecessor k (in-edge order) that
 // reaches it: the diagonal into column j and the vertical into column j - 1
 // both take the first predecessor with the best H[p][j - 1], which is the
-// predecessor spoa's traceback prefers. Columns are kept as H - j * gap
-// through the horizontal chain, so every value of a live column lies within
-// max_abs * (V + 2L + 2) of zero; the caller takes this path only when that
-// fits 16 bits. Columns past L compute unused values (they flow only into
-// later columns).
+// predecessor spoa's traceback prefers. Every value is kept as
+// G = H - j * gap (the rows in LDS or global memory too): a horizontal move
+// keeps G, a vertical one adds gap and a diagonal one adds s - gap, so the
+// horizontal chain is a plain prefix max, and every value of a live column
+// lies within max_abs * (V + 2L + 2) of zero; the caller takes this path only
+// when that fits 16 bits. Columns past L compute unused values (they flow
+// only into later columns). Returns H[row][L].
 template <int W, bool kMulti>
 __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const uint32_t* pp,
                                               uint32_t hp0, const uint32_t* kk, uint32_t k0,
@@ -858,9 +858,9 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
     const uint32_t d = q == 0 ? ((pp[0] << 16) | (hp0 & 0xffffu))
                               : __builtin_amdgcn_alignbit(pp[q], pp[q - 1], 16);
     const uint32_t mis = pk_min_u(ps.base[q] ^ bb, ps.one);  // 1 where the bases differ
-    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match));
+    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match_gap));
     bv[q] = pk_add(pp[q], ps.gap);
-    carry = pk_max_hi(pk_scan(pk_sub(pk_max(bd[q], bv[q]), ps.cg[q])), carry);
+    carry = pk_max_hi(pk_scan(pk_max(bd[q], bv[q])), carry);
     pre[q] = carry;
   }
   const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
@@ -869,7 +869,7 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
   uint32_t h[H2], cd[H2];
 #pragma unroll
   for (int q = 0; q < H2; ++q) {
-    h[q] = pk_add(pk_max_lo(pre[q], static_cast<uint32_t>(excl)), ps.cg[q]);
+    h[q] = pk_max_lo(pre[q], static_cast<uint32_t>(excl));
     // diagonal first, then vertical, else horizontal, from 0/1 flags
     // fd (not diagonal) and fv (not vertical)
     const uint32_t fd = pk_min_u(pk_sub(h[q], bd[q]), ps.one);
@@ -917,7 +917,7 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
   for (int q = 1; q < H2; ++q) {
     if (q == (end_u >> 1)) lastp = h[q];
   }
-  return __builtin_amdgcn_readlane(pk_half(lastp, end_u & 1), end_lane);
+  return __builtin_amdgcn_readlane(pk_half(lastp, end_u & 1), end_lane) + L * gap;
 }
 
 // DP rows of one layer with 16-bit scores, every row in one pass of the
