@@ -1944,6 +1944,113 @@ __device__ __forceinline__ int sink_tie_by_sets(const Slot& s, int V, bool full,
   return best_at;
 }
 
+// Heaviest-bundle scores over a topological rank order (spoa's
+// traverse_heaviest_bundle scan): a node takes the in-edge of the highest
+// weight, a later in-edge of equal weight when its tail scores at least as
+// much, and adds that tail's score; nodes without in-edges keep -1. The
+// recurrence is sequential in rank order, but its inputs are not: each round
+// loads, for 64 ranks at once, up to kE in-edges per node (tail, weight,
+// tail rank) and the scores of tails from earlier rounds; the scan then runs
+// on lane registers (scores of this round's ranks are written into a lane
+// register as they are found). Nodes with more in-edges read the rest from
+// memory. Scores are 32-bit: the caller takes this path only when every
+// path weight fits (|edge weight| <= 252 per inserted position). Writes
+// scores[] and preds[]; returns the first node in rank order with the top
+// score, with bit 31 set when another node ties it, or kScanUnordered (and
+// nothing usable) when some tail does not precede its node in node_rank.
+constexpr uint32_t kScanUnordered = 0xffffffffu;
+
+__device__ __noinline__ uint32_t bundle_scan(GByte* slot_base, CLayout* layout, int N) {
+  constexpr int kE = 4;
+  const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
+  N = uniform(N);
+  const int lane = lane_id();
+  const int32_t* r2n = s.rank_to_node();
+  const int32_t* in_head = s.in_head();
+  const int32_t* e_next_in = s.e_next_in();
+  const int32_t* e_tail = s.e_tail();
+  const int32_t* e_weight = s.e_weight();
+  const int32_t* node_rank = s.node_rank();
+  int64_t* scores = s.scores();
+  int32_t* preds = s.preds();
+  int max_node = 0;
+  int max_score = -1;  // the initial max_node (node 0) scores -1 until it is scanned
+  bool tied = false;
+  bool unordered = false;
+  for (int r0 = 0; r0 < N; r0 += kWave) {
+    const int r = r0 + lane;
+    const int v = r < N ? r2n[r] : -1;
+    int t[kE], w[kE], tr[kE], ts[kE];
+    int e = v >= 0 ? in_head[v] : -1;
+#pragma unroll
+    for (int j = 0; j < kE; ++j) {
+      t[j] = -1;
+      w[j] = 0;
+      if (e != -1) {
+        t[j] = e_tail[e];
+        w[j] = e_weight[e];
+        e = e_next_in[e];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kE; ++j) tr[j] = t[j] != -1 ? node_rank[t[j]] : r0;
+#pragma unroll
+    for (int j = 0; j < kE; ++j) {
+      ts[j] = t[j] != -1 && tr[j] < r0 ? static_cast<int>(scores[t[j]]) : 0;
+    }
+    int sc = -1;  // lane k: score of rank r0 + k
+    int pr = -1;  // lane k: its predecessor
+    const int n = uniform(min(kWave, N - r0));
+    for (int k = 0; k < n; ++k) {
+      const int vk = __builtin_amdgcn_readlane(v, k);
+      int sv = -1, pv = -1, spv = 0;
+      auto consider = [&](int tj, int wj, int stj) {
+        if (sv < wj || (sv == wj && pv != -1 && spv <= stj)) {
+          sv = wj;
+          pv = tj;
+          spv = stj;
+        }
+      };
+#pragma unroll
+      for (int j = 0; j < kE; ++j) {
+        const int tj = __builtin_amdgcn_readlane(t[j], k);
+        if (tj == -1) break;
+        const int trj = __builtin_amdgcn_readlane(tr[j], k);
+        unordered |= trj >= r0 + k;
+        const int stj = trj >= r0 ? __builtin_amdgcn_readlane(sc, (trj - r0) & (kWave - 1))
+                                  : __builtin_amdgcn_readlane(ts[j], k);
+        consider(tj, __builtin_amdgcn_readlane(w[j], k), stj);
+      }
+      for (int f = uniform(__builtin_amdgcn_readlane(e, k)); f != -1; f = uniform(e_next_in[f])) {
+        const int tj = uniform(e_tail[f]);
+        const int trj = uniform(node_rank[tj]);
+        unordered |= trj >= r0 + k;
+        const int stj = trj >= r0 ? __builtin_amdgcn_readlane(sc, (trj - r0) & (kWave - 1))
+                                  : uniform(static_cast<int>(scores[tj]));
+        consider(tj, uniform(e_weight[f]), stj);
+      }
+      const int score = pv != -1 ? sv + spv : sv;
+      sc = lane_write(score, k, sc);
+      pr = lane_write(pv, k, pr);
+      if (vk == max_node) {
+        max_score = score;
+      } else if (max_score < score) {
+        max_node = vk;
+        max_score = score;
+        tied = false;
+      } else if (max_score == score) {
+        tied = true;
+      }
+    }
+    if (v >= 0) {
+      scores[v] = sc;
+      preds[v] = pr;
+    }
+  }
+  if (unordered) return kScanUnordered;
+  return static_cast<uint32_t>(max_node) | (tied ? 0x80000000u : 0u);
+}
+
 // ---- the window ----
 
 __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds, CWindow& W,
@@ -2375,6 +2482,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
   // again. (With no layers the backbone chain is already in DFS order.)
   const int N = s.n_nodes;
   int max_node = 0;
+  // 32-bit scores: every path weight is at most 252 per inserted position
+  const bool fast_scan = static_cast<int64_t>(W.node_bound) * 252 < 0x7fffffffll;
   for (int pass = 0;; ++pass) {
     const bool exact = pass > 0 || W.n_layers == 0;
     if (pass > 0) {
@@ -2385,15 +2494,27 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
         return;
       }
       wave_sync();
+      // ranks of the new order (the scan looks tails up by rank)
+      for (int r = lane; r < N; r += kWave) {
+        s.node_rank()[s.rank_to_node()[r]] = r;
+      }
+      wave_sync();
       clk.lap(R, kPhaseSort);
     }
+    max_node = 0;
+    bool top_tied = false;
+    uint32_t mt = kScanUnordered;
+    if (fast_scan) {
+      mt = static_cast<uint32_t>(uniform(static_cast<int>(bundle_scan(s.p, s.L, N))));
+      max_node = static_cast<int>(mt & 0x7fffffffu);
+      top_tied = (mt >> 31) != 0;
+    }
+    if (mt == kScanUnordered) {
     for (int v = lane; v < N; v += kWave) {
       s.scores()[v] = -1;
       s.preds()[v] = -1;
     }
     wave_sync();
-    max_node = 0;
-    bool top_tied = false;
     for (int r = 0; r < N; ++r) {
       const int v = uniform(s.rank_to_node()[r]);
       for (int e = uniform(s.in_head()[v]); e != -1; e = uniform(s.e_next_in()[e])) {
@@ -2417,6 +2538,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
         top_tied = true;
       }
     }
+    }
+    wave_sync();
     if (!exact && (top_tied || s.out_head()[max_node] != -1)) continue;
     if (s.out_head()[max_node] != -1) {
       for (int r = lane; r < N; r += kWave) {
