@@ -2289,10 +2289,15 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     // 4. traceback, from the end cell to (0, 0). The alignment of layer
     // position p (the graph node of a diagonal move into column p + 1, or -1
     // for a horizontal move) goes to pos_node[p]; vertical moves align no
-    // layer position. The wave loads the 8 x 8 block of codes at and before
-    // the current cell (and the nodes of its 8 rows) in one round trip and
-    // follows the path through it with lane reads until the path leaves it;
-    // the block's positions are collected in one register (lane t: position
+    // layer position. Per round trip the wave loads the 16-row x 8-column
+    // block of codes at and before the current cell (the path falls about two
+    // rows per column) with the node and the predecessor-delta table of each
+    // block row, and every lane turns its two cells into their successor cell
+    // in the block (or "leaves the block", or "needs the lists"). The walk
+    // through the block is then a chase of successor indices read from lane
+    // registers; the move that leaves the block (and any the table left to
+    // the predecessor lists) is taken by the full single-step decoder. The
+    // block's positions are collected in one register (lane t: position
     // j0 - 1 - t) and stored together.
     {
       const uint16_t* code = s.code();
@@ -2301,6 +2306,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       const int32_t* pred_rows = s.pred_rows();
       const uint64_t* row_pred4 = s.row_pred4();
       int32_t* pos_node = s.path();
+      constexpr uint32_t kNextExit = 0xff;  // successor: the decoder takes the move
+      constexpr uint32_t kDiagBit = 0x100;
       int i = best_row, j = L;
       int steps = 0;
       const int step_cap = uniform(V + L + 1);  // every move lowers i or j
@@ -2309,51 +2316,95 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
         i = uniform(i);
         j = uniform(j);
         const int i0 = i, j0 = j;
-        const int ri = i0 - (lane >> 3), cj = j0 - (lane & 7);
-        const int cw = ri >= 1 && cj >= 0
-                           ? code[static_cast<int64_t>(ri) * pitch + code_index(cj, pitch)]
-                           : 0;
-        const int nw = lane < 8 && i0 - lane >= 1 ? sub_order[i0 - lane - 1] : -1;
-        // lanes 8..15: the first four predecessor row deltas of the block's rows
-        uint64_t d4 = 0;
-        if (lane >= 8 && lane < 16 && i0 - (lane - 8) >= 1) d4 = row_pred4[i0 - (lane - 8) - 1];
-        const uint32_t d4lo = static_cast<uint32_t>(d4), d4hi = static_cast<uint32_t>(d4 >> 32);
+        // lane l: row i0 - (l >> 2), columns j0 - 2 (l & 3) - 0..1 (two codes
+        // per register); indices outside the matrix are clamped to a valid
+        // cell and the value dropped, so the loads issue unconditionally
+        const int di_l = lane >> 2;
+        const int ri = i0 - di_l;
+        const int rc = max(ri, 1);
+        const int cj = j0 - 2 * (lane & 3);
+        const uint16_t* crow = code + static_cast<int64_t>(rc) * pitch;
+        const uint32_t c0 = crow[code_index(max(cj, 0), pitch)];
+        const uint32_t c1 = crow[code_index(max(cj - 1, 0), pitch)];
+        const uint64_t d4 = row_pred4[rc - 1];  // first four predecessor row deltas of the row
+        const int nrow = sub_order[rc - 1];     // graph node of the row
+        const uint32_t cw = ri >= 1 ? ((cj >= 0 ? c0 : 0u) | ((cj >= 1 ? c1 : 0u) << 16)) : 0u;
         wait_stores();  // the block is in registers before the walk starts
-        int out = -1;   // lane t: the node of position j0 - 1 - t
-        while (i > 0 && i0 - i < 8 && j0 - j < 8) {
-          i = uniform(i);  // keeps the walk in scalar registers and branches
-          j = uniform(j);
-          steps = uniform(steps);
-          const int c = __builtin_amdgcn_readlane(cw, (i0 - i) * 8 + (j0 - j));
+        // successor of the lane's cell (di_l, dj) as di * 8 + dj, with the
+        // diagonal flag; kNextExit when the move leaves the block or row 1..,
+        // reaches column 0 other than vertically, or needs the lists
+        auto successor = [&](uint32_t c, int dj) -> uint32_t {
+          const int kind = static_cast<int>(c & 0xc000u);
+          const int pay = static_cast<int>(c & kCodePayload);
+          int delta = pay;
+          if ((c & kCodeViaK) != 0) {
+            delta = pay < 4 ? static_cast<int>((d4 >> (16 * pay)) & 0xffffu) : 0;
+          }
+          const int jj = j0 - dj;
+          if (ri < 1 || jj < 0 || (kind != kCodeVert && jj == 0) || kind == 0xc000) return kNextExit;
+          int ndi = di_l, ndj = dj;
+          if (kind == kCodeHorz) {
+            ndj = dj + 1;
+          } else {
+            if (delta == 0) return kNextExit;
+            ndi = di_l + delta;
+            if (kind == kCodeDiag) ndj = dj + 1;
+          }
+          if (ndi >= 16 || ndj >= 8 || i0 - ndi <= 0) return kNextExit;
+          return static_cast<uint32_t>(ndi * 8 + ndj) | (kind == kCodeDiag ? kDiagBit : 0u);
+        };
+        const int dja = 2 * (lane & 3);
+        const uint32_t ent = successor(cw & 0xffffu, dja) | (successor(cw >> 16, dja + 1) << 16);
+        int out = -1;  // lane t: the node of position j0 - 1 - t
+        // moves inside the block (each raises di * 8 + dj, so at most 127)
+        int idx = 0;
+        for (;;) {
+          const uint32_t e = (__builtin_amdgcn_readlane(ent, idx >> 1) >> (16 * (idx & 1))) & 0xffffu;
+          if ((e & 0xffu) == kNextExit || steps >= step_cap) break;
+          ++steps;
+          if ((e & kDiagBit) != 0) {
+            const int node = __builtin_amdgcn_readlane(nrow, (idx >> 3) * 4);
+            out = lane == (idx & 7) ? node : out;
+          }
+          idx = static_cast<int>(e & 0xffu);
+        }
+        i = i0 - (idx >> 3);
+        j = j0 - (idx & 7);
+        // the move out of the block: the full decoder
+        {
+          const int dj = idx & 7;
+          const uint32_t cp = __builtin_amdgcn_readlane(cw, idx >> 1);
+          const int c = static_cast<int>((cp >> (16 * (idx & 1))) & 0xffffu);
           const int kind = c & 0xc000;
           if (++steps > step_cap || (kind != kCodeVert && j == 0)) {
             bad = true;  // only vertical moves leave column 0
-            break;
-          }
-          if (kind == kCodeHorz) {
+          } else if (kind == kCodeHorz) {
             --j;
-            continue;
-          }
-          if (kind == kCodeDiag) {
-            const int node = __builtin_amdgcn_readlane(nw, i0 - i);
-            out = lane == j0 - j ? node : out;
-            --j;
-          }
-          const int pay = c & static_cast<int>(kCodePayload);
-          int delta = pay;
-          if ((c & kCodeViaK) != 0) {
-            // predecessor by in-edge index: the block's delta table, else the lists
-            delta = 0;
-            if (pay < 4) {
-              const uint32_t w = __builtin_amdgcn_readlane(pay < 2 ? d4lo : d4hi, 8 + (i0 - i));
-              delta = static_cast<int>((w >> (16 * (pay & 1))) & 0xffffu);
+          } else {
+            if (kind == kCodeDiag) {
+              const int node = __builtin_amdgcn_readlane(nrow, (idx >> 3) * 4);
+              out = lane == dj ? node : out;
+              --j;
             }
-            if (delta == 0) {
-              delta = i - (uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask);
+            const int pay = c & static_cast<int>(kCodePayload);
+            int delta = pay;
+            if ((c & kCodeViaK) != 0) {
+              // predecessor by in-edge index: the row's delta table, else the lists
+              delta = 0;
+              if (pay < 4) {
+                const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(d4), (idx >> 3) * 4);
+                const uint32_t hi =
+                    __builtin_amdgcn_readlane(static_cast<uint32_t>(d4 >> 32), (idx >> 3) * 4);
+                const uint32_t w = pay < 2 ? lo : hi;
+                delta = static_cast<int>((w >> (16 * (pay & 1))) & 0xffffu);
+              }
+              if (delta == 0) {
+                delta = i - (uniform(pred_rows[uniform(pred_off[i - 1]) + pay]) & kPredRowMask);
+              }
             }
+            i -= delta;
+            bad = i < 0;
           }
-          i -= delta;
-          bad = i < 0;
         }
         if (lane < j0 - j) {
           pos_node[j0 - 1 - lane] = out;
