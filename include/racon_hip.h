@@ -185,6 +185,31 @@ int32_t rh_align_download(rh_align_batch* batch, uint8_t* ops, uint64_t capacity
                           uint64_t* offsets, uint32_t* n_ops, int32_t* edit_distance);
 int32_t rh_align_get_stats(const rh_align_batch* batch, rh_align_stats* stats);
 
+/* Where pair i's alignment sits on the contig, for its breaking points:
+ * target span [t_begin, t_end) and q_first, the query position the walk
+ * starts after (q_length - q_end for a '-' overlap, else q_begin), the
+ * origins of src/overlap.cpp:228-240. */
+typedef struct {
+  uint32_t t_begin;
+  uint32_t t_end;
+  uint32_t q_first;
+} rh_break_span;
+
+/* Points room for n spans: two per window of window_length target bases
+ * that a span crosses. */
+uint64_t rh_breaking_points_bound(uint32_t window_length, const rh_break_span* spans, uint32_t n);
+/* Breaking points of every pair of the last rh_align_run, computed on the
+ * device from its edit operations (replaces racon's walk over the edlib
+ * CIGAR, Overlap::find_breaking_points_from_cigar, src/overlap.cpp:226-292):
+ * for each window the alignment crosses, the (target, query) position of
+ * its first match and the position after its last match. Pair i's points
+ * are points[2 * offsets[i] ...], counts[i] of them (2 per window with a
+ * match); capacity (in points) comes from rh_breaking_points_bound. spans
+ * has one entry per uploaded pair, in upload order. */
+int32_t rh_align_breaking_points(rh_align_batch* batch, uint32_t window_length,
+                                 const rh_break_span* spans, uint64_t* offsets, uint32_t* counts,
+                                 uint32_t* points, uint64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

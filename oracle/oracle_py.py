@@ -145,3 +145,46 @@ def poa_align(chains, sequence: bytes, match: int, mismatch: int, gap: int,
                                     mismatch, gap, tails, heads, ecap)
     return ([(nodes[i], pos[i]) for i in range(k)], bases.raw[:nn.value],
             [(tails[i], heads[i]) for i in range(ne)])
+
+
+def breaking_points(ops: bytes, t_begin: int, t_end: int, q_first: int,
+                    window_length: int) -> List[Tuple[int, int]]:
+    """racon's breaking-point walk (src/overlap.cpp:226-292) over edit
+    operations (0 match, 1 insertion, 2 deletion, 3 mismatch; the CIGAR of
+    src/overlap.cpp:205-216 run by run): for each window of window_length
+    target bases the alignment crosses, the (target, query) position of its
+    first match and the position after its last match."""
+    ends = [i - 1 for i in range(0, t_end, window_length) if i > t_begin]
+    ends.append(t_end - 1)
+    out: List[Tuple[int, int]] = []
+    w = 0
+    found = False
+    first = last = (0, 0)
+    q = q_first - 1
+    t = t_begin - 1
+
+    def end_reached():
+        nonlocal w, found
+        if w < len(ends) and t == ends[w]:
+            if found:
+                out.append(first)
+                out.append(last)
+            found = False
+            w += 1
+
+    for o in ops:
+        o &= 3
+        if o == 1:
+            q += 1
+        elif o == 2:
+            t += 1
+            end_reached()
+        else:
+            q += 1
+            t += 1
+            if not found:
+                found = True
+                first = (t, q)
+            last = (t + 1, q + 1)
+            end_reached()
+    return out

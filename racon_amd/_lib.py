@@ -69,6 +69,14 @@ class RhPair(ctypes.Structure):
     ]
 
 
+class RhBreakSpan(ctypes.Structure):
+    _fields_ = [
+        ("t_begin", ctypes.c_uint32),
+        ("t_end", ctypes.c_uint32),
+        ("q_first", ctypes.c_uint32),
+    ]
+
+
 class RhAlignStats(ctypes.Structure):
     _fields_ = [
         ("pack_ms", ctypes.c_double),
@@ -117,6 +125,14 @@ _SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_uint32),
                                            ctypes.POINTER(ctypes.c_int32)]),
     "rh_align_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhAlignStats)]),
+    "rh_breaking_points_bound": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.POINTER(RhBreakSpan),
+                                                   ctypes.c_uint32]),
+    "rh_align_breaking_points": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                                  ctypes.POINTER(RhBreakSpan),
+                                                  ctypes.POINTER(ctypes.c_uint64),
+                                                  ctypes.POINTER(ctypes.c_uint32),
+                                                  ctypes.POINTER(ctypes.c_uint32),
+                                                  ctypes.c_uint64]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -305,6 +321,25 @@ class AlignBatch:
         raw = ops.raw
         return ([raw[offsets[i]:offsets[i] + n_ops[i]] for i in range(self.n)],
                 [dist[i] for i in range(self.n)])
+
+    def breaking_points(self, window_length: int, spans: Sequence[tuple]):
+        """Device breaking points of the last run: spans[i] = (t_begin, t_end,
+        q_first) of pair i; returns per pair a list of (target, query) points."""
+        n = self.n
+        arr = (RhBreakSpan * max(1, n))()
+        for i, (tb, te, qf) in enumerate(spans):
+            arr[i].t_begin, arr[i].t_end, arr[i].q_first = tb, te, qf
+        cap = self.lib.rh_breaking_points_bound(window_length, arr, n)
+        points = (ctypes.c_uint32 * max(2, 2 * cap))()
+        offsets = (ctypes.c_uint64 * max(1, n))()
+        counts = (ctypes.c_uint32 * max(1, n))()
+        _check(self.lib.rh_align_breaking_points(self.handle, window_length, arr, offsets, counts,
+                                                 points, cap), "rh_align_breaking_points")
+        out = []
+        for i in range(n):
+            o = 2 * offsets[i]
+            out.append([(points[o + 2 * k], points[o + 2 * k + 1]) for k in range(counts[i])])
+        return out
 
     def stats(self) -> RhAlignStats:
         st = RhAlignStats()

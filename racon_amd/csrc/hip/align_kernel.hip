@@ -445,6 +445,89 @@ hipError_t launch_align_batch(const AlignParams& params, uint32_t grid, hipStrea
   return hipGetLastError();
 }
 
+// Breaking points of every aligned pair (racon overlap.cpp:226-292, the walk
+// over edlib's CIGAR), from the edit operations the batch kernel left in
+// device memory: one lane per pair. A match or mismatch advances target and
+// query, an insertion the query, a deletion the target; the first match of a
+// window and the position after its last match are emitted as a pair when
+// the target position reaches the window's last base (k * W - 1 for the
+// multiples of W inside (t_begin, t_end), then t_end - 1). counts[i] is the
+// number of points (2 per window with a match), or 0xffffffff when they
+// would outgrow the pair's room.
+__global__ __launch_bounds__(256) void breaking_points_kernel(
+    const AlignPair* __restrict__ pairs, const AlignResult* __restrict__ results,
+    const uint8_t* __restrict__ ops, const BreakSpan* __restrict__ spans, uint32_t n,
+    uint32_t window_length, uint32_t* __restrict__ points, uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BreakSpan sp = spans[i];
+  const uint8_t* op = ops + pairs[i].out_off;
+  const uint32_t n_ops = results[i].n_ops;
+  const uint32_t W = window_length;
+  // next window end: the first multiple of W above t_begin, less one, while
+  // below t_end; then t_end - 1; then none
+  uint64_t next_mult = (static_cast<uint64_t>(sp.t_begin) / W + 1) * W;
+  int64_t end = next_mult < sp.t_end ? static_cast<int64_t>(next_mult) - 1
+                                     : static_cast<int64_t>(sp.t_end) - 1;
+  bool last_end = next_mult >= sp.t_end;
+  int64_t t = static_cast<int64_t>(sp.t_begin) - 1;
+  int64_t q = static_cast<int64_t>(sp.q_first) - 1;
+  bool found = false;
+  uint32_t ft = 0, fq = 0, lt = 0, lq = 0;
+  uint32_t cnt = 0;
+  bool overflow = false;
+  uint32_t* out = points + 2 * sp.out;
+  for (uint32_t k = 0; k < n_ops; ++k) {
+    const uint32_t o = op[k] & 3u;
+    if (o == 1u) {
+      ++q;
+      continue;
+    }
+    ++t;
+    if (o != 2u) {
+      ++q;
+      if (!found) {
+        found = true;
+        ft = static_cast<uint32_t>(t);
+        fq = static_cast<uint32_t>(q);
+      }
+      lt = static_cast<uint32_t>(t + 1);
+      lq = static_cast<uint32_t>(q + 1);
+    }
+    if (t == end) {
+      if (found) {
+        if (cnt + 2 > sp.cap) {
+          overflow = true;
+        } else {
+          out[2 * cnt] = ft;
+          out[2 * cnt + 1] = fq;
+          out[2 * cnt + 2] = lt;
+          out[2 * cnt + 3] = lq;
+        }
+        cnt += 2;
+      }
+      found = false;
+      if (last_end) {
+        end = -2;  // past the last window
+      } else {
+        next_mult += W;
+        last_end = next_mult >= sp.t_end;
+        end = last_end ? static_cast<int64_t>(sp.t_end) - 1 : static_cast<int64_t>(next_mult) - 1;
+      }
+    }
+  }
+  counts[i] = overflow ? 0xffffffffu : cnt;
+}
+
+hipError_t launch_breaking_points(const AlignPair* pairs, const AlignResult* results,
+                                  const uint8_t* ops, const BreakSpan* spans, uint32_t n,
+                                  uint32_t window_length, uint32_t* points, uint32_t* counts,
+                                  hipStream_t stream) {
+  hipLaunchKernelGGL(breaking_points_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pairs,
+                     results, ops, spans, n, window_length, points, counts);
+  return hipGetLastError();
+}
+
 hipError_t align_occupancy(int* blocks_per_cu) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, align_batch_kernel, 64, 0);
 }

@@ -13,6 +13,18 @@ struct AlignPair {
   uint32_t t_len;
 };
 
+// Breaking-point input of one aligned pair: the target span [t_begin, t_end)
+// on the contig, the query position before the first aligned base minus one
+// plus one (q_first: the walk starts at q_first - 1), and where its points go
+// (out, in points of two uint32 values) with room for cap points.
+struct BreakSpan {
+  uint32_t t_begin;
+  uint32_t t_end;
+  uint32_t q_first;
+  uint32_t cap;
+  uint64_t out;
+};
+
 struct AlignResult {
   uint32_t n_ops;
   int32_t edit_distance;

@@ -16,6 +16,20 @@
 namespace rh {
 hipError_t launch_align_batch(const AlignParams& params, uint32_t grid, hipStream_t stream);
 hipError_t align_occupancy(int* blocks_per_cu);
+hipError_t launch_breaking_points(const AlignPair* pairs, const AlignResult* results,
+                                  const uint8_t* ops, const BreakSpan* spans, uint32_t n,
+                                  uint32_t window_length, uint32_t* points, uint32_t* counts,
+                                  hipStream_t stream);
+
+// Windows of window_length target bases that the span [t_begin, t_end)
+// crosses: one per multiple of W inside (t_begin, t_end), plus the last.
+uint64_t span_windows(uint32_t window_length, const rh_break_span& s) {
+  if (s.t_end <= s.t_begin) return 1;
+  const uint64_t W = window_length;
+  const uint64_t first = s.t_begin / W + 1;   // multiples k * W > t_begin
+  const uint64_t last = (s.t_end - 1ull) / W;  // multiples k * W < t_end
+  return (last >= first ? last - first + 1 : 0) + 1;
+}
 }  // namespace rh
 
 struct rh_align_batch {
@@ -27,6 +41,7 @@ struct rh_align_batch {
   uint64_t ops_bytes = 0;
   uint32_t max_q = 0, max_t = 0;
   rh::DeviceBuffer d_arena, d_pairs, d_ops, d_results, d_order, d_queue, d_scratch;
+  rh::DeviceBuffer d_spans, d_points, d_counts;  // breaking points
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool uploaded = false, ran = false;
@@ -71,6 +86,9 @@ void rh_align_destroy(rh_align_batch* b) {
   b->d_pairs.release();
   b->d_ops.release();
   b->d_results.release();
+  b->d_spans.release();
+  b->d_points.release();
+  b->d_counts.release();
   b->d_order.release();
   b->d_queue.release();
   b->d_scratch.release();
@@ -273,6 +291,77 @@ int32_t rh_align_download(rh_align_batch* b, uint8_t* ops, uint64_t capacity,
     pos += len;
   }
   b->stats.d2h_ms = rh::ms_since(t0);
+  return RH_OK;
+}
+
+uint64_t rh_breaking_points_bound(uint32_t window_length, const rh_break_span* spans, uint32_t n) {
+  if (spans == nullptr || window_length == 0) return 0;
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; ++i) total += 2 * rh::span_windows(window_length, spans[i]);
+  return total;
+}
+
+int32_t rh_align_breaking_points(rh_align_batch* b, uint32_t window_length,
+                                 const rh_break_span* spans, uint64_t* offsets, uint32_t* counts,
+                                 uint32_t* points, uint64_t capacity) {
+  rh::clear_error();
+  if (b == nullptr || !b->ran) {
+    rh::set_error("rh_align_breaking_points: batch has not run");
+    return RH_ERR_STATE;
+  }
+  if (window_length == 0 || spans == nullptr || offsets == nullptr || counts == nullptr ||
+      points == nullptr) {
+    rh::set_error("rh_align_breaking_points: null argument or zero window length");
+    return RH_ERR_ARGUMENT;
+  }
+  const uint32_t n = static_cast<uint32_t>(b->pairs.size());
+  if (n == 0) return RH_OK;
+  std::vector<rh::BreakSpan> bs(n);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t cap = 2 * rh::span_windows(window_length, spans[i]);
+    bs[i] = rh::BreakSpan{spans[i].t_begin, spans[i].t_end, spans[i].q_first,
+                          static_cast<uint32_t>(cap), total};
+    offsets[i] = total;
+    total += cap;
+  }
+  if (total > capacity) {
+    rh::set_error("rh_align_breaking_points: output capacity too small");
+    return RH_ERR_ARGUMENT;
+  }
+  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  rh::DeviceBuffer& d_spans = b->d_spans;
+  rh::DeviceBuffer& d_points = b->d_points;
+  rh::DeviceBuffer& d_counts = b->d_counts;
+  RH_HIP_CHECK(d_spans.reserve(n * sizeof(rh::BreakSpan)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(d_points.reserve(std::max<uint64_t>(total, 1) * 2 * sizeof(uint32_t)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(d_counts.reserve(n * sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(hipMemcpyAsync(d_spans.ptr, bs.data(), n * sizeof(rh::BreakSpan),
+                              hipMemcpyHostToDevice, b->stream),
+               RH_ERR_DEVICE);
+  RH_HIP_CHECK(rh::launch_breaking_points(b->d_pairs.as<rh::AlignPair>(),
+                                          b->d_results.as<rh::AlignResult>(),
+                                          b->d_ops.as<uint8_t>(), d_spans.as<rh::BreakSpan>(), n,
+                                          window_length, d_points.as<uint32_t>(),
+                                          d_counts.as<uint32_t>(), b->stream),
+               RH_ERR_KERNEL);
+  RH_HIP_CHECK(hipMemcpyAsync(counts, d_counts.ptr, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              b->stream),
+               RH_ERR_DEVICE);
+  if (total > 0) {
+    RH_HIP_CHECK(hipMemcpyAsync(points, d_points.ptr, total * 2 * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, b->stream),
+                 RH_ERR_DEVICE);
+  }
+  RH_HIP_CHECK(hipStreamSynchronize(b->stream), RH_ERR_DEVICE);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (counts[i] == 0xffffffffu) {
+      rh::set_error("rh_align_breaking_points: pair " + std::to_string(i) +
+                    " has more points than its span allows");
+      return RH_ERR_KERNEL;
+    }
+  }
   return RH_OK;
 }
 

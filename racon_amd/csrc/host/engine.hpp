@@ -24,6 +24,12 @@ struct AlignTask {
   uint32_t query_length;
   const char* target;
   uint32_t target_length;
+  // where the alignment sits, for its breaking points (overlap.cpp:228-240):
+  // target span [t_begin, t_end) on the contig and the query position the
+  // walk starts after
+  uint32_t t_begin = 0;
+  uint32_t t_end = 0;
+  uint32_t q_first = 0;
 };
 
 // Scores and switches shared by every engine (racon CLI defaults:
@@ -61,6 +67,20 @@ class PolishEngine {
   virtual void align_overlaps(
       const std::vector<AlignTask>& tasks,
       const std::function<void(uint64_t, const uint8_t*, uint64_t)>& done) = 0;
+
+  // Aligns every task and calls done(i, points, n_points) once per task
+  // with its breaking points as (target, query) pairs, two per window of
+  // window_length target bases with a match (overlap.cpp:226-292), computed
+  // by the engine itself. Returns false, calling nothing, when the engine
+  // leaves that walk to the caller (over align_overlaps' operations).
+  virtual bool align_breaking_points(
+      const std::vector<AlignTask>& tasks, uint32_t window_length,
+      const std::function<void(uint64_t, const uint32_t*, uint64_t)>& done) {
+    (void)tasks;
+    (void)window_length;
+    (void)done;
+    return false;
+  }
 
   // Generates the consensus of every window (set through
   // Window::set_consensus) and fills status[i] with kWindow* codes.

@@ -74,6 +74,52 @@ class HipEngine : public PolishEngine {
     rh_align_destroy(batch);
   }
 
+  // The breaking-point walk runs on the device over the operations the
+  // aligner left there; only the points come back.
+  bool align_breaking_points(
+      const std::vector<AlignTask>& tasks, uint32_t window_length,
+      const std::function<void(uint64_t, const uint32_t*, uint64_t)>& done) override {
+    rh_align_batch* batch = nullptr;
+    int32_t rc = rh_align_create(0, &batch);
+    if (rc != RH_OK) die("rh_align_create", rc);
+    const uint64_t kChunkBytes = 1ull << 30;
+    uint64_t first = 0;
+    std::vector<rh_pair> pairs;
+    std::vector<rh_break_span> spans;
+    std::vector<uint64_t> offsets;
+    std::vector<uint32_t> counts;
+    std::vector<uint32_t> points;
+    while (first < tasks.size()) {
+      uint64_t last = first, bytes = 0;
+      pairs.clear();
+      spans.clear();
+      while (last < tasks.size() && (last == first || bytes < kChunkBytes)) {
+        const AlignTask& t = tasks[last];
+        pairs.push_back(rh_pair{t.query, t.query_length, t.target, t.target_length});
+        spans.push_back(rh_break_span{t.t_begin, t.t_end, t.q_first});
+        bytes += t.query_length + t.target_length;
+        ++last;
+      }
+      const uint32_t n = static_cast<uint32_t>(pairs.size());
+      rc = rh_align_upload(batch, n, pairs.data());
+      if (rc != RH_OK) die("rh_align_upload", rc);
+      rc = rh_align_run(batch);
+      if (rc != RH_OK) die("rh_align_run", rc);
+      points.resize(2 * rh_breaking_points_bound(window_length, spans.data(), n));
+      offsets.resize(n);
+      counts.resize(n);
+      rc = rh_align_breaking_points(batch, window_length, spans.data(), offsets.data(),
+                                    counts.data(), points.data(), points.size() / 2);
+      if (rc != RH_OK) die("rh_align_breaking_points", rc);
+      for (uint32_t i = 0; i < n; ++i) {
+        done(first + i, points.data() + 2 * offsets[i], counts[i]);
+      }
+      first = last;
+    }
+    rh_align_destroy(batch);
+    return true;
+  }
+
   void generate_consensus(std::vector<std::shared_ptr<Window>>& windows, bool trim,
                           std::vector<uint8_t>& status) override {
     status.assign(windows.size(), kWindowUnpolished);

@@ -249,6 +249,23 @@ void Overlap::find_breaking_points_from_runs(const std::vector<CigarRun>& runs,
   }
 }
 
+void Overlap::breaking_point_span(uint32_t* t_begin, uint32_t* t_end, uint32_t* q_first) const {
+  // the origins of the walk in find_breaking_points_from_runs
+  *t_begin = t_begin_;
+  *t_end = t_end_;
+  *q_first = strand_ ? (q_length_ - q_end_) : q_begin_;
+}
+
+void Overlap::set_breaking_points(const uint32_t* points, uint64_t n_points) {
+  check_transmuted();
+  breaking_points_.clear();
+  breaking_points_.reserve(n_points);
+  for (uint64_t k = 0; k < n_points; ++k) {
+    breaking_points_.emplace_back(points[2 * k], points[2 * k + 1]);
+  }
+  std::string().swap(cigar_);
+}
+
 void Overlap::find_breaking_points_from_cigar(uint32_t window_length) {
   check_transmuted();
   if (!breaking_points_.empty()) {

@@ -83,6 +83,13 @@ class Overlap {
   void find_breaking_points_from_ops(const uint8_t* ops, uint64_t n_ops,
                                      uint32_t window_length);
 
+  // Target span and query start of the breaking-point walk
+  // (overlap.cpp:228-240), for an engine that computes the points itself.
+  void breaking_point_span(uint32_t* t_begin, uint32_t* t_end, uint32_t* q_first) const;
+
+  // Breaking points computed elsewhere: n_points (target, query) pairs.
+  void set_breaking_points(const uint32_t* points, uint64_t n_points);
+
   // Breaking points from the SAM CIGAR held by the overlap (overlap.cpp:200).
   void find_breaking_points_from_cigar(uint32_t window_length);
 

@@ -428,6 +428,7 @@ void Polisher::find_overlap_breaking_points(
       AlignTask t;
       overlaps[i]->alignment_input(sequences_, &t.query, &t.query_length,
                                    &t.target, &t.target_length);
+      overlaps[i]->breaking_point_span(&t.t_begin, &t.t_end, &t.q_first);
       tasks.push_back(t);
       task_overlap.push_back(i);
     }
@@ -470,11 +471,17 @@ void Polisher::find_overlap_breaking_points(
   }
 
   if (!tasks.empty()) {
-    engine_->align_overlaps(
-        tasks, [&](uint64_t k, const uint8_t* ops, uint64_t n_ops) {
-          overlaps[task_overlap[k]]->find_breaking_points_from_ops(ops, n_ops,
-                                                                   window_length_);
+    const bool done = engine_->align_breaking_points(
+        tasks, window_length_, [&](uint64_t k, const uint32_t* points, uint64_t n_points) {
+          overlaps[task_overlap[k]]->set_breaking_points(points, n_points);
         });
+    if (!done) {
+      engine_->align_overlaps(
+          tasks, [&](uint64_t k, const uint8_t* ops, uint64_t n_ops) {
+            overlaps[task_overlap[k]]->find_breaking_points_from_ops(ops, n_ops,
+                                                                     window_length_);
+          });
+    }
   }
   logger_->log("[racon::Polisher::initialize] aligned overlaps");
 }
