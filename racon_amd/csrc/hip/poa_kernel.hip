@@ -921,19 +921,22 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
 }
 
 // DP rows of one layer with 16-bit scores, every row in one pass of the
-// wave: lane k owns columns 1 + kW .. (k + 1)W (W = 4, 8 or 12, so that
-// 64W >= L), the layer bytes of its strip stay in registers, and column 0
-// heads lane 0's horizontal chain. Rows with one predecessor (most of them)
-// take no argmax; rows with several keep (score, 255 - k) in one integer so
-// that one max keeps the first predecessor with the best score (in-edge
-// order, as spoa's traceback prefers). Same moves, codes and end cell as
+// wave: lane k owns columns 1 + kW .. (k + 1)W (W = 4, 8, 12, or 16 when
+// packed, so that 64W >= L), the layer bytes of its strip stay in
+// registers, and column 0 heads lane 0's horizontal chain. Same moves,
+// codes and end cell as
 // dp_layer. Requires in-degree <= 255, 64W + pad <= pitch and 16-bit scores
-// over the whole strip. With kPacked, single-predecessor rows run in packed
-// 16-bit pairs (packed_row), which needs max_abs * (V + 2L + 2) in 16 bits.
+// over the whole strip. With kPacked every row runs in packed 16-bit pairs
+// (packed_row; a row with several predecessors first folds them with a
+// packed max and first argmax), which needs max_abs * (V + 2L + 2) in 16
+// bits. Without it (W <= 12), rows with one predecessor take no argmax and
+// rows with several keep (score, 255 - k) in one integer so that one max
+// keeps the first predecessor with the best score (in-edge order, as spoa's
+// traceback prefers).
 template <int W, bool kPacked>
 __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t* lds, int V, int L,
                               int* best_score, bool* tied) {
-  static_assert(W % 4 == 0 && W >= 4 && W <= 12, "strip width");
+  static_assert(W % 4 == 0 && W >= 4 && W <= (kPacked ? 16 : 12), "strip width");
   const int lane = lane_id();
   const int pitch = static_cast<int>(P.caps.cols);
   // scores as uniform registers (a select between two parameter fields
@@ -1100,6 +1103,7 @@ enum DpKind : int {
   kDpPacked4,
   kDpPacked8,
   kDpPacked12,
+  kDpPacked16,
   kDpGeneric16,
   kDpGeneric32
 };
@@ -1135,6 +1139,8 @@ __device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLa
     row = dp_layer_strip<8, true>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpPacked12) {
     row = dp_layer_strip<12, true>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpPacked16) {
+    row = dp_layer_strip<16, true>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpGeneric16) {
     row = dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied);
   } else {
@@ -2239,6 +2245,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       dp = dp_layer_call<kDpPacked8>(pp, s.p, ly, lds_addr, V, L);
     } else if (packed && L <= 12 * kWave) {
       dp = dp_layer_call<kDpPacked12>(pp, s.p, ly, lds_addr, V, L);
+    } else if (packed && L <= 16 * kWave) {
+      dp = dp_layer_call<kDpPacked16>(pp, s.p, ly, lds_addr, V, L);
     } else if (strip_ok && L <= 4 * kWave) {
       dp = dp_layer_call<kDpStrip4>(pp, s.p, ly, lds_addr, V, L);
     } else if (strip_ok && L <= 8 * kWave) {
