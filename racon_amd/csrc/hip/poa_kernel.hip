@@ -1570,15 +1570,21 @@ __device__ __forceinline__ int32_t wave_prefix_sum(int32_t x) {
 // Each lane takes kU rows per round (64 kU rows) and walks their edge lists
 // in lockstep, one edge of every list per step, so that the loads of the
 // kU lists are in flight together; the walk keeps the first kP predecessor
-// rows of each row, so only rows with more walk their in-edges again. Out
-// of line, like the DP, for a register allocation of its own.
+// rows of each row, so only rows with more walk their in-edges again. The
+// slot of every row is also kept in LDS (one byte per row, in the DP ring,
+// which is idle here) for the predecessor entries, which then need no
+// reload of the row table just written. Out of line, like the DP, for a
+// register allocation of its own.
 template <bool kFull>
-__device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V, int n_slots) {
+__device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V, int n_slots,
+                                       uint32_t lds_addr, int lds_bytes) {
   constexpr int kU = 4;
   constexpr int kP = 4;
   const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
   V = uniform(V);
   n_slots = uniform(n_slots);
+  const bool slots_in_lds = V <= uniform(lds_bytes);
+  LByte* lslot = (LByte*)(size_t)__builtin_amdgcn_readfirstlane(lds_addr);
   const int lane = lane_id();
   const int32_t* sub_order = s.sub_order();
   const int32_t* in_head = s.in_head();
@@ -1597,7 +1603,9 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   auto included = [&](int v) { return kFull || in_sub[v] != 0; };
   // predecessor entry of DP row pr: the row and the slot it lives in
   auto pred_entry = [&](int pr) {
-    return pr | (((row_meta[pr - 1] >> kRowMetaSlotShift) & 31) << kPredRowBits);
+    const int sl = slots_in_lds ? static_cast<int>(lslot[pr - 1])
+                                : (row_meta[pr - 1] >> kRowMetaSlotShift) & 31;
+    return pr | (sl << kPredRowBits);
   };
   int run = 0;
   int max_np = 0;
@@ -1694,6 +1702,7 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
       if (r < V) {
         row_meta[r] = meta[i] | (my_slot << kRowMetaSlotShift);
         pred_off[r] = off[i];
+        if (slots_in_lds) lslot[r] = static_cast<uint8_t>(my_slot);
       }
       run += __builtin_amdgcn_readlane(x, kWave - 1);
     }
@@ -2182,10 +2191,13 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     // topological order is used; spoa's exact DFS order is computed only
     // where it decides a tie)
     int V = 0;
+    // (with the DP row of every included node, row_of)
     if (full) {
       V = nn;
       for (int r = lane; r < V; r += kWave) {
-        s.sub_order()[r] = s.rank_to_node()[r];
+        const int v = s.rank_to_node()[r];
+        s.sub_order()[r] = v;
+        s.row_of()[v] = r + 1;
       }
     } else {
       for (int r0 = 0; r0 < nn; r0 += kWave) {
@@ -2194,7 +2206,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
         const bool keep = r < nn && s.in_sub()[v] != 0;
         const unsigned long long mask = __ballot(keep);
         if (keep) {
-          s.sub_order()[V + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+          const int at = V + __popcll(mask & ((1ull << lane) - 1ull));
+          s.sub_order()[at] = v;
+          s.row_of()[v] = at + 1;
         }
         V += __popcll(mask);
       }
@@ -2215,13 +2229,13 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
 
     bool strip_ok = false;  // the one-pass DP applies (dp_layer_strip)
 
-    // DP row of every included node; predecessor rows, sink flags, slots
-    for (int r = lane; r < V; r += kWave) {
-      s.row_of()[s.sub_order()[r]] = r + 1;
-    }
-    wave_sync();
-    const int max_np = uniform(full ? build_rows<true>(s.p, s.L, V, n_slots)
-                                    : build_rows<false>(s.p, s.L, V, n_slots));
+    // predecessor rows, sink flags, slots
+    uint32_t rows_lds = static_cast<uint32_t>((size_t)(LByte*)lds);
+    asm volatile("" : "+v"(rows_lds));
+    const int ring_bytes = static_cast<int>(P.ring_rows) * pitch * 2;
+    const int max_np =
+        uniform(full ? build_rows<true>(s.p, s.L, V, n_slots, rows_lds, ring_bytes)
+                     : build_rows<false>(s.p, s.L, V, n_slots, rows_lds, ring_bytes));
     strip_ok = narrow && L >= 1 && max_abs <= 4096 && max_np <= 255;
     wave_sync();
 
