@@ -2495,21 +2495,38 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
           wave_fence();
         }
         const int first_moved = uniform(anchor[0]) + 1;
-        for (int r0 = ((nodes_before - 1) / kWave) * kWave; r0 + kWave > first_moved; r0 -= kWave) {
-          const int r = r0 + lane;
-          const bool moves = r >= first_moved && r < nodes_before;
-          int lo = 1, hi = M;  // anchor[0] < r for every moved rank
-          while (__ballot(moves && lo < hi) != 0) {
-            if (moves && lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              const int a = anchors_in_lds ? static_cast<int>(la[mid]) : anchor[mid];
-              if (a < r) lo = mid + 1; else hi = mid;
-            }
+        // four chunks of 64 ranks per round, loaded before any of their
+        // stores (every store lands at or above its own rank, and later
+        // rounds read only below this one)
+        constexpr int kMU = 4;
+        for (int r0 = ((nodes_before - 1) / kWave) * kWave; r0 + kWave > first_moved;
+             r0 -= kMU * kWave) {
+          int r[kMU], lo[kMU], v[kMU];
+          bool moves[kMU];
+#pragma unroll
+          for (int u = 0; u < kMU; ++u) {
+            r[u] = r0 - u * kWave + lane;
+            moves[u] = r[u] >= first_moved && r[u] < nodes_before;
+            v[u] = moves[u] ? s.rank_to_node()[r[u]] : 0;
           }
-          if (moves) {
-            const int v = s.rank_to_node()[r];
-            s.rank_to_node()[r + lo] = v;
-            s.node_rank()[v] = r + lo;
+#pragma unroll
+          for (int u = 0; u < kMU; ++u) {
+            int l = 1, hi = M;  // anchor[0] < r for every moved rank
+            while (__ballot(moves[u] && l < hi) != 0) {
+              if (moves[u] && l < hi) {
+                const int mid = (l + hi) >> 1;
+                const int a = anchors_in_lds ? static_cast<int>(la[mid]) : anchor[mid];
+                if (a < r[u]) l = mid + 1; else hi = mid;
+              }
+            }
+            lo[u] = l;
+          }
+#pragma unroll
+          for (int u = 0; u < kMU; ++u) {
+            if (moves[u]) {
+              s.rank_to_node()[r[u] + lo[u]] = v[u];
+              s.node_rank()[v[u]] = r[u] + lo[u];
+            }
           }
           wave_fence();
         }
