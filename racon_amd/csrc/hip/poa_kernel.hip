@@ -436,6 +436,11 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
   const int base = __builtin_amdgcn_readlane(off_v, 0);
   const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
   const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+  // (up to four entries per row of the chunk without a reload)
+  const int pr2 =
+      base + 2 * kWave + lane < total_preds ? s.pred_rows()[base + 2 * kWave + lane] : 0;
+  const int pr3 =
+      base + 3 * kWave + lane < total_preds ? s.pred_rows()[base + 3 * kWave + lane] : 0;
   // complete the chunk's loads here: otherwise the row loop below would wait
   // for them (and for every code store before them) on each row
   wait_stores();
@@ -449,6 +454,8 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
       const int idx = off + k - base;
       if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
       if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
+      if (idx < 3 * kWave) return __builtin_amdgcn_readlane(pr2, idx - 2 * kWave);
+      if (idx < 4 * kWave) return __builtin_amdgcn_readlane(pr3, idx - 3 * kWave);
       const int e = uniform(s.pred_rows()[off + k]);
       wait_stores();
       return e;
@@ -971,6 +978,11 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
     const int base = __builtin_amdgcn_readlane(off_v, 0);
     const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
     const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+    // (up to four entries per row of the chunk without a reload)
+    const int pr2 =
+        base + 2 * kWave + lane < total_preds ? s.pred_rows()[base + 2 * kWave + lane] : 0;
+    const int pr3 =
+        base + 3 * kWave + lane < total_preds ? s.pred_rows()[base + 3 * kWave + lane] : 0;
     // complete the chunk's loads here: otherwise the row loop would wait for
     // them (and for every code store issued before them) on each row
     wait_stores();
@@ -984,6 +996,8 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         const int idx = off + k - base;
         if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
         if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
+        if (idx < 3 * kWave) return __builtin_amdgcn_readlane(pr2, idx - 2 * kWave);
+        if (idx < 4 * kWave) return __builtin_amdgcn_readlane(pr3, idx - 3 * kWave);
         const int e = uniform(s.pred_rows()[off + k]);
         wait_stores();
         return e;
