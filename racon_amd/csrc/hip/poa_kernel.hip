@@ -1990,7 +1990,7 @@ __device__ __forceinline__ int sink_tie_by_sets(const Slot& s, int V, bool full,
 constexpr uint32_t kScanUnordered = 0xffffffffu;
 
 __device__ __noinline__ uint32_t bundle_scan(GByte* slot_base, CLayout* layout, int N) {
-  constexpr int kE = 4;
+  constexpr int kE = 8;
   const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
   N = uniform(N);
   const int lane = lane_id();
