@@ -1380,11 +1380,15 @@ __device__ __noinline__ InsertResult insert_parallel(GByte* slot_base, CLayout* 
 #pragma unroll
       for (int k = 0; k < kA; ++k) al[i][k] = v[i] != -1 ? aligned[v[i] * kA + k] : 0;
     }
+    int nal[kUA][kA];  // aligned-set sizes of the members (read before any write)
 #pragma unroll
     for (int i = 0; i < kUA; ++i) {
       int rk[kA];
 #pragma unroll
-      for (int k = 0; k < kA; ++k) rk[k] = k < cnt[i] ? node_rank[al[i][k]] : -1;
+      for (int k = 0; k < kA; ++k) {
+        rk[k] = k < cnt[i] ? node_rank[al[i][k]] : -1;
+        nal[i][k] = k < cnt[i] ? static_cast<int>(n_aligned[al[i][k]]) : 0;
+      }
 #pragma unroll
       for (int k = 0; k < kA; ++k) bound[i] = max(bound[i], rk[k]);  // highest rank in v's set
     }
@@ -1414,12 +1418,15 @@ __device__ __noinline__ InsertResult insert_parallel(GByte* slot_base, CLayout* 
         if (v[i] != -1) {
           // the mismatch node joins v's aligned set
           const int c = cnt[i];
-          for (int k = 0; k < c; ++k) {
-            const int a = al[i][k];
-            const int ca = n_aligned[a];
-            aligned[id * kA + k] = a;
-            aligned[a * kA + ca] = id;
-            n_aligned[a] = static_cast<uint8_t>(ca + 1);
+#pragma unroll
+          for (int k = 0; k < kA; ++k) {
+            if (k < c) {
+              const int a = al[i][k];
+              const int ca = nal[i][k];
+              aligned[id * kA + k] = a;
+              aligned[a * kA + ca] = id;
+              n_aligned[a] = static_cast<uint8_t>(ca + 1);
+            }
           }
           aligned[id * kA + c] = v[i];
           n_aligned[id] = static_cast<uint8_t>(c + 1);
