@@ -1,6 +1,7 @@
 """Window consensus on the GPU vs the CPU oracle, byte for byte."""
 import sys
 
+import numpy as np
 import pytest
 
 from racon_amd import _lib
@@ -58,6 +59,24 @@ def test_no_trim_and_ngs():
     check(wins, (3, -5, -4))
     wins = make_windows(17, 4, length=250, depth=10)
     check(wins, (3, -5, -4), trim=False)
+
+
+def test_other_bytes():
+    # N and lower-case letters in the backbone and the layers: bytes are
+    # compared as they are (a different byte is a mismatch, as in spoa)
+    rng = np.random.default_rng(25)
+
+    def spice(seq):
+        out = bytearray(seq)
+        for i in rng.choice(len(out), size=max(1, len(out) // 20), replace=False):
+            out[i] = b"Nacgt"[rng.integers(0, 5)]
+        return bytes(out)
+
+    wins = []
+    for w in make_windows(24, 4, length=300, depth=12):
+        layers = [_lib.Layer(spice(l.sequence), l.quality, l.begin, l.end) for l in w.layers]
+        wins.append(_lib.Window(spice(w.backbone), w.backbone_quality, layers, w.tgs))
+    check(wins, (3, -5, -4))
 
 
 def test_racon_sized_windows():
