@@ -121,6 +121,80 @@ def prepare(cfg_id: int, workdir: str, threads: int):
     return wdump, odump, (reads, paf, contigs)
 
 
+def serving_bench(windows, args, scores) -> dict:
+    """Poisson arrivals of polishing requests into the serving loop.
+
+    A request stands for a contig of log-uniform 20 kbp - 2 Mbp polished at
+    a depth uniform in 20-100: that many consecutive windows of the loaded
+    config (window length 500), each keeping a random subset of at most
+    `depth` layers. Latency is submit to last consensus; windows/s is over
+    the span from the first arrival to the last completion."""
+    import math
+    import random
+
+    from racon_amd import Window
+    from racon_amd.serving import PolishServer, percentile
+
+    rng = random.Random(1005)
+
+    def make_request():
+        contig = math.exp(rng.uniform(math.log(20e3), math.log(2e6)))
+        n_w = max(1, math.ceil(contig / 500))
+        start = rng.randrange(len(windows))
+        depth = rng.randint(20, 100)
+        out = []
+        for i in range(n_w):
+            w = windows[(start + i) % len(windows)]
+            keep = sorted(rng.sample(range(len(w.layers)), min(depth, len(w.layers))))
+            out.append(Window(w.backbone, w.backbone_quality, [w.layers[k] for k in keep], w.tgs))
+        return out
+
+    budget = int(args.serving_budget * (1 << 30))
+    rates = [float(r) for r in args.serving.split(",") if r]
+    report = {"metric": "serving: p50/p99 request latency and windows/s under Poisson arrivals",
+              "unit": "s", "n_gpus": 1, "data": "synthetic",
+              "config": {"workload": f"requests cut from cfg{args.config} windows "
+                                     "(contig 20 kbp-2 Mbp log-uniform, depth 20-100)",
+                         "budget_bytes": budget, "executors": args.serving_executors,
+                         "launch_windows": args.serving_launch},
+              "rates": []}
+    with PolishServer(budget, executors=args.serving_executors,
+                      launch_windows=args.serving_launch, scores=scores) as srv:
+        # every executor sizes its scratch on a full launch of the largest windows
+        size = [len(w.backbone) + sum(len(l.sequence) for l in w.layers) for w in windows]
+        heavy = sorted(range(len(windows)), key=lambda i: -size[i])[:args.serving_launch]
+        srv.warm([windows[i] for i in heavy])
+        for lam in rates:
+            n_req = max(6, int(lam * args.serving_seconds))
+            reqs = [make_request() for _ in range(n_req)]
+            gaps = [rng.expovariate(lam) for _ in range(n_req)]
+            launches0 = len(srv.launch_log)
+            handles = []
+            t0 = time.perf_counter()
+            due = t0
+            for ws, gap in zip(reqs, gaps):
+                due += gap
+                time.sleep(max(0.0, due - time.perf_counter()))
+                handles.append(srv.submit(ws))
+            for h in handles:
+                h.wait()
+            t1 = max(h.done for h in handles)
+            lat = [h.latency for h in handles]
+            n_win = sum(len(ws) for ws in reqs)
+            log_slice = srv.launch_log[launches0:]
+            row = {"rate": lam, "requests": n_req, "windows": n_win,
+                   "windows_per_s": round(n_win / (t1 - t0), 1),
+                   "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
+                   "mean_wait_s": round(sum(h.admitted - h.arrival for h in handles) / n_req, 4),
+                   "launches": len(log_slice),
+                   "mixed_launches": sum(1 for _, _, k in log_slice if k > 1)}
+            log(f"serving rate {lam}/s: {n_req} requests, {n_win} windows, "
+                f"p50 {row['p50_s']} s, p99 {row['p99_s']} s, {row['windows_per_s']} windows/s, "
+                f"{row['mixed_launches']}/{row['launches']} launches mixed")
+            report["rates"].append(row)
+    return report
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +212,17 @@ def main() -> int:
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
                     help="HBM bytes per POA launch measured with rocprofv3 (FETCH_SIZE x2 + "
                          "WRITE_SIZE, copy-calibrated) to report in roofline.traffic")
+    ap.add_argument("--serving", default="", metavar="RATES",
+                    help="only run the serving loop: Poisson request arrivals at these rates "
+                         "(requests/s, comma separated), requests cut from the config's windows")
+    ap.add_argument("--serving-seconds", type=float, default=8.0,
+                    help="arrival span per rate (at least 6 requests per rate)")
+    ap.add_argument("--serving-budget", type=float, default=8.0,
+                    help="admission budget in GiB of window bytes")
+    ap.add_argument("--serving-executors", type=int, default=4,
+                    help="concurrent launches (one POA batch and HIP stream each)")
+    ap.add_argument("--serving-launch", type=int, default=2048,
+                    help="most windows per launch")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (torch), to scale "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE readings, then exit")
@@ -172,6 +257,9 @@ def main() -> int:
     pairs = read_overlaps(odump)
     log(f"loaded {len(windows)} windows, {len(pairs)} overlaps in {time.time() - t0:.1f} s")
     m, x, g = 3, -5, -4
+    if args.serving:
+        print(json.dumps(serving_bench(windows, args, (m, x, g))), flush=True)
+        return 0
 
     # ---- batched persistent kernel, windows resident in HBM ----
     batch = racon_amd.PoaBatch(m, x, g, trim=True)

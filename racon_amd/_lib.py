@@ -7,7 +7,10 @@ built: there is no fallback implementation behind it.
 from __future__ import annotations
 
 import ctypes
+import itertools
+import operator
 import os
+import sys
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -178,6 +181,84 @@ class Window:
     tgs: bool = True
 
 
+def _address(blob: bytes) -> int:
+    return ctypes.cast(ctypes.c_char_p(blob), ctypes.c_void_p).value or 0
+
+
+# CPython keeps a bytes object's data at a fixed offset from the object
+# (the address id() returns); checked once here, else the parts are joined
+# into one buffer first.
+_BYTES_DATA = sys.getsizeof(b"") - 1
+_probe = b"ACGT"
+_DIRECT = _address(_probe) == id(_probe) + _BYTES_DATA
+
+
+def _pointers(parts: Sequence[Optional[bytes]]):
+    """Address of each part's bytes (0 for None) without copying them where
+    the interpreter allows; returns (addresses, lengths, objects to keep)."""
+    import numpy as np
+    n = len(parts)
+    present = np.fromiter(map(operator.is_not, parts, itertools.repeat(None)), dtype=bool,
+                          count=n)
+    if _DIRECT:
+        addr = np.fromiter(map(id, parts), dtype=np.uint64, count=n) + np.uint64(_BYTES_DATA)
+        lens = np.zeros(n, dtype=np.uint64)
+        lens[present] = np.fromiter(map(len, itertools.compress(parts, present)),
+                                    dtype=np.uint64, count=int(present.sum()))
+        keep = parts
+    else:
+        lens = np.fromiter((len(p) if p is not None else 0 for p in parts), dtype=np.uint64,
+                           count=n)
+        keep = b"".join(p for p in parts if p is not None) + b"\0"
+        addr = np.zeros(n, dtype=np.uint64)
+        if n > 1:
+            np.cumsum(lens[:-1], out=addr[1:])
+        addr += np.uint64(_address(keep))
+    return np.where(present, addr, np.uint64(0)), lens, keep
+
+
+def pack_windows(windows: Sequence[Window]):
+    """The rh_window / rh_layer arrays of `windows` built in bulk (numpy
+    records laid out as the C structs, pointers into a few joined buffers).
+    Returns (window records, objects that must outlive the upload call)."""
+    import numpy as np
+    layer_dt = np.dtype([("sequence", "<u8"), ("quality", "<u8"), ("length", "<u4"),
+                         ("begin", "<u4"), ("end", "<u4")], align=True)
+    window_dt = np.dtype([("backbone", "<u8"), ("backbone_quality", "<u8"),
+                          ("backbone_length", "<u4"), ("n_layers", "<u4"),
+                          ("layers", "<u8"), ("tgs", "<u4")], align=True)
+    assert layer_dt.itemsize == ctypes.sizeof(RhLayer)
+    assert window_dt.itemsize == ctypes.sizeof(RhWindow)
+    n = len(windows)
+    flat = [l for w in windows for l in w.layers]
+    layers = np.zeros(max(1, len(flat)), dtype=layer_dt)
+    get_begin, get_end = operator.attrgetter("begin"), operator.attrgetter("end")
+    s_ptr, s_len, s_blob = _pointers(list(map(operator.attrgetter("sequence"), flat)))
+    q_ptr, _, q_blob = _pointers(list(map(operator.attrgetter("quality"), flat)))
+    if flat:
+        layers["sequence"] = s_ptr
+        layers["quality"] = q_ptr
+        layers["length"] = s_len
+        layers["begin"] = np.fromiter(map(get_begin, flat), dtype=np.uint32, count=len(flat))
+        layers["end"] = np.fromiter(map(get_end, flat), dtype=np.uint32, count=len(flat))
+    arr = np.zeros(max(1, n), dtype=window_dt)
+    b_ptr, b_len, b_blob = _pointers([w.backbone for w in windows])
+    bq_ptr, _, bq_blob = _pointers([w.backbone_quality for w in windows])
+    n_layers = np.fromiter((len(w.layers) for w in windows), dtype=np.uint64, count=n)
+    first = np.zeros(n, dtype=np.uint64)
+    if n > 1:
+        np.cumsum(n_layers[:-1], out=first[1:])
+    if n:
+        arr["backbone"][:n] = b_ptr
+        arr["backbone_quality"][:n] = bq_ptr
+        arr["backbone_length"][:n] = b_len
+        arr["n_layers"][:n] = n_layers
+        arr["layers"][:n] = np.uint64(layers.ctypes.data) + first * np.uint64(layer_dt.itemsize)
+        arr["tgs"][:n] = np.fromiter((1 if w.tgs else 0 for w in windows), dtype=np.uint32,
+                                     count=n)
+    return arr, (layers, s_blob, q_blob, b_blob, bq_blob)
+
+
 class PoaBatch:
     """A device batch of windows (rh_poa_*)."""
 
@@ -206,24 +287,10 @@ class PoaBatch:
                "rh_poa_set_tuning")
 
     def upload(self, windows: Sequence[Window]) -> None:
-        arr = (RhWindow * len(windows))()
-        keep = []
-        for i, w in enumerate(windows):
-            layers = (RhLayer * max(1, len(w.layers)))()
-            for k, l in enumerate(w.layers):
-                layers[k].sequence = l.sequence
-                layers[k].quality = l.quality
-                layers[k].length = len(l.sequence)
-                layers[k].begin = l.begin
-                layers[k].end = l.end
-            keep.append(layers)
-            arr[i].backbone = w.backbone
-            arr[i].backbone_quality = w.backbone_quality
-            arr[i].backbone_length = len(w.backbone)
-            arr[i].n_layers = len(w.layers)
-            arr[i].layers = layers
-            arr[i].tgs = 1 if w.tgs else 0
-        _check(self.lib.rh_poa_upload(self.handle, len(windows), arr), "rh_poa_upload")
+        arr, keep = pack_windows(windows)
+        _check(self.lib.rh_poa_upload(self.handle, len(windows),
+                                      ctypes.cast(arr.ctypes.data, ctypes.POINTER(RhWindow))),
+               "rh_poa_upload")
         self._keep = keep
         self.n = len(windows)
 
