@@ -831,6 +831,109 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
   }
 }
 
+// Scores and move codes of one packed row (registers only).
+template <int W>
+struct PackedOut {
+  uint32_t h[W / 2];
+  uint32_t cd[W / 2];
+  int32_t h0;  // column 0
+};
+
+template <int W, bool kMulti>
+__device__ __forceinline__ void packed_compute(const PackedStrip<W>& ps, const uint32_t* pp,
+                                               uint32_t hp0, const uint32_t* kk, uint32_t k0,
+                                               uint32_t b, uint32_t pay0, int lane, int32_t gap,
+                                               PackedOut<W>& o) {
+  constexpr int H2 = W / 2;
+  const uint32_t bb = b * 0x10001u;
+  // column 0 (hp0 + gap) heads lane 0's chain
+  o.h0 = pk_half(hp0 + static_cast<uint32_t>(gap), 0);
+  uint32_t carry = lane == 0 ? pk_splat(o.h0) : 0x80008000u;
+  uint32_t bd[H2], bv[H2], pre[H2];
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    // diagonal sources H[p][j - 1] of the pair
+    const uint32_t d = q == 0 ? ((pp[0] << 16) | (hp0 & 0xffffu))
+                              : __builtin_amdgcn_alignbit(pp[q], pp[q - 1], 16);
+    const uint32_t mis = pk_min_u(ps.base[q] ^ bb, ps.one);  // 1 where the bases differ
+    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match_gap));
+    bv[q] = pk_add(pp[q], ps.gap);
+    carry = pk_max_hi(pk_scan(pk_max(bd[q], bv[q])), carry);
+    pre[q] = carry;
+  }
+  const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
+  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
+  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    o.h[q] = pk_max_lo(pre[q], static_cast<uint32_t>(excl));
+    // diagonal first, then vertical, else horizontal, from 0/1 flags
+    // fd (not diagonal) and fv (not vertical)
+    const uint32_t fd = pk_min_u(pk_sub(o.h[q], bd[q]), ps.one);
+    const uint32_t fv = pk_min_u(pk_sub(o.h[q], bv[q]), ps.one);
+    if (kMulti) {
+      // codes of the first best predecessor: kCodeViaK | k
+      const uint32_t kd = q == 0 ? ((kk[0] << 16) | (k0 & 0xffffu))
+                                 : __builtin_amdgcn_alignbit(kk[q], kk[q - 1], 16);
+      const uint32_t dc = pk_add(kd, 0x20002000u);     // diagonal | kCodeViaK | k
+      const uint32_t vc = pk_add(kk[q], 0x60006000u);  // vertical | kCodeViaK | k
+      const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
+      o.cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
+    } else {
+      // pay + fd * (0x4000 + fv * (0x4000 - pay))
+      o.cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+    }
+    // computed here, not sunk into the store branch, so that a second row's
+    // scan can interleave with this row's codes
+    asm volatile("" : "+v"(o.cd[q]));
+  }
+}
+
+// Stores of one packed row (codes, the row's scores where later rows read
+// them) and H[row][L] as the return value.
+template <int W, bool kMulti>
+__device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t k0, uint32_t pay0,
+                                                int lane, int j0, bool live, int L, int32_t gap,
+                                                int own, int pitch, int16_t* ring, int32_t* Hg,
+                                                uint16_t* Cr, int end_lane, int end_u) {
+  constexpr int H2 = W / 2;
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(o.cd[2 * q], o.cd[2 * q + 1]);
+    }
+    if (own < kSlotGlobal) {
+      int16_t* row = ring + own * pitch + kColShift;
+#pragma unroll
+      for (int q = 0; q < W / 4; ++q) {
+        *reinterpret_cast<uint2*>(row + j0 + 4 * q) = make_uint2(o.h[2 * q], o.h[2 * q + 1]);
+      }
+    }
+  }
+  if (own == kSlotGlobal) {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      if (j0 + u <= L) Hg[j0 + u] = pk_half(o.h[u / 2], u & 1);
+    }
+  }
+  if (lane == 0) {
+    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(o.h0);
+    if (own == kSlotGlobal) Hg[0] = o.h0;
+    Cr[pitch - 1] = static_cast<uint16_t>(
+        kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu)) : (kCodeVert | pay0));
+  }
+  // (each pair opaque, so that the selection stays register selects rather
+  // than an indexed load from a stack copy of the row)
+  uint32_t lastp = o.h[0];
+#pragma unroll
+  for (int q = 1; q < H2; ++q) {
+    uint32_t v = o.h[q];
+    asm volatile("" : "+v"(v));
+    if (q == (end_u >> 1)) lastp = v;
+  }
+  return __builtin_amdgcn_readlane(pk_half(lastp, end_u & 1), end_lane) + L * gap;
+}
+
 // One DP row in packed pairs: the moves, codes, stores and end-column value
 // of finish_row, two columns per instruction. pp holds the predecessor
 // scores at the strip's columns and hp0 (low half) the one at column j0 - 1.
@@ -853,78 +956,10 @@ __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const ui
                                               int j0, bool live, int L, int32_t gap, int own,
                                               int pitch, int16_t* ring, int32_t* Hg, uint16_t* Cr,
                                               int end_lane, int end_u) {
-  constexpr int H2 = W / 2;
-  const uint32_t bb = b * 0x10001u;
-  // column 0 (hp0 + gap) heads lane 0's chain
-  const int32_t h0 = pk_half(hp0 + static_cast<uint32_t>(gap), 0);
-  uint32_t carry = lane == 0 ? pk_splat(h0) : 0x80008000u;
-  uint32_t bd[H2], bv[H2], pre[H2];
-#pragma unroll
-  for (int q = 0; q < H2; ++q) {
-    // diagonal sources H[p][j - 1] of the pair
-    const uint32_t d = q == 0 ? ((pp[0] << 16) | (hp0 & 0xffffu))
-                              : __builtin_amdgcn_alignbit(pp[q], pp[q - 1], 16);
-    const uint32_t mis = pk_min_u(ps.base[q] ^ bb, ps.one);  // 1 where the bases differ
-    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match_gap));
-    bv[q] = pk_add(pp[q], ps.gap);
-    carry = pk_max_hi(pk_scan(pk_max(bd[q], bv[q])), carry);
-    pre[q] = carry;
-  }
-  const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
-  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
-  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
-  uint32_t h[H2], cd[H2];
-#pragma unroll
-  for (int q = 0; q < H2; ++q) {
-    h[q] = pk_max_lo(pre[q], static_cast<uint32_t>(excl));
-    // diagonal first, then vertical, else horizontal, from 0/1 flags
-    // fd (not diagonal) and fv (not vertical)
-    const uint32_t fd = pk_min_u(pk_sub(h[q], bd[q]), ps.one);
-    const uint32_t fv = pk_min_u(pk_sub(h[q], bv[q]), ps.one);
-    if (kMulti) {
-      // codes of the first best predecessor: kCodeViaK | k
-      const uint32_t kd = q == 0 ? ((kk[0] << 16) | (k0 & 0xffffu))
-                                 : __builtin_amdgcn_alignbit(kk[q], kk[q - 1], 16);
-      const uint32_t dc = pk_add(kd, 0x20002000u);     // diagonal | kCodeViaK | k
-      const uint32_t vc = pk_add(kk[q], 0x60006000u);  // vertical | kCodeViaK | k
-      const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
-      cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
-    } else {
-      // pay + fd * (0x4000 + fv * (0x4000 - pay))
-      cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
-    }
-  }
-  if (live) {
-#pragma unroll
-    for (int q = 0; q < W / 4; ++q) {
-      *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(cd[2 * q], cd[2 * q + 1]);
-    }
-    if (own < kSlotGlobal) {
-      int16_t* row = ring + own * pitch + kColShift;
-#pragma unroll
-      for (int q = 0; q < W / 4; ++q) {
-        *reinterpret_cast<uint2*>(row + j0 + 4 * q) = make_uint2(h[2 * q], h[2 * q + 1]);
-      }
-    }
-  }
-  if (own == kSlotGlobal) {
-#pragma unroll
-    for (int u = 0; u < W; ++u) {
-      if (j0 + u <= L) Hg[j0 + u] = pk_half(h[u / 2], u & 1);
-    }
-  }
-  if (lane == 0) {
-    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(h0);
-    if (own == kSlotGlobal) Hg[0] = h0;
-    Cr[pitch - 1] = static_cast<uint16_t>(
-        kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu)) : (kCodeVert | pay0));
-  }
-  uint32_t lastp = h[0];
-#pragma unroll
-  for (int q = 1; q < H2; ++q) {
-    if (q == (end_u >> 1)) lastp = h[q];
-  }
-  return __builtin_amdgcn_readlane(pk_half(lastp, end_u & 1), end_lane) + L * gap;
+  PackedOut<W> o;
+  packed_compute<W, kMulti>(ps, pp, hp0, kk, k0, b, pay0, lane, gap, o);
+  return packed_store<W, kMulti>(o, k0, pay0, lane, j0, live, L, gap, own, pitch, ring, Hg, Cr,
+                                 end_lane, end_u);
 }
 
 // DP rows of one layer with 16-bit scores, every row in one pass of the
@@ -967,6 +1002,21 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
   int best_row = 0;
   int ties = 0;
   const int total_preds = uniform(s.pred_off()[V]);
+  // sink rows: record the end-column score and keep the best (in row order)
+  auto account = [&](int32_t h, int row, int m) {
+    if ((m & kRowMetaSink) != 0) {
+      if (lane == 0) {
+        s.sink_score()[row] = h;
+      }
+      if (h > best) {
+        best = h;
+        best_row = row;
+        ties = 0;
+      } else if (h == best) {
+        ++ties;
+      }
+    }
+  };
   for (int r0 = 1; r0 <= V; r0 += kWave) {
     const int rk = r0 + lane;
     int off_v = 0, end_v = 0, meta_v = 0;
@@ -992,16 +1042,17 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
       const int off = __builtin_amdgcn_readlane(off_v, rr);
       const int np = __builtin_amdgcn_readlane(end_v, rr) - off;
       const int meta = __builtin_amdgcn_readlane(meta_v, rr);
-      auto pred_at = [&](int k) -> int {
-        const int idx = off + k - base;
+      auto pred_of = [&](int o, int k) -> int {
+        const int idx = o + k - base;
         if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
         if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
         if (idx < 3 * kWave) return __builtin_amdgcn_readlane(pr2, idx - 2 * kWave);
         if (idx < 4 * kWave) return __builtin_amdgcn_readlane(pr3, idx - 3 * kWave);
-        const int e = uniform(s.pred_rows()[off + k]);
+        const int e = uniform(s.pred_rows()[o + k]);
         wait_stores();
         return e;
       };
+      auto pred_at = [&](int k) -> int { return pred_of(off, k); };
       const uint32_t b = static_cast<uint32_t>(meta & 0xff);
       const int own = (meta >> kRowMetaSlotShift) & 31;
       int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
@@ -1011,15 +1062,59 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         return ((sw[u / 4] >> (8 * (u % 4))) & 0xffu) == b ? match : mismatch;
       };
       int32_t bd[W], bv[W];
-      int32_t h_last;
+      int32_t h_last = 0;
+      // the row the common tail below accounts for (the second of a pair)
+      int tail_row = r, tail_meta = meta, tail_own = own;
       if (kPacked && np == 1) {
         const int e = pred_at(0);
         const uint32_t delta = static_cast<uint32_t>(r - (e & kPredRowMask));
         const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
-        uint32_t pp[W / 2];
-        const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pp);
-        h_last = packed_row<W, false>(ps, pp, hp0, pp, 0u, b, pay0, lane, j0, live, L, gap, own,
-                                      pitch, ring, Hg, Cr, end_lane, end_u);
+        // The next row as well when it has one predecessor other than this
+        // row and neither row keeps its scores in global memory: the rows
+        // are independent, so both loads go out before either row's
+        // arithmetic and the two rows' instructions interleave. (This row's
+        // LDS slot holds no row the next one reads: a slot is reused only
+        // after the last reader of its row.)
+        bool paired = false;
+        if (W <= 12 && rr + 1 < chunk && own != kSlotGlobal) {  // (16 would spill)
+          const int off2 = __builtin_amdgcn_readlane(off_v, rr + 1);
+          const int np2 = __builtin_amdgcn_readlane(end_v, rr + 1) - off2;
+          const int meta2 = __builtin_amdgcn_readlane(meta_v, rr + 1);
+          const int own2 = (meta2 >> kRowMetaSlotShift) & 31;
+          if (np2 == 1 && own2 != kSlotGlobal) {
+            const int e2 = pred_of(off2, 0);
+            if ((e2 & kPredRowMask) != r) {
+              paired = true;
+              const uint32_t delta2 = static_cast<uint32_t>(r + 1 - (e2 & kPredRowMask));
+              const uint32_t pay2 = delta2 <= kCodePayload ? delta2 : kCodeViaK;
+              uint32_t pa[W / 2], pb[W / 2];
+              const uint32_t ha0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pa);
+              const uint32_t hb0 = load_pairs<W>(ring, pitch, s, e2, j0, jr, L, gap, ps, pb);
+              PackedOut<W> oa, ob;
+              packed_compute<W, false>(ps, pa, ha0, pa, 0u, b, pay0, lane, gap, oa);
+              packed_compute<W, false>(ps, pb, hb0, pb, 0u, static_cast<uint32_t>(meta2 & 0xff),
+                                       pay2, lane, gap, ob);
+              const int32_t h_a = packed_store<W, false>(oa, 0u, pay0, lane, j0, live, L, gap,
+                                                         own, pitch, ring, Hg, Cr, end_lane,
+                                                         end_u);
+              h_last = packed_store<W, false>(ob, 0u, pay2, lane, j0, live, L, gap, own2, pitch,
+                                              ring, s.H() + static_cast<int64_t>(r + 1) * pitch,
+                                              s.code() + static_cast<int64_t>(r + 1) * pitch,
+                                              end_lane, end_u);
+              account(h_a, r, meta);
+              tail_row = r + 1;
+              tail_meta = meta2;
+              tail_own = own2;
+              ++rr;
+            }
+          }
+        }
+        if (!paired) {
+          uint32_t pp[W / 2];
+          const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pp);
+          h_last = packed_row<W, false>(ps, pp, hp0, pp, 0u, b, pay0, lane, j0, live, L, gap,
+                                        own, pitch, ring, Hg, Cr, end_lane, end_u);
+        }
       } else if (kPacked) {
         // best predecessor score of every column and the first predecessor
         // reaching it (a later one replaces it only when strictly better)
@@ -1078,19 +1173,8 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         h_last = finish_row<W, true>(bd, bv, c0k, 0u, lane, j0, live, L, gap, own, pitch, ring,
                                      Hg, Cr, end_lane, end_u);
       }
-      if ((meta & kRowMetaSink) != 0) {
-        if (lane == 0) {
-          s.sink_score()[r] = h_last;
-        }
-        if (h_last > best) {
-          best = h_last;
-          best_row = r;
-          ties = 0;
-        } else if (h_last == best) {
-          ++ties;
-        }
-      }
-      if (own == kSlotGlobal) {
+      account(h_last, tail_row, tail_meta);
+      if (tail_own == kSlotGlobal) {
         wait_stores();
       }
       wave_fence();
