@@ -8,6 +8,7 @@
 
 #include "overlap.hpp"
 #include "sequence.hpp"
+#include "thread_pool.hpp"
 
 namespace racon {
 
@@ -48,6 +49,26 @@ bool overlap_format_of(const std::string& path, OverlapFormat* format) {
 }
 
 std::string read_file(const std::string& path) {
+  // plain files: one read into a buffer of the file's size
+  if (FILE* p = fopen(path.c_str(), "rb")) {
+    unsigned char magic[2] = {0, 0};
+    const size_t got = fread(magic, 1, 2, p);
+    const bool gz = got == 2 && magic[0] == 0x1f && magic[1] == 0x8b;
+    if (!gz && fseek(p, 0, SEEK_END) == 0) {
+      const long size = ftell(p);
+      if (size >= 0 && fseek(p, 0, SEEK_SET) == 0) {
+        std::string out(static_cast<size_t>(size), '\0');
+        const size_t n = size > 0 ? fread(&out[0], 1, out.size(), p) : 0;
+        fclose(p);
+        if (n != out.size()) {
+          fprintf(stderr, "[racon::io] error: unable to read file %s!\n", path.c_str());
+          exit(1);
+        }
+        return out;
+      }
+    }
+    fclose(p);
+  }
   gzFile f = gzopen(path.c_str(), "rb");
   if (f == nullptr) {
     fprintf(stderr, "[racon::io] error: unable to open file %s!\n", path.c_str());
@@ -156,36 +177,58 @@ void malformed(const std::string& path) {
 
 }  // namespace
 
+namespace {
+
+// A parsed record before its Sequence is built: the name and the line spans
+// of its data (and quality), as [first, last) into a flat span list.
+struct Span {
+  const char* p;
+  size_t n;
+};
+struct Record {
+  Span name;
+  uint32_t d0, d1, q0, q1;
+};
+
+// Bytes of spans [a, b) as one buffer: the line itself when there is one,
+// else joined into `buf`.
+Span joined(const std::vector<Span>& spans, uint32_t a, uint32_t b, std::string& buf) {
+  if (b == a + 1) return spans[a];
+  buf.clear();
+  for (uint32_t i = a; i < b; ++i) buf.append(spans[i].p, spans[i].n);
+  return Span{buf.data(), buf.size()};
+}
+
+}  // namespace
+
+// One serial pass over the lines finds the records (and every malformed
+// one, in file order, as before); the Sequence objects (upper-casing,
+// copies, the quality check) are then built in parallel on `pool`.
 void parse_sequences(const std::string& path, SequenceFormat format,
-                     std::vector<std::unique_ptr<Sequence>>& dst) {
+                     std::vector<std::unique_ptr<Sequence>>& dst, ThreadPool* pool) {
   std::string text = read_file(path);
   Lines lines(text);
   const char* line;
   size_t length;
-  if (format == SequenceFormat::kFasta) {
-    std::string name, data;
+  std::vector<Record> recs;
+  std::vector<Span> spans;
+  const bool fastq = format != SequenceFormat::kFasta;
+  if (!fastq) {
     bool open = false;
-    auto flush = [&]() {
-      if (open) {
-        dst.emplace_back(new Sequence(name.data(), name.size(), data.data(),
-                                      data.size()));
-      }
-    };
     while (lines.next(&line, &length)) {
       if (length > 0 && line[0] == '>') {
-        flush();
-        name.assign(line + 1, name_length(line + 1, length - 1));
-        data.clear();
+        if (open) recs.back().d1 = static_cast<uint32_t>(spans.size());
+        recs.push_back(Record{Span{line + 1, name_length(line + 1, length - 1)},
+                              static_cast<uint32_t>(spans.size()), 0, 0, 0});
         open = true;
       } else if (open) {
-        data.append(line, length);
+        spans.push_back(Span{line, length});
       } else if (length > 0) {
         malformed(path);
       }
     }
-    flush();
+    if (open) recs.back().d1 = static_cast<uint32_t>(spans.size());
   } else {
-    std::string name, data, quality;
     while (lines.next(&line, &length)) {
       if (length == 0) {
         continue;
@@ -193,30 +236,61 @@ void parse_sequences(const std::string& path, SequenceFormat format,
       if (line[0] != '@') {
         malformed(path);
       }
-      name.assign(line + 1, name_length(line + 1, length - 1));
-      data.clear();
-      quality.clear();
+      Record r{Span{line + 1, name_length(line + 1, length - 1)},
+               static_cast<uint32_t>(spans.size()), 0, 0, 0};
+      size_t data_size = 0;
       bool plus = false;
       while (lines.next(&line, &length)) {
         if (length > 0 && line[0] == '+') {
           plus = true;
           break;
         }
-        data.append(line, length);
+        spans.push_back(Span{line, length});
+        data_size += length;
       }
       if (!plus) {
         malformed(path);
       }
-      while (quality.size() < data.size() && lines.next(&line, &length)) {
-        quality.append(line, length);
+      r.d1 = r.q0 = static_cast<uint32_t>(spans.size());
+      size_t quality_size = 0;
+      while (quality_size < data_size && lines.next(&line, &length)) {
+        spans.push_back(Span{line, length});
+        quality_size += length;
       }
-      if (quality.size() != data.size()) {
+      if (quality_size != data_size) {
         malformed(path);
       }
-      dst.emplace_back(new Sequence(name.data(), name.size(), data.data(),
-                                    data.size(), quality.data(), quality.size()));
+      r.q1 = static_cast<uint32_t>(spans.size());
+      recs.push_back(r);
     }
   }
+
+  const size_t base = dst.size();
+  dst.resize(base + recs.size());
+  auto build = [&](size_t begin, size_t end) {
+    std::string dbuf, qbuf;
+    for (size_t i = begin; i < end; ++i) {
+      const Record& r = recs[i];
+      const Span d = r.d1 > r.d0 ? joined(spans, r.d0, r.d1, dbuf) : Span{"", 0};
+      if (fastq) {
+        const Span q = r.q1 > r.q0 ? joined(spans, r.q0, r.q1, qbuf) : Span{"", 0};
+        dst[base + i].reset(new Sequence(r.name.p, r.name.n, d.p, d.n, q.p, q.n));
+      } else {
+        dst[base + i].reset(new Sequence(r.name.p, r.name.n, d.p, d.n));
+      }
+    }
+  };
+  const size_t workers = pool != nullptr ? pool->size() : 1;
+  if (workers <= 1 || recs.size() < 64) {
+    build(0, recs.size());
+    return;
+  }
+  const size_t chunk = (recs.size() + 4 * workers - 1) / (4 * workers);
+  std::vector<std::future<void>> done;
+  for (size_t b = 0; b < recs.size(); b += chunk) {
+    done.emplace_back(pool->submit(build, b, std::min(recs.size(), b + chunk)));
+  }
+  for (auto& f : done) f.wait();
 }
 
 void parse_overlaps(const std::string& path, OverlapFormat format,

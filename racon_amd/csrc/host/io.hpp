@@ -12,6 +12,7 @@ namespace racon {
 
 class Sequence;
 class Overlap;
+class ThreadPool;
 
 enum class SequenceFormat { kFasta, kFastq };
 enum class OverlapFormat { kMhap, kPaf, kSam };
@@ -25,8 +26,10 @@ bool overlap_format_of(const std::string& path, OverlapFormat* format);
 // failure.
 std::string read_file(const std::string& path);
 
+// Appends the file's records to dst; with a pool, the Sequence objects are
+// built on its threads (same objects, same order).
 void parse_sequences(const std::string& path, SequenceFormat format,
-                     std::vector<std::unique_ptr<Sequence>>& dst);
+                     std::vector<std::unique_ptr<Sequence>>& dst, ThreadPool* pool = nullptr);
 void parse_overlaps(const std::string& path, OverlapFormat format,
                     std::vector<std::unique_ptr<Overlap>>& dst);
 

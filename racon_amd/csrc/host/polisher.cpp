@@ -135,7 +135,7 @@ void Polisher::initialize() {
   auto t_load = std::chrono::steady_clock::now();
   logger_->log();
 
-  parse_sequences(target_path_, target_format_, sequences_);
+  parse_sequences(target_path_, target_format_, sequences_, thread_pool_.get());
   uint64_t targets_size = sequences_.size();
   if (targets_size == 0) {
     fprintf(stderr, "[racon::Polisher::initialize] error: "
@@ -160,7 +160,7 @@ void Polisher::initialize() {
   uint64_t sequences_size = 0, total_sequences_length = 0;
   {
     uint64_t l = sequences_.size();
-    parse_sequences(sequences_path_, sequences_format_, sequences_);
+    parse_sequences(sequences_path_, sequences_format_, sequences_, thread_pool_.get());
     uint64_t n = 0;
     for (uint64_t i = l; i < sequences_.size(); ++i, ++sequences_size) {
       total_sequences_length += sequences_[i]->data().size();

@@ -9,12 +9,29 @@ std::unique_ptr<Sequence> createSequence(const std::string& name,
   return std::unique_ptr<Sequence>(new Sequence(name, data));
 }
 
+namespace {
+
+// toupper of the "C" locale (the only one racon runs in) as a byte table
+struct UpperTable {
+  char map[256];
+  UpperTable() {
+    for (int c = 0; c < 256; ++c) {
+      map[c] = static_cast<char>(c >= 'a' && c <= 'z' ? c - 'a' + 'A' : c);
+    }
+  }
+};
+const UpperTable kUpper;
+
+}  // namespace
+
 Sequence::Sequence(const char* name, uint32_t name_length, const char* data,
                    uint32_t data_length)
     : name_(name, name_length) {
+  // sequence.cpp:26: upper case
   data_.resize(data_length);
+  char* out = &data_[0];
   for (uint32_t i = 0; i < data_length; ++i) {
-    data_[i] = static_cast<char>(std::toupper(static_cast<unsigned char>(data[i])));
+    out[i] = kUpper.map[static_cast<unsigned char>(data[i])];
   }
 }
 
