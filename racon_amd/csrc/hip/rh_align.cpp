@@ -34,7 +34,7 @@ uint64_t span_windows(uint32_t window_length, const rh_break_span& s) {
 
 struct rh_align_batch {
   int32_t device = 0;
-  std::vector<uint8_t> arena;
+  rh::HostArena arena;  // sized without zero-filling
   std::vector<rh::AlignPair> pairs;
   std::vector<uint32_t> order;
   std::vector<rh::AlignResult> results;
@@ -116,17 +116,23 @@ int32_t rh_align_upload(rh_align_batch* b, uint32_t n, const rh_pair* pairs) {
     }
     total += pairs[i].query_length + pairs[i].target_length;
   }
+  // offsets in pair order first, then the byte copies (on host threads)
   b->arena.clear();
-  b->arena.reserve(total);
+  b->arena.resize(total);
+  std::vector<rh::ArenaCopy> copies;
+  copies.reserve(2ull * n);
   b->pairs.assign(n, rh::AlignPair{});
   b->max_q = b->max_t = 0;
   uint64_t out = 0;
+  uint64_t cur = 0;
   for (uint32_t i = 0; i < n; ++i) {
     rh::AlignPair& p = b->pairs[i];
-    p.q_off = b->arena.size();
-    b->arena.insert(b->arena.end(), pairs[i].query, pairs[i].query + pairs[i].query_length);
-    p.t_off = b->arena.size();
-    b->arena.insert(b->arena.end(), pairs[i].target, pairs[i].target + pairs[i].target_length);
+    p.q_off = cur;
+    copies.push_back(rh::ArenaCopy{pairs[i].query, cur, pairs[i].query_length});
+    cur += pairs[i].query_length;
+    p.t_off = cur;
+    copies.push_back(rh::ArenaCopy{pairs[i].target, cur, pairs[i].target_length});
+    cur += pairs[i].target_length;
     p.q_len = pairs[i].query_length;
     p.t_len = pairs[i].target_length;
     p.out_off = out;
@@ -136,6 +142,7 @@ int32_t rh_align_upload(rh_align_batch* b, uint32_t n, const rh_pair* pairs) {
     b->stats.cells += static_cast<uint64_t>(p.q_len) * p.t_len;
   }
   b->ops_bytes = out;
+  rh::run_copies(b->arena.data(), copies, total);
   b->order.resize(n);
   std::iota(b->order.begin(), b->order.end(), 0u);
   std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {

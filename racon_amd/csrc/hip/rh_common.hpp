@@ -4,9 +4,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "racon_hip.h"
 
@@ -27,6 +32,57 @@ void clear_error();
 inline double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
       .count();
+}
+
+// Allocator whose resize leaves new elements uninitialised (host arenas
+// whose bytes are all overwritten by the packing copies).
+template <typename T>
+struct DefaultInit : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  using std::allocator<T>::allocator;
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <typename U, typename... Args>
+  void construct(U* p, Args&&... args) {
+    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+  }
+};
+
+using HostArena = std::vector<uint8_t, DefaultInit<uint8_t>>;
+
+// One host-to-arena copy of a packing pass.
+struct ArenaCopy {
+  const void* src;
+  uint64_t dst;
+  uint32_t len;
+};
+
+// Runs the copies into `base`, on up to 16 host threads when the batch is
+// large (the copies do not overlap).
+inline void run_copies(uint8_t* base, const std::vector<ArenaCopy>& copies, uint64_t bytes) {
+  const size_t nc = copies.size();
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const unsigned workers = bytes >= (16u << 20) ? hw : 1u;
+  auto run = [&](size_t begin, size_t end) {
+    for (size_t c = begin; c < end; ++c) {
+      if (copies[c].len != 0) std::memcpy(base + copies[c].dst, copies[c].src, copies[c].len);
+    }
+  };
+  if (workers <= 1 || nc < 2) {
+    run(0, nc);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t chunk = (nc + workers - 1) / workers;
+  for (size_t c = 0; c < nc; c += chunk) {
+    pool.emplace_back(run, c, std::min(nc, c + chunk));
+  }
+  for (auto& t : pool) t.join();
 }
 
 // Device memory buffer that grows on demand (never shrinks).

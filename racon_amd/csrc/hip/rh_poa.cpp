@@ -36,25 +36,6 @@ const char* last_error() { return g_error.c_str(); }
 
 }  // namespace rh
 
-// Allocator whose resize leaves new bytes uninitialised (they are all
-// overwritten by the packing copies).
-template <typename T>
-struct DefaultInit : std::allocator<T> {
-  template <typename U>
-  struct rebind {
-    using other = DefaultInit<U>;
-  };
-  using std::allocator<T>::allocator;
-  template <typename U>
-  void construct(U* p) noexcept {
-    ::new (static_cast<void*>(p)) U;
-  }
-  template <typename U, typename... Args>
-  void construct(U* p, Args&&... args) {
-    ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
-  }
-};
-
 struct rh_poa_batch {
   int32_t device = 0;
   int32_t match = 3, mismatch = -5, gap = -4;
@@ -64,7 +45,7 @@ struct rh_poa_batch {
   uint32_t tune_ring = 0;
 
   // host copies of the packed batch
-  std::vector<uint8_t, DefaultInit<uint8_t>> arena;  // sized without zero-filling
+  rh::HostArena arena;  // sized without zero-filling
   std::vector<rh::PoaWindow> windows;
   std::vector<rh::PoaLayer> layers;
   std::vector<uint32_t> gpu_order;        // windows that need the kernel
@@ -330,16 +311,11 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   }
 
   // offsets first (same layout as appending in order), bytes copied after
-  struct Copy {
-    const char* src;
-    uint64_t dst;
-    uint32_t len;
-  };
-  std::vector<Copy> copies;
+  std::vector<rh::ArenaCopy> copies;
   copies.reserve(total / 256 + 2ull * n);
   uint64_t cur = 0;
   auto place = [&](const char* src, uint32_t len) {
-    copies.push_back(Copy{src, cur, len});
+    copies.push_back(rh::ArenaCopy{src, cur, len});
     const uint64_t at = cur;
     cur += len;
     return at;
@@ -417,27 +393,7 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   }
   b->out_bytes = out_off;
   b->arena.resize(cur);
-  {
-    // the byte copies, split over host threads for large batches
-    const size_t nc = copies.size();
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    const unsigned workers = cur >= (16u << 20) ? hw : 1u;
-    auto run = [&](size_t begin, size_t end) {
-      for (size_t c = begin; c < end; ++c) {
-        std::memcpy(b->arena.data() + copies[c].dst, copies[c].src, copies[c].len);
-      }
-    };
-    if (workers <= 1) {
-      run(0, nc);
-    } else {
-      std::vector<std::thread> pool;
-      const size_t chunk = (nc + workers - 1) / workers;
-      for (size_t c = 0; c < nc; c += chunk) {
-        pool.emplace_back(run, c, std::min(nc, c + chunk));
-      }
-      for (auto& t : pool) t.join();
-    }
-  }
+  rh::run_copies(b->arena.data(), copies, cur);
   // largest estimated cost first (longest-processing-time order)
   std::stable_sort(b->gpu_order.begin(), b->gpu_order.end(), [&](uint32_t x, uint32_t y) {
     return window_cost(b->windows[x]) > window_cost(b->windows[y]);
