@@ -65,29 +65,24 @@ struct Seg {
   __device__ uint8_t tc(int j) const { return rev ? t[tl - 1 - j] : t[j]; }
 };
 
-// Match words of one block for A, C, G, T and N; rows past the query end
-// match everything.
+// Match words of one block as bit planes: rows holding A, C, G or T (acgt)
+// with their 2-bit codes (lo, hi), rows holding N, and rows past the query
+// end (pad), which match everything. A target byte then finds its rows with
+// a few word operations and no per-byte table.
 struct Peq {
-  Word w[5];
+  Word lo, hi, acgt, n;
   Word pad;  // rows beyond the query
 };
 
-__device__ __forceinline__ int code_of(uint8_t c) {
-  switch (c) {
-    case 'A': return 0;
-    case 'C': return 1;
-    case 'G': return 2;
-    case 'T': return 3;
-    case 'N': return 4;
-    default: return 5;
-  }
+// 2-bit code of A, C, G, T (0..3) from the byte's bits 1 and 2; any byte
+// maps to some code, so is_acgt checks the byte against the code's letter.
+__device__ __forceinline__ int code2(int c) { return ((c >> 1) ^ (c >> 2)) & 3; }
+__device__ __forceinline__ bool is_acgt(int c, int code) {
+  return static_cast<int>((0x54474341u >> (8 * code)) & 0xffu) == c;  // "ACGT"
 }
 
 __device__ Peq build_peq(const Seg& g, int block) {
-  Peq p;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) p.w[k] = 0;
-  p.pad = 0;
+  Peq p{0, 0, 0, 0, 0};
   const int r0 = block * 64;
   for (int i = 0; i < 64; ++i) {
     const int r = r0 + i;
@@ -96,11 +91,16 @@ __device__ Peq build_peq(const Seg& g, int block) {
       p.pad |= bit;
       continue;
     }
-    const int c = code_of(g.qc(r));
-    if (c < 5) p.w[c] |= bit;
+    const int c = g.qc(r);
+    const int code = code2(c);
+    if (is_acgt(c, code)) {
+      p.acgt |= bit;
+      if (code & 1) p.lo |= bit;
+      if (code & 2) p.hi |= bit;
+    } else if (c == 'N') {
+      p.n |= bit;
+    }
   }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) p.w[k] |= p.pad;
   return p;
 }
 
@@ -144,23 +144,49 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
     }
     Word P = ~Word(0), M = 0;
     int score = (b + 1) * 64;  // D at the block's last row before column 0
-    int hprev = 0;             // carry received from the lane above
     const int steps = g.tl + nbs - 1;
     const uint64_t store_base = static_cast<uint64_t>(k) * (g.tl + 63) * 64;
+    // Lane b works on column c = st - b, the column lane b - 1 had one step
+    // earlier: each lane hands its target byte and its bottom carry (hout + 1
+    // in bits 8..9) to the next lane with one DPP wave shift. Lane 0 takes
+    // its byte (and, below the first strip, its carry from the strip above)
+    // from 64-column chunks that every lane loads one chunk ahead.
+    auto t_at = [&](int j) -> int { return j < g.tl ? g.tc(j) : 0; };
+    auto carry_at = [&](int j) -> int {
+      return k == 0 ? 1 : (j < g.tl ? static_cast<int>(s.carry[j]) : 1);
+    };
+    int t_cur = t_at(lane), t_next = t_at(kWave + lane);
+    int c_cur = carry_at(lane), c_next = carry_at(kWave + lane);
+    int fwd = 1 << 8;  // (byte, hout + 1) of this lane's last step
     for (int st = 0; st < steps; ++st) {
       const int c = st - lane;
-      const int hup = __shfl_up(hprev, 1, kWave);
+      if (st != 0 && (st & (kWave - 1)) == 0) {
+        t_cur = t_next;
+        c_cur = c_next;
+        t_next = t_at(st + kWave + lane);
+        c_next = carry_at(st + kWave + lane);
+      }
+      const int up = __builtin_amdgcn_update_dpp(0, fwd, 0x138, 0xf, 0xf, false);
       int hin;
+      uint8_t tc;
       if (lane == 0) {
-        hin = k == 0 ? 1 : (c >= 0 && c < g.tl ? static_cast<int>(s.carry[c]) : 1);
+        tc = static_cast<uint8_t>(__builtin_amdgcn_readlane(t_cur, st & (kWave - 1)));
+        hin = __builtin_amdgcn_readlane(c_cur, st & (kWave - 1));
       } else {
-        hin = hup;
+        tc = static_cast<uint8_t>(up & 0xff);
+        hin = (up >> 8) - 1;
       }
       int hout = 0;
       if (active_lane && c >= 0 && c < g.tl) {
-        const uint8_t tc = g.tc(c);
-        const int code = code_of(tc);
-        const Word eq = code < 5 ? peq.w[code] : rare_eq(g, b, tc, peq.pad);
+        // rows equal to the target byte (exact byte equality, as edlib's
+        // alphabet without equalities)
+        const int code = code2(tc);
+        const Word mlo = (code & 1) ? ~Word(0) : Word(0);
+        const Word mhi = (code & 2) ? ~Word(0) : Word(0);
+        Word eq = (peq.acgt & ~((peq.lo ^ mlo) | (peq.hi ^ mhi))) | peq.pad;
+        if (!is_acgt(tc, code)) {
+          eq = tc == 'N' ? (peq.n | peq.pad) : rare_eq(g, b, tc, peq.pad);
+        }
         hout = block_step(P, M, eq, hin);
         score += hout;
         if (store) {
@@ -183,7 +209,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
           s.carry[c] = static_cast<int8_t>(hout);
         }
       }
-      hprev = hout;
+      fwd = static_cast<int>(tc) | ((hout + 1) << 8);
     }
     // score of the last row (in the last strip's last block)
     if (k == strips - 1) {
