@@ -167,15 +167,11 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
         c_next = carry_at(st + kWave + lane);
       }
       const int up = __builtin_amdgcn_update_dpp(0, fwd, 0x138, 0xf, 0xf, false);
-      int hin;
-      uint8_t tc;
-      if (lane == 0) {
-        tc = static_cast<uint8_t>(__builtin_amdgcn_readlane(t_cur, st & (kWave - 1)));
-        hin = __builtin_amdgcn_readlane(c_cur, st & (kWave - 1));
-      } else {
-        tc = static_cast<uint8_t>(up & 0xff);
-        hin = (up >> 8) - 1;
-      }
+      // lane 0's inputs are uniform reads; selected, not branched to
+      const int t0 = __builtin_amdgcn_readlane(t_cur, st & (kWave - 1));
+      const int h0 = __builtin_amdgcn_readlane(c_cur, st & (kWave - 1));
+      const uint8_t tc = static_cast<uint8_t>(lane == 0 ? t0 : (up & 0xff));
+      const int hin = lane == 0 ? h0 : (up >> 8) - 1;
       int hout = 0;
       if (active_lane && c >= 0 && c < g.tl) {
         // rows equal to the target byte (exact byte equality, as edlib's
