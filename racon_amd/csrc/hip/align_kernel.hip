@@ -155,17 +155,22 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
     auto carry_at = [&](int j) -> int {
       return k == 0 ? 1 : (j < g.tl ? static_cast<int>(s.carry[j]) : 1);
     };
-    int t_cur = t_at(lane), t_next = t_at(kWave + lane);
-    int c_cur = carry_at(lane), c_next = carry_at(kWave + lane);
+    // (an outer loop per chunk: a loaded chunk is only read in the chunk
+    // after the one that issued its load, and no loaded value is carried
+    // through the step loop, which would make every step wait for memory)
+    int t_next = t_at(lane);
+    int c_next = carry_at(lane);
     int fwd = 1 << 8;  // (byte, hout + 1) of this lane's last step
-    for (int st = 0; st < steps; ++st) {
+    for (int st0 = 0; st0 < steps; st0 += kWave) {
+     const int t_cur = t_next, c_cur = c_next;
+     // this chunk's bytes and carries (loaded a chunk ago) are waited for
+     // here, once, before the next chunk's loads go out
+     asm volatile("" ::"v"(t_cur), "v"(c_cur));
+     t_next = t_at(st0 + kWave + lane);
+     c_next = carry_at(st0 + kWave + lane);
+     const int st_end = min(steps, st0 + kWave);
+     for (int st = st0; st < st_end; ++st) {
       const int c = st - lane;
-      if (st != 0 && (st & (kWave - 1)) == 0) {
-        t_cur = t_next;
-        c_cur = c_next;
-        t_next = t_at(st + kWave + lane);
-        c_next = carry_at(st + kWave + lane);
-      }
       const int up = __builtin_amdgcn_update_dpp(0, fwd, 0x138, 0xf, 0xf, false);
       // lane 0's inputs are uniform reads; selected, not branched to
       const int t0 = __builtin_amdgcn_readlane(t_cur, st & (kWave - 1));
@@ -206,6 +211,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
         }
       }
       fwd = static_cast<int>(tc) | ((hout + 1) << 8);
+     }
     }
     // score of the last row (in the last strip's last block)
     if (k == strips - 1) {
