@@ -315,6 +315,52 @@ void Polisher::initialize() {
 
   targets_coverages_.resize(targets_size, 0);
 
+  // the two layer filters of each breaking-point pair (span length, mean
+  // quality), decided on the pool; the layers are then added in overlap order
+  std::vector<std::vector<uint8_t>> passes(overlaps.size());
+  auto decide = [&](uint64_t begin, uint64_t end) {
+    for (uint64_t i = begin; i < end; ++i) {
+      const auto& sequence = sequences_[overlaps[i]->q_id()];
+      const auto& breaking_points = overlaps[i]->breaking_points();
+      auto& pass = passes[i];
+      pass.assign(breaking_points.size() / 2, 0);
+      for (uint32_t j = 0; j < breaking_points.size(); j += 2) {
+        if (breaking_points[j + 1].second - breaking_points[j].second <
+            0.02 * window_length_) {
+          continue;
+        }
+        if (!sequence->quality().empty() || !sequence->reverse_quality().empty()) {
+          const auto& quality = overlaps[i]->strand() ? sequence->reverse_quality()
+                                                      : sequence->quality();
+          double average_quality = 0;
+          for (uint32_t k = breaking_points[j].second;
+               k < breaking_points[j + 1].second; ++k) {
+            average_quality += static_cast<uint32_t>(quality[k]) - 33;
+          }
+          average_quality /= breaking_points[j + 1].second - breaking_points[j].second;
+          if (average_quality < quality_threshold_) {
+            continue;
+          }
+        }
+        pass[j / 2] = 1;
+      }
+    }
+  };
+  {
+    const uint64_t n = overlaps.size();
+    const uint64_t workers = thread_pool_->size();
+    if (workers <= 1 || n < 256) {
+      decide(0, n);
+    } else {
+      const uint64_t chunk = (n + 4 * workers - 1) / (4 * workers);
+      std::vector<std::future<void>> done;
+      for (uint64_t b = 0; b < n; b += chunk) {
+        done.emplace_back(thread_pool_->submit(decide, b, std::min(n, b + chunk)));
+      }
+      for (auto& f : done) f.wait();
+    }
+  }
+
   for (uint64_t i = 0; i < overlaps.size(); ++i) {
     ++targets_coverages_[overlaps[i]->t_id()];
 
@@ -322,23 +368,8 @@ void Polisher::initialize() {
     const auto& breaking_points = overlaps[i]->breaking_points();
 
     for (uint32_t j = 0; j < breaking_points.size(); j += 2) {
-      if (breaking_points[j + 1].second - breaking_points[j].second <
-          0.02 * window_length_) {
+      if (passes[i][j / 2] == 0) {
         continue;
-      }
-
-      if (!sequence->quality().empty() || !sequence->reverse_quality().empty()) {
-        const auto& quality = overlaps[i]->strand() ? sequence->reverse_quality()
-                                                    : sequence->quality();
-        double average_quality = 0;
-        for (uint32_t k = breaking_points[j].second;
-             k < breaking_points[j + 1].second; ++k) {
-          average_quality += static_cast<uint32_t>(quality[k]) - 33;
-        }
-        average_quality /= breaking_points[j + 1].second - breaking_points[j].second;
-        if (average_quality < quality_threshold_) {
-          continue;
-        }
       }
 
       uint64_t window_id = id_to_first_window_id[overlaps[i]->t_id()] +
