@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of racon's polishing hot path on one MI355X.
 
-Workload: one seeded random 5 Mbp contig with 50x
+Workload (cfg2): one seeded random 5 Mbp contig with 50x
 ONT-like reads (lognormal 8 +- 4 kbp, 12% errors at 0.4:0.2:0.4), PAF from
 true positions, window 500, scores 3/-5/-4. The inputs are generated on the
 box (racon_amd/bin/simgen, seed 1002), the product CLI aligns the overlaps on
@@ -229,9 +229,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the path would shard "
-                                   "windows across devices (see DESIGN.md), but no multi-GPU "
-                                   "figure is produced", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; windows would be "
+                                   "sharded across devices (independent windows, host-side "
+                                   "gather), but no multi-GPU figure is produced", "n_gpus": args.gpus}))
         return 2
 
     if args.copy_calibration:

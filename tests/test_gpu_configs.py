@@ -1,4 +1,4 @@
-"""BASELINE configs as parity cases: the GPU racon CLI vs the CPU oracle CLI,
+"""The benchmark workloads as parity cases: the GPU racon CLI vs the CPU oracle CLI,
 byte for byte, on seeded synthetic inputs (racon_amd/bin/simgen).
 
 - cfg1 at full size (50 kbp contig, 30x ONT 12%, w500, -t 1 semantics);
