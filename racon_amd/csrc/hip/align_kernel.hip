@@ -125,10 +125,12 @@ struct SlotA {
   uint8_t* ops_tmp;
 };
 
-// Runs the bit-vector DP of seg over all columns. With stop >= 0, writes the
-// exact scores of column `stop` for rows 0..ql-1 into col. With store, keeps
-// (P, M, score) of every (column, block) for the traceback. Returns the score
-// of the last cell (row ql-1, column tl-1) when stop < 0.
+// Runs the bit-vector DP of seg over all columns. With stop >= 0 (which must
+// be the last column, tl - 1: the only column the recursion asks for),
+// writes the exact scores of that column for rows 0..ql-1 into col, from
+// each block's state after its last step. With store, keeps (P, M, score)
+// of every (column, block) for the traceback. Returns the score of the last
+// cell (row ql-1, column tl-1) when stop < 0.
 __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const SlotA& s) {
   const int lane = lane_id();
   const int nb = (g.ql + 63) / 64;
@@ -196,22 +198,22 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
           s.store_m[idx] = M;
           s.store_s[idx] = score;
         }
-        if (c == stop) {
-          // exact values of this block's rows at the stop column
-          int v = score;
-          for (int i = 63; i >= 0; --i) {
-            const int r = b * 64 + i;
-            if (r < g.ql) col[r] = v;
-            if ((P >> i) & 1) v -= 1;
-            if ((M >> i) & 1) v += 1;
-          }
-        }
         if (lane == nbs - 1 && k + 1 < strips) {
           s.carry[c] = static_cast<int8_t>(hout);
         }
       }
       fwd = static_cast<int>(tc) | ((hout + 1) << 8);
      }
+    }
+    if (stop >= 0 && active_lane) {
+      // exact values of this block's rows at the last column
+      int v = score;
+      for (int i = 63; i >= 0; --i) {
+        const int r = b * 64 + i;
+        if (r < g.ql) col[r] = v;
+        if ((P >> i) & 1) v -= 1;
+        if ((M >> i) & 1) v += 1;
+      }
     }
     // score of the last row (in the last strip's last block)
     if (k == strips - 1) {
