@@ -140,7 +140,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
     const int b = k * kWave + lane;
     const int nbs = min(kWave, nb - k * kWave);  // blocks in this strip
     const bool active_lane = lane < nbs;
-    Peq peq;
+    Peq peq{0, 0, 0, 0, 0};
     if (active_lane) {
       peq = build_peq(g, b);
     }
@@ -162,15 +162,53 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
     // through the step loop, which would make every step wait for memory)
     int t_next = t_at(lane);
     int c_next = carry_at(lane);
+    const bool carry_lane = lane == nbs - 1 && k + 1 < strips;
+    bool prev_acgt = true;  // (no columns before the first chunk)
     int fwd = 1 << 8;  // (byte, hout + 1) of this lane's last step
     for (int st0 = 0; st0 < steps; st0 += kWave) {
      const int t_cur = t_next, c_cur = c_next;
      // this chunk's bytes and carries (loaded a chunk ago) are waited for
      // here, once, before the next chunk's loads go out
      asm volatile("" ::"v"(t_cur), "v"(c_cur));
+     // lanes work on columns st0 - 63 .. st0 + 63 in this chunk: when those
+     // are all A/C/G/T and every active lane's column is inside the matrix
+     // on every step, the steps run without the range and byte checks
+     const bool mine_acgt = st0 + lane >= g.tl || is_acgt(t_cur, code2(t_cur));
+     const bool chunk_acgt = __all(mine_acgt);
+     const int st_end = min(steps, st0 + kWave);
+     const bool fast = chunk_acgt && prev_acgt && st0 >= nbs - 1 && st_end <= g.tl;
+     prev_acgt = chunk_acgt;
      t_next = t_at(st0 + kWave + lane);
      c_next = carry_at(st0 + kWave + lane);
-     const int st_end = min(steps, st0 + kWave);
+     if (fast) {
+      for (int st = st0; st < st_end; ++st) {
+       const int c = st - lane;
+       const int up = __builtin_amdgcn_update_dpp(0, fwd, 0x138, 0xf, 0xf, false);
+       const int t0 = __builtin_amdgcn_readlane(t_cur, st & (kWave - 1));
+       const int h0 = __builtin_amdgcn_readlane(c_cur, st & (kWave - 1));
+       const uint8_t tc = static_cast<uint8_t>(lane == 0 ? t0 : (up & 0xff));
+       const int hin = lane == 0 ? h0 : (up >> 8) - 1;
+       // (lanes past the strip compute unused values; they store nothing and
+       // feed only lanes past the strip)
+       const int code = code2(tc);
+       const Word mlo = (code & 1) ? ~Word(0) : Word(0);
+       const Word mhi = (code & 2) ? ~Word(0) : Word(0);
+       const Word eq = (peq.acgt & ~((peq.lo ^ mlo) | (peq.hi ^ mhi))) | peq.pad;
+       const int hout = block_step(P, M, eq, hin);
+       score += hout;
+       if (store && active_lane) {
+         const uint64_t idx = store_base + static_cast<uint64_t>(st) * nbs + lane;
+         s.store_p[idx] = P;
+         s.store_m[idx] = M;
+         s.store_s[idx] = score;
+       }
+       if (carry_lane) {
+         s.carry[c] = static_cast<int8_t>(hout);
+       }
+       fwd = static_cast<int>(tc) | ((hout + 1) << 8);
+      }
+      continue;
+     }
      for (int st = st0; st < st_end; ++st) {
       const int c = st - lane;
       const int up = __builtin_amdgcn_update_dpp(0, fwd, 0x138, 0xf, 0xf, false);
@@ -198,7 +236,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
           s.store_m[idx] = M;
           s.store_s[idx] = score;
         }
-        if (lane == nbs - 1 && k + 1 < strips) {
+        if (carry_lane) {
           s.carry[c] = static_cast<int8_t>(hout);
         }
       }
@@ -447,7 +485,8 @@ __device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResul
 
 }  // namespace
 
-__global__ __launch_bounds__(64) void align_batch_kernel(AlignParams A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void align_batch_kernel(
+    AlignParams A) {
   __shared__ uint32_t next;
   while (true) {
     if (threadIdx.x == 0) {
