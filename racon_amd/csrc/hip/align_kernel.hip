@@ -269,57 +269,85 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
   return last_score;
 }
 
-// D[r][c] of the stored leaf DP (r, c >= 0).
-__device__ int stored_value(const Seg& g, const SlotA& s, int r, int c) {
-  const int nb = (g.ql + 63) / 64;
-  const int b = r / 64;
-  const int k = b / kWave;
-  const int lane = b % kWave;
-  const int nbs = min(kWave, nb - k * kWave);
-  const uint64_t idx = static_cast<uint64_t>(k) * (g.tl + 63) * 64 +
-                       static_cast<uint64_t>(c + lane) * nbs + lane;
-  const Word P = s.store_p[idx];
-  const Word M = s.store_m[idx];
-  int v = s.store_s[idx];
-  const int bit = r % 64;
-  if (bit < 63) {
-    const Word above = ~Word(0) << (bit + 1);
-    v -= __popcll(P & above);
-    v += __popcll(M & above);
-  }
-  return v;
-}
-
-// Boundary-aware D value: row -1 is c + 1, column -1 is r + 1.
-__device__ int dval(const Seg& g, const SlotA& s, int r, int c) {
-  if (r < 0) return c + 1;
-  if (c < 0) return r + 1;
-  return stored_value(g, s, r, c);
+// Lane l's 64-bit value, as a uniform value.
+__device__ __forceinline__ Word readlane64(Word x, int l) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), l));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x >> 32), l));
+  return (static_cast<Word>(hi) << 32) | lo;
 }
 
 // Traceback of a stored leaf; writes ops (forward order) at out, returns count.
 __device__ int leaf_traceback(const Seg& g, const SlotA& s, uint8_t* out) {
+  // The walk is uniform (every lane follows it); the stored words it reads
+  // come from a window loaded by all lanes in one round trip: lane l holds
+  // column cw - l of block bw (the block of row r) and of block bw - 1, so
+  // the walk reads them with readlane until it leaves the window (64
+  // columns, the two blocks), then reloads. The values and the move order
+  // are those of reading every cell's stored words directly.
+  const int lane = lane_id();
+  const int nb = (g.ql + 63) / 64;
+  auto index_of = [&](int b, int c) -> uint64_t {
+    const int k = b / kWave;
+    const int lb = b % kWave;
+    const int nbs = min(kWave, nb - k * kWave);
+    return static_cast<uint64_t>(k) * (g.tl + 63) * 64 +
+           static_cast<uint64_t>(c + lb) * nbs + lb;
+  };
   int r = g.ql - 1, c = g.tl - 1;
   int n = 0;
   uint8_t* tmp = s.ops_tmp;
   while (r >= 0 && c >= 0) {
-    const int cur = dval(g, s, r, c);
-    const int up = dval(g, s, r - 1, c);
-    if (up + 1 == cur) {
-      tmp[n++] = 1;  // insertion
+    const int bw = uniform(r / 64), cw = uniform(c);
+    const int col = cw - lane;
+    Word pb = 0, mb = 0, pa = 0, ma = 0;
+    int sb = 0, sa = 0;
+    if (col >= 0) {
+      const uint64_t ib = index_of(bw, col);
+      pb = s.store_p[ib];
+      mb = s.store_m[ib];
+      sb = s.store_s[ib];
+      if (bw >= 1) {
+        const uint64_t ia = index_of(bw - 1, col);
+        pa = s.store_p[ia];
+        ma = s.store_m[ia];
+        sa = s.store_s[ia];
+      }
+    }
+    auto val = [&](int rr, int cc) -> int {
+      if (rr < 0) return cc + 1;
+      if (cc < 0) return rr + 1;
+      const int l = cw - cc;
+      const bool above = rr / 64 < bw;
+      const Word P = readlane64(above ? pa : pb, l);
+      const Word M = readlane64(above ? ma : mb, l);
+      int v = above ? __builtin_amdgcn_readlane(sa, l) : __builtin_amdgcn_readlane(sb, l);
+      const int bit = rr % 64;
+      if (bit < 63) {
+        const Word mask = ~Word(0) << (bit + 1);
+        v -= __popcll(P & mask);
+        v += __popcll(M & mask);
+      }
+      return v;
+    };
+    while (r >= 0 && c >= 0 && r / 64 >= bw && c >= cw - 62) {
+      const int cur = val(r, c);
+      const int up = val(r - 1, c);
+      if (up + 1 == cur) {
+        tmp[n++] = 1;  // insertion
+        --r;
+        continue;
+      }
+      const int left = val(r, c - 1);
+      if (left + 1 == cur) {
+        tmp[n++] = 2;  // deletion
+        --c;
+        continue;
+      }
+      const int diag = val(r - 1, c - 1);
+      tmp[n++] = diag == cur ? 0 : 3;
       --r;
-      continue;
-    }
-    const int left = dval(g, s, r, c - 1);
-    if (left + 1 == cur) {
-      tmp[n++] = 2;  // deletion
       --c;
-      continue;
     }
-    const int diag = dval(g, s, r - 1, c - 1);
-    tmp[n++] = diag == cur ? 0 : 3;
-    --r;
-    --c;
   }
   while (c >= 0) {
     tmp[n++] = 2;
