@@ -4,10 +4,15 @@
 Workload (cfg2): one seeded random 5 Mbp contig with 50x
 ONT-like reads (lognormal 8 +- 4 kbp, 12% errors at 0.4:0.2:0.4), PAF from
 true positions, window 500, scores 3/-5/-4. The inputs are generated on the
-box (racon_amd/bin/simgen, seed 1002), the product CLI aligns the overlaps on
-the GPU and writes the windows, and the windows are uploaded once: a step is
-one pass of the persistent POA kernel over every window, inputs resident in
-HBM. Printed: one JSON line (see the keys below).
+box (racon_amd/bin/simgen, seed 1002), and the product CLI aligns the
+overlaps on the GPU and writes the windows. A step is racon's polish phase
+over every window through the C ABI: rh_poa_upload (host pack + H2D),
+rh_poa_run (the persistent POA kernel) and rh_poa_download (D2H); the
+kernel-only time (HIP events on the batch's stream) is reported beside it.
+Also reported: the static-grid comparator (one workgroup per window, one
+launch), the plain mode (one launch per window), the overlap aligner, and
+the CPU baseline (racon's CPU path restated, 1 and T threads). Printed: one
+JSON line (see the keys below).
 """
 from __future__ import annotations
 
@@ -25,7 +30,6 @@ METRIC = "polished windows/s + POA GCUPS, batched vs plain, 1 MI355X, bit-exact"
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 39.3  # 256 CUs x 64 lanes x 2.4 GHz int32
 POA_OPS_PER_CELL = 6    # linear gap: diag add, vertical add, 2 max, horizontal add + max
-POA_BYTES_PER_CELL = 2  # one 16-bit traceback record written per DP cell
 
 CONFIGS = {
     1: dict(name="cfg1: 50 kbp contig, 30x ONT 12%, w500", seed=1001,
@@ -195,23 +199,99 @@ def serving_bench(windows, args, scores) -> dict:
     return report
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(windows, gpu_out, gpu_status, threads, per_thread, reps, scores):
+    """racon's CPU path, restated (the oracle's scalar C++ POA, one window per
+    task as Polisher::polish does, src/polisher.cpp:486-504), timed on this
+    host with 1 and `threads` worker threads over the first windows of the
+    same workload; median of `reps` runs each, every sampled window checked
+    byte-identical to the GPU output."""
+    import concurrent.futures
+    import statistics
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    m, x, g = scores
+
+    def one(i):
+        w = windows[i]
+        cons, status, _, _ = oracle_py.window_consensus(
+            w.backbone, w.backbone_quality,
+            [(l.sequence, l.quality, l.begin, l.end) for l in w.layers], w.tgs, True, m, x, g)
+        if cons != gpu_out[i] or status != gpu_status[i]:
+            raise RuntimeError(f"window {i}: GPU output differs from the oracle")
+        return i
+
+    legs = {}
+    for t in sorted({1, threads}):
+        n = min(len(windows), per_thread * t)
+        times = []
+        for r in range(reps):
+            t0 = time.perf_counter()
+            with concurrent.futures.ThreadPoolExecutor(t) as ex:
+                list(ex.map(one, range(n)))
+            times.append(time.perf_counter() - t0)
+            log(f"cpu -t {t}: {n} windows in {times[-1]:.2f} s (run {r + 1}/{reps})")
+        med = statistics.median(times)
+        legs[t] = {"windows": n, "seconds": [round(s, 3) for s in times],
+                   "median_s": round(med, 3), "windows_per_s": round(n / med, 3)}
+    top = legs[threads]
+    return {"value": top["windows_per_s"], "unit": "windows/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"first {top['windows']} of {len(windows)} windows of the same workload "
+                       f"(-t {threads}) and first {legs[1]['windows']} (-t 1); racon CPU path, "
+                       f"restated: the oracle's scalar C++ POA (upstream spoa would use SIMD), "
+                       f"one window per task, median of {reps} runs; every sampled window "
+                       f"byte-identical to the GPU output"),
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "t1": legs[1], f"t{threads}": top}
+
+
+def profile_traffic(path):
+    """HBM bytes per POA launch from the committed rocprofv3 summary (line
+    'traffic_bytes_per_launch: N'), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--workdir", default="/tmp/racon_amd_bench")
     ap.add_argument("--plain-windows", type=int, default=48,
                     help="windows timed in the one-launch-per-window mode")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="budget of the single-core CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="worker threads of the CPU baseline (0: the host's CPUs, at most 16)")
+    ap.add_argument("--cpu-windows-per-thread", type=int, default=240,
+                    help="CPU baseline sample: windows per worker thread")
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--traffic-bytes", type=float, default=0.0,
-                    help="HBM bytes per POA launch measured with rocprofv3 (FETCH_SIZE x2 + "
-                         "WRITE_SIZE, copy-calibrated) to report in roofline.traffic")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="profiling runs: time only rh_poa_run (inputs resident), skip the "
+                         "comparators and the CPU baseline")
+    ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "poa_cfg2_counters.txt"),
+                    help="committed rocprofv3 summary with the POA kernel's HBM bytes per launch")
     ap.add_argument("--serving", default="", metavar="RATES",
                     help="only run the serving loop: Poisson request arrivals at these rates "
                          "(requests/s, comma separated), requests cut from the config's windows")
@@ -246,6 +326,7 @@ def main() -> int:
         return 0
 
     import racon_amd
+    from racon_amd._lib import pack_windows
     from racon_amd.dumps import read_overlaps, read_windows
     lib = racon_amd.load_library()
     if lib.rh_device_count() < 1:
@@ -261,112 +342,139 @@ def main() -> int:
         print(json.dumps(serving_bench(windows, args, (m, x, g))), flush=True)
         return 0
 
-    # ---- batched persistent kernel, windows resident in HBM ----
+    # the caller's rh_window array (racon's Window objects), built once
+    n_windows = len(windows)
+    arr, keep = pack_windows(windows)
+
+    # ---- autotune the LDS ring depth on the resident batch ----
     batch = racon_amd.PoaBatch(m, x, g, trim=True)
-    batch.upload(windows)
-    up = batch.stats()
-    log(f"uploaded {up.input_bytes / 1e6:.1f} MB (pack {up.pack_ms:.0f} ms, "
-        f"H2D {up.h2d_ms:.0f} ms)")
-    # autotune the LDS ring depth (DP rows kept on chip per wave) on the
-    # resident batch; 0 = the occupancy-derived default
+    batch.upload_packed(arr, n_windows, keep)
     tuned = {}
+    best_ring = 0
     if not args.no_tune:
-        # one untimed launch first, so the first candidate is not charged
-        # with the first launch's one-time costs
-        batch.run(racon_amd.RH_POA_BATCHED)
+        batch.run(racon_amd.RH_POA_BATCHED)  # first launch: one-time costs
         for ring in (0, 4, 8):
             batch.set_tuning(ring_rows=ring)
-            t0 = time.perf_counter()
             batch.run(racon_amd.RH_POA_BATCHED)
-            tuned[ring] = (time.perf_counter() - t0) * 1e3
-            log(f"tune ring_rows={ring or 'auto'}: {tuned[ring]:.1f} ms")
+            tuned[ring] = batch.stats().kernel_ms
+            log(f"tune ring_rows={ring or 'auto'}: kernel {tuned[ring]:.1f} ms")
         best_ring = min(tuned, key=tuned.get)
         batch.set_tuning(ring_rows=best_ring)
-    else:
-        best_ring = 0
-    for i in range(args.warmup):
+
+    # ---- the polish phase: pack + H2D + kernels + D2H per step ----
+    bufs = batch.output_buffers(2 * batch.lib.rh_poa_output_bound(batch.handle))
+
+    def step():
+        batch.upload_packed(arr, n_windows, keep)
         batch.run(racon_amd.RH_POA_BATCHED)
+        batch.download_into(bufs)
+
+    for i in range(args.warmup):
+        step()
         log(f"warmup {i + 1}/{args.warmup} done")
-    step_ms, kernel_ms = [], []
+    step_ms, parts = [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        batch.run(racon_amd.RH_POA_BATCHED)
+        if args.kernel_only:
+            batch.run(racon_amd.RH_POA_BATCHED)
+        else:
+            step()
         step_ms.append((time.perf_counter() - t0) * 1e3)
         st = batch.stats()
-        kernel_ms.append(st.kernel_ms)
-        log(f"step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms (kernel {st.kernel_ms:.1f} ms, "
-            f"slots {st.slots}, retried {st.retried})")
+        parts.append((st.pack_ms, st.h2d_ms, st.kernel_ms, st.d2h_ms))
+        log(f"step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms (pack {st.pack_ms:.1f}, H2D "
+            f"{st.h2d_ms:.1f}, kernel {st.kernel_ms:.1f}, D2H {st.d2h_ms:.1f} ms; slots "
+            f"{st.slots}, retried {st.retried})")
     st = batch.stats()
     phases = batch.phase_seconds()
     total_phase = sum(phases.values()) or 1.0
     log("phase share of wave time: " + ", ".join(
         f"{k} {100 * v / total_phase:.1f}%" for k, v in phases.items()))
-    # busy wave time over resident-slot time: below 1 when waves idle at the
-    # end of the queue (load imbalance) or between windows
     busy_frac = total_phase / (st.slots * (st.kernel_ms / 1e3)) if st.slots else 0.0
     log(f"wave busy fraction {busy_frac:.3f} ({st.slots} resident waves)")
     sched = queue_report(windows, batch, st.slots, st.kernel_ms)
-    gpu_out, gpu_status = batch.download()
-    n_windows = len(windows)
+    if args.kernel_only:
+        batch.download_into(bufs)
+    gpu_out, gpu_status = batch.split(bufs, n_windows)
     cells = st.cells
     ms_per_step = sum(step_ms) / len(step_ms)
-    kernel_avg = sum(kernel_ms) / len(kernel_ms)
+    kernel_avg = sum(p[2] for p in parts) / len(parts)
     windows_per_s = n_windows / (ms_per_step / 1e3)
+    kernel_wps = n_windows / (kernel_avg / 1e3)
     poa_gcups = cells / (kernel_avg / 1e3) / 1e9
+    input_bytes = st.input_bytes
     batch.close()
 
-    # ---- plain: one launch per window, same kernel body ----
-    subset = windows[:args.plain_windows]
-    plain = racon_amd.PoaBatch(m, x, g, trim=True)
-    plain.upload(subset)
-    plain.run(racon_amd.RH_POA_PLAIN)  # warm
-    t0 = time.perf_counter()
-    plain.run(racon_amd.RH_POA_PLAIN)
-    plain_s = time.perf_counter() - t0
-    plain_out, _ = plain.download()
-    plain.close()
-    plain_wps = len(subset) / plain_s
-    if plain_out != gpu_out[:len(subset)]:
-        raise RuntimeError("plain and batched outputs differ")
-    log(f"plain: {len(subset)} windows in {plain_s:.2f} s -> {plain_wps:.1f} windows/s")
+    extra = {}
+    if not args.kernel_only:
+        # ---- static grid: one workgroup per window, one launch, no queue ----
+        if not args.skip_static:
+            sb = racon_amd.PoaBatch(m, x, g, trim=True)
+            sb.upload_packed(arr, n_windows, keep)
+            sb.run(racon_amd.RH_POA_STATIC)  # sizes the slots
+            sms = []
+            for _ in range(2):
+                sb.run(racon_amd.RH_POA_STATIC)
+                sms.append(sb.stats().kernel_ms)
+            sst = sb.stats()
+            sout, _ = sb.download()
+            sb.close()
+            if sout != gpu_out:
+                raise RuntimeError("static-grid and batched outputs differ")
+            static_ms = min(sms)
+            extra["static_kernel_ms"] = round(static_ms, 3)
+            extra["static_launches"] = int(sst.launches)
+            extra["static_windows_per_s_kernel"] = round(n_windows / (static_ms / 1e3), 2)
+            extra["batched_over_static_kernel"] = round(static_ms / kernel_avg, 3)
+            log(f"static grid: kernel {static_ms:.1f} ms ({sst.launches} launches) vs batched "
+                f"{kernel_avg:.1f} ms")
 
-    # ---- overlap alignment kernel ----
-    ab = racon_amd.AlignBatch()
-    ab.upload(pairs)
-    ab.run()  # warm
-    ab.run()
-    ast = ab.stats()
-    ab.close()
-    align_gcups = ast.cells / (ast.kernel_ms / 1e3) / 1e9
-    log(f"align: {len(pairs)} overlaps, kernel {ast.kernel_ms:.1f} ms, "
-        f"{align_gcups:.1f} GCUPS (full-matrix equivalent)")
+        # ---- plain: one launch per window, same kernel body ----
+        subset = windows[:args.plain_windows]
+        plain = racon_amd.PoaBatch(m, x, g, trim=True)
+        plain.upload(subset)
+        plain.run(racon_amd.RH_POA_PLAIN)  # warm
+        t0 = time.perf_counter()
+        plain.run(racon_amd.RH_POA_PLAIN)
+        plain_s = time.perf_counter() - t0
+        plain_out, _ = plain.download()
+        plain.close()
+        plain_wps = len(subset) / plain_s
+        if plain_out != gpu_out[:len(subset)]:
+            raise RuntimeError("plain and batched outputs differ")
+        log(f"plain: {len(subset)} windows in {plain_s:.2f} s -> {plain_wps:.1f} windows/s")
+        extra["plain_windows_per_s"] = round(plain_wps, 3)
+        extra["batched_over_plain"] = round(windows_per_s / plain_wps, 2)
 
-    # ---- CPU baseline: the oracle, single core, bounded sample ----
+        # ---- overlap alignment kernel ----
+        ab = racon_amd.AlignBatch()
+        ab.upload(pairs)
+        ab.run()  # warm
+        ab.run()
+        ast = ab.stats()
+        ab.close()
+        align_gcups = ast.cells / (ast.kernel_ms / 1e3) / 1e9
+        log(f"align: {len(pairs)} overlaps, kernel {ast.kernel_ms:.1f} ms, "
+            f"{align_gcups:.1f} GCUPS (full-matrix equivalent)")
+        extra["align_kernel_ms"] = round(ast.kernel_ms, 3)
+        extra["align_gcups_full_matrix"] = round(align_gcups, 2)
+        extra["align_roofline_valu_frac"] = round(
+            ast.cells * 20 / 32 / (ast.kernel_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 5)
+
+    # ---- CPU baseline: racon's CPU path restated, 1 and T threads ----
     cpu = None
-    if not args.skip_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py
-        done, t0 = 0, time.perf_counter()
-        for w in windows:
-            cons, status, _, _ = oracle_py.window_consensus(
-                w.backbone, w.backbone_quality,
-                [(l.sequence, l.quality, l.begin, l.end) for l in w.layers], w.tgs, True,
-                m, x, g)
-            if cons != gpu_out[done] or status != gpu_status[done]:
-                raise RuntimeError(f"window {done}: GPU output differs from the oracle")
-            done += 1
-            if time.perf_counter() - t0 > args.cpu_seconds:
-                break
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(done / cpu_s, 3), "unit": "windows/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {done} windows of the same workload, oracle POA "
-                         f"(scalar C++ restatement of spoa), {cpu_s:.1f} s; every sampled "
-                         f"window byte-identical to the GPU output"}
-        log(f"cpu: {done} windows in {cpu_s:.1f} s")
+    if not args.skip_cpu and not args.kernel_only:
+        try:
+            avail = len(os.sched_getaffinity(0))
+        except AttributeError:
+            avail = os.cpu_count() or 1
+        threads = args.cpu_threads or max(1, min(16, avail))
+        cpu = cpu_baseline(windows, gpu_out, gpu_status, threads, args.cpu_windows_per_thread,
+                           args.cpu_reps, (m, x, g))
 
-    algo_bytes = cells * POA_BYTES_PER_CELL + st.input_bytes
-    achieved_gbs = algo_bytes / (kernel_avg / 1e3) / 1e9
+    valu_tops = cells * POA_OPS_PER_CELL / (kernel_avg / 1e3) / 1e12
+    traffic = profile_traffic(args.profile)
+    pack_ms, h2d_ms, _, d2h_ms = (sum(p[k] for p in parts) / len(parts) for k in range(4))
     result = {
         "metric": METRIC,
         "value": round(windows_per_s, 2),
@@ -382,30 +490,35 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": CONFIGS[args.config]["name"], "windows": n_windows,
                    "gpu_windows": int(st.gpu_windows), "overlaps": len(pairs),
-                   "window_length": 500, "scores": [m, x, g], "seed": CONFIGS[args.config]["seed"]},
+                   "window_length": 500, "scores": [m, x, g], "seed": CONFIGS[args.config]["seed"],
+                   "step": ("rh_poa_run only (inputs resident)" if args.kernel_only else
+                            "polish phase: rh_poa_upload (pack + H2D) + rh_poa_run (kernels) + "
+                            "rh_poa_download (D2H)")},
+        "step_ms_parts": {"pack": round(pack_ms, 3), "h2d": round(h2d_ms, 3),
+                          "kernel": round(kernel_avg, 3), "d2h": round(d2h_ms, 3)},
+        "kernel_ms": round(kernel_avg, 3),
+        "kernel_windows_per_s": round(kernel_wps, 2),
         "poa_gcups": round(poa_gcups, 3),
         "poa_cells": int(cells),
-        "kernel_ms": round(kernel_avg, 3),
-        "plain_windows_per_s": round(plain_wps, 3),
-        "batched_over_plain": round(windows_per_s / plain_wps, 2),
-        "align_gcups_full_matrix": round(align_gcups, 2),
-        "align_kernel_ms": round(ast.kernel_ms, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": args.traffic_bytes or None},
-        "roofline_valu": {"bound": "valu-int32",
-                          "achieved": round(cells * POA_OPS_PER_CELL / (kernel_avg / 1e3) / 1e12, 4),
-                          "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
-                          "frac": round(cells * POA_OPS_PER_CELL / (kernel_avg / 1e3) / 1e12
-                                        / VALU_PEAK_TOPS, 5)},
+        "input_bytes": int(input_bytes),
+        "roofline": {"bound": "valu", "achieved": round(valu_tops, 4), "peak": VALU_PEAK_TOPS,
+                     "unit": "TOP/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 5),
+                     "traffic": traffic,
+                     "ops_per_cell": POA_OPS_PER_CELL},
+        "roofline_hbm": ({"achieved": round(traffic / (kernel_avg / 1e3) / 1e9, 2),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(traffic / (kernel_avg / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+                          "traffic_source": os.path.relpath(args.profile, ROOT)}
+                         if traffic else None),
         "cpu_baseline": cpu,
         "poa_phase_share": {k: round(v / total_phase, 4) for k, v in phases.items()},
         "poa_wave_busy_frac": round(busy_frac, 4),
         "poa_queue": sched,
         "poa_resident_waves": st.slots,
-        "autotune_ms": {str(k): round(v, 1) for k, v in tuned.items()},
+        "autotune_kernel_ms": {str(k): round(v, 1) for k, v in tuned.items()},
         "ring_rows": best_ring,
     }
+    result.update(extra)
     print(json.dumps(result), flush=True)
     return 0
 

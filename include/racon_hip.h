@@ -81,7 +81,9 @@ enum {
 /* Kernel launch modes. */
 enum {
   RH_POA_BATCHED = 0,  /* persistent kernel over a size-ordered work queue */
-  RH_POA_PLAIN = 1     /* one kernel launch per window                     */
+  RH_POA_PLAIN = 1,    /* one kernel launch per window                     */
+  RH_POA_STATIC = 2    /* one workgroup per window in upload order, one
+                          launch, no queue (the comparison for BATCHED)     */
 };
 
 typedef struct rh_poa_batch rh_poa_batch;

@@ -21,6 +21,7 @@ RACON_BIN = os.path.join(PKG_DIR, "bin", "racon")
 RH_OK = 0
 RH_POA_BATCHED = 0
 RH_POA_PLAIN = 1
+RH_POA_STATIC = 2
 RH_WINDOW_UNPOLISHED = 0
 RH_WINDOW_POLISHED = 1
 RH_WINDOW_POLISHED_UNTRIMMED = 2
@@ -193,10 +194,25 @@ _probe = b"ACGT"
 _DIRECT = _address(_probe) == id(_probe) + _BYTES_DATA
 
 
+def _as_bytes(p):
+    """`p` as an immutable bytes object (None stays None). bytearray and
+    memoryview are copied; anything else (str, lists) is refused, since its
+    memory is not a byte string."""
+    if p is None or type(p) is bytes:
+        return p
+    if isinstance(p, (bytearray, memoryview)):
+        return bytes(p)
+    raise TypeError(f"sequence and quality parts must be bytes, not {type(p).__name__}")
+
+
 def _pointers(parts: Sequence[Optional[bytes]]):
     """Address of each part's bytes (0 for None) without copying them where
-    the interpreter allows; returns (addresses, lengths, objects to keep)."""
+    the interpreter allows; returns (addresses, lengths, objects to keep).
+    The id() shortcut is only taken for exact `bytes` objects: every other
+    part is converted by _as_bytes first."""
     import numpy as np
+    if not all(p is None or type(p) is bytes for p in parts):
+        parts = [_as_bytes(p) for p in parts]
     n = len(parts)
     present = np.fromiter(map(operator.is_not, parts, itertools.repeat(None)), dtype=bool,
                           count=n)
@@ -288,26 +304,44 @@ class PoaBatch:
 
     def upload(self, windows: Sequence[Window]) -> None:
         arr, keep = pack_windows(windows)
-        _check(self.lib.rh_poa_upload(self.handle, len(windows),
+        self.upload_packed(arr, len(windows), keep)
+
+    def upload_packed(self, arr, n: int, keep=None) -> None:
+        """rh_poa_upload of an rh_window array made by pack_windows (the
+        caller's window list, built once)."""
+        _check(self.lib.rh_poa_upload(self.handle, n,
                                       ctypes.cast(arr.ctypes.data, ctypes.POINTER(RhWindow))),
                "rh_poa_upload")
-        self._keep = keep
-        self.n = len(windows)
+        self._keep = (arr, keep)
+        self.n = n
 
     def run(self, mode: int = RH_POA_BATCHED) -> None:
         _check(self.lib.rh_poa_run(self.handle, mode), "rh_poa_run")
 
-    def download(self):
-        cap = self.lib.rh_poa_output_bound(self.handle)
-        out = ctypes.create_string_buffer(max(1, cap))
-        offsets = (ctypes.c_uint64 * max(1, self.n))()
-        lengths = (ctypes.c_uint32 * max(1, self.n))()
-        status = (ctypes.c_uint8 * max(1, self.n))()
+    def output_buffers(self, capacity: int = 0):
+        """Buffers for download_into: (out, capacity, offsets, lengths, status)."""
+        cap = max(1, capacity or self.lib.rh_poa_output_bound(self.handle))
+        return (ctypes.create_string_buffer(cap), cap, (ctypes.c_uint64 * max(1, self.n))(),
+                (ctypes.c_uint32 * max(1, self.n))(), (ctypes.c_uint8 * max(1, self.n))())
+
+    def download_into(self, bufs) -> None:
+        """rh_poa_download into buffers from output_buffers (no Python
+        objects per window)."""
+        out, cap, offsets, lengths, status = bufs
         _check(self.lib.rh_poa_download(self.handle, out, cap, offsets, lengths, status),
                "rh_poa_download")
+
+    @staticmethod
+    def split(bufs, n: int):
+        out, _, offsets, lengths, status = bufs
         raw = out.raw
-        return ([raw[offsets[i]:offsets[i] + lengths[i]] for i in range(self.n)],
-                [status[i] for i in range(self.n)])
+        return ([raw[offsets[i]:offsets[i] + lengths[i]] for i in range(n)],
+                [status[i] for i in range(n)])
+
+    def download(self):
+        bufs = self.output_buffers()
+        self.download_into(bufs)
+        return self.split(bufs, self.n)
 
     PHASES = ("include", "subsort", "rows", "column0", "dp", "traceback", "insert", "sort",
               "consensus", "setup")

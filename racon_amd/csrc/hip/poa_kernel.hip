@@ -2919,6 +2919,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES
   }
 }
 
+// Static-grid kernel: one workgroup per window, no queue and no cost order.
+// Block b takes window order[first + b] and scratch slot b; the hardware
+// dispatcher starts blocks in index order as CUs free up. Kept as the
+// comparison for the persistent kernel's queue.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_static_kernel(
+    const PoaParams* __restrict__ params, uint32_t first) {
+  CParams& P = *(CParams*)params;
+  extern __shared__ uint8_t lds[];
+  const uint32_t w = global(P.order)[first + blockIdx.x];
+  Slot s = carve(global(P.scratch) + static_cast<uint64_t>(blockIdx.x) * P.slot_bytes, P.layout);
+  __shared__ PoaResult res;
+  if (threadIdx.x == 0) {
+    res = PoaResult{};
+    res.status = kPoaNotRun;
+  }
+  __syncthreads();
+  process_window(P, s, lds, ((CWindow*)P.windows)[w], res);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.results[w] = res;
+  }
+}
+
 }  // namespace rh
 
 namespace rh {
@@ -2933,6 +2956,12 @@ hipError_t launch_poa_batch(const PoaParams* d_params, uint32_t lds_bytes, uint3
 hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
                              hipStream_t stream) {
   hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), lds_bytes, stream, d_params, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_poa_static(const PoaParams* d_params, uint32_t lds_bytes, uint32_t first,
+                             uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL(poa_static_kernel, dim3(count), dim3(64), lds_bytes, stream, d_params, first);
   return hipGetLastError();
 }
 

@@ -24,6 +24,8 @@ hipError_t launch_poa_batch(const PoaParams* d_params, uint32_t lds_bytes, uint3
                             hipStream_t stream);
 hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
                              hipStream_t stream);
+hipError_t launch_poa_static(const PoaParams* d_params, uint32_t lds_bytes, uint32_t first,
+                             uint32_t count, hipStream_t stream);
 hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes);
 
 namespace {
@@ -138,7 +140,9 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   // keep the slots within ~60% of free memory (plus what we already hold)
   const uint64_t budget = static_cast<uint64_t>((free_b + b->d_scratch.bytes) * 0.6);
   uint64_t slots = std::max<uint64_t>(1, budget / slot_bytes);
-  slots = std::min<uint64_t>(slots, max_slots);
+  if (mode != RH_POA_STATIC) {
+    slots = std::min<uint64_t>(slots, max_slots);
+  }
   slots = std::min<uint64_t>(slots, order.size());
   if (mode == RH_POA_PLAIN) {
     slots = 1;
@@ -182,6 +186,16 @@ int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaS
   if (mode == RH_POA_PLAIN) {
     for (uint32_t q = 0; q < order.size(); ++q) {
       RH_HIP_CHECK(rh::launch_poa_single(dp, p.lds_bytes, q, b->stream), RH_ERR_KERNEL);
+      b->stats.launches++;
+    }
+  } else if (mode == RH_POA_STATIC) {
+    // one workgroup (and slot) per window; launches of `slots` windows when
+    // the slots of all windows do not fit in memory at once
+    for (uint64_t first = 0; first < order.size(); first += slots) {
+      const uint64_t count = std::min<uint64_t>(slots, order.size() - first);
+      RH_HIP_CHECK(rh::launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
+                                         static_cast<uint32_t>(count), b->stream),
+                   RH_ERR_KERNEL);
       b->stats.launches++;
     }
   } else {
@@ -441,7 +455,7 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
     rh::set_error("rh_poa_run: nothing uploaded");
     return RH_ERR_STATE;
   }
-  if (mode != RH_POA_BATCHED && mode != RH_POA_PLAIN) {
+  if (mode != RH_POA_BATCHED && mode != RH_POA_PLAIN && mode != RH_POA_STATIC) {
     rh::set_error("rh_poa_run: unknown mode");
     return RH_ERR_ARGUMENT;
   }
@@ -489,7 +503,13 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   }
 
   double kernel_ms = 0;
-  int32_t rc = run_pass(b, b->gpu_order, caps, resident, mode, &kernel_ms);
+  std::vector<uint32_t> upload_order;
+  if (mode == RH_POA_STATIC) {
+    upload_order = b->gpu_order;
+    std::sort(upload_order.begin(), upload_order.end());
+  }
+  int32_t rc = run_pass(b, mode == RH_POA_STATIC ? upload_order : b->gpu_order, caps, resident,
+                        mode, &kernel_ms);
   if (rc != RH_OK) {
     return rc;
   }

@@ -60,3 +60,34 @@ def test_unequal_and_other_bytes():
     q = mutate(rng, t, 0.15)
     q2 = q[:700] + b"NNNN" + q[700:]
     check([(q[:900], t), (q2, t), (q2.replace(b"A", b"R", 3), t)])
+
+
+def _plant(seq: bytes, at, run: bytes) -> bytes:
+    """`seq` with `run` written over it at each position of `at` (clipped)."""
+    b = bytearray(seq)
+    for p in at:
+        for k, c in enumerate(run):
+            if 0 <= p + k < len(b):
+                b[p + k] = c
+    return bytes(b)
+
+
+@pytest.mark.parametrize("length", [600, 3000, 9000])
+def test_target_side_other_bytes(length):
+    # N runs and IUPAC codes in the target (contigs with gaps), at 64-column
+    # chunk boundaries and in the first and last chunk; lengths cover leaf
+    # tracebacks, the Hirschberg split and multi-strip (> 4096-row) queries
+    rng = np.random.default_rng(100 + length)
+    t = random_seq(rng, length)
+    q = mutate(rng, t, 0.12)
+    edges = [0, 1, 62, 63, 64, 65, 126, 127, 128, 129, length // 2, length - 65, length - 64,
+             length - 2, length - 1]
+    pairs = [
+        (q, _plant(t, edges, b"N")),
+        (q, _plant(t, edges[::2], b"NNNNN")),
+        (q, _plant(t, edges, b"R")),
+        (_plant(q, edges[1::3], b"Y"), _plant(t, edges, b"RYKMSWN")),
+        (_plant(q, [length // 3], b"N" * 40), _plant(t, [length // 3 + 5], b"N" * 40)),
+        (q, t.lower() if length == 600 else _plant(t, [63, 127], b"acgt")),
+    ]
+    check(pairs)

@@ -91,6 +91,15 @@ def test_plain_matches_batched():
     assert out_b == out_p and st_b == st_p
 
 
+def test_static_grid_matches_oracle():
+    # one workgroup per window in upload order (the comparator of the
+    # persistent kernel), shallow windows mixed in
+    wins = make_windows(25, 40, length=500, depth=20)
+    wins[3:3] = make_windows(26, 2, length=200, depth=1)
+    st = check(wins, (3, -5, -4), mode=_lib.RH_POA_STATIC)
+    assert st.launches >= 1
+
+
 def test_long_layers_two_tiles():
     # layers longer than one 512-column tile
     wins = make_windows(20, 3, length=1000, depth=12)

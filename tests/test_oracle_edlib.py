@@ -93,3 +93,23 @@ def test_banded_equals_exact_unequal_lengths():
     t = random_seq(rng, 2500)
     for q in (mutate(rng, t[:1800], 0.1), mutate(rng, t, 0.25), t[300:] + t[:50]):
         assert oracle_py.edlib_align(q, t) == oracle_py.edlib_align(q, t, exact=True)
+
+
+def test_other_bytes_match_brute_force():
+    # N runs, IUPAC codes and lower case on both sides: edlib compares bytes
+    # as symbols of the pair's own alphabet, so the distance is Levenshtein's
+    rng = np.random.default_rng(103)
+    extra = np.frombuffer(b"NRYKMSWacgt", dtype=np.uint8)
+    for _ in range(120):
+        n = int(rng.integers(1, 45))
+        t = bytearray(random_seq(rng, n))
+        q = bytearray(mutate(rng, bytes(t), float(rng.uniform(0, 0.3))))
+        for s in (q, t):
+            for _ in range(int(rng.integers(0, 4))):
+                s[int(rng.integers(0, len(s)))] = int(extra[rng.integers(0, len(extra))])
+        q, t = bytes(q), bytes(t)
+        d, ops, st = oracle_py.edlib_align(q, t)
+        assert st == 0
+        assert d == levenshtein(q, t)
+        check_path(q, t, d, ops)
+        assert (d, ops) == oracle_py.edlib_align(q, t, exact=True)[:2]

@@ -50,3 +50,25 @@ def test_pack_joined_buffers(monkeypatch):
     wins += make_windows(35, 2, length=70, depth=3, quality=False)
     arr, keep = pack_windows(wins)
     assert read_back(arr, len(wins)) == wins
+
+
+def test_pack_bytearray_and_memoryview_parts():
+    # non-bytes parts are converted before their addresses are taken: the
+    # library must read the sequence, not the object header
+    seq = b"ACGTACGTAC"
+    qual = b"++++++++++"
+    w = Window(bytearray(b"ACGTACGTACGT"), memoryview(b"!!!!!!!!!!!!"),
+               [Layer(bytearray(seq), memoryview(qual), 0, 9),
+                Layer(memoryview(seq), bytearray(qual), 1, 10)], True)
+    arr, keep = pack_windows([w])
+    back = read_back(arr, 1)[0]
+    assert back.backbone == b"ACGTACGTACGT"
+    assert back.backbone_quality == b"!!!!!!!!!!!!"
+    assert [(l.sequence, l.quality) for l in back.layers] == [(seq, qual), (seq, qual)]
+
+
+def test_pack_refuses_str_parts():
+    import pytest
+    w = Window(b"ACGT", b"!!!!", [Layer("ACGT", None, 0, 3)], True)
+    with pytest.raises(TypeError):
+        pack_windows([w])
