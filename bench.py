@@ -458,6 +458,10 @@ def main() -> int:
             f"{align_gcups:.1f} GCUPS (full-matrix equivalent)")
         extra["align_kernel_ms"] = round(ast.kernel_ms, 3)
         extra["align_gcups_full_matrix"] = round(align_gcups, 2)
+        extra["align_cells_full_matrix"] = int(ast.cells)
+        # cells the banded passes computed: at most 64 blocks x 64 rows per wave step
+        extra["align_wave_steps"] = int(ast.steps)
+        extra["align_cells_computed_max"] = int(ast.steps) * 4096
         extra["align_roofline_valu_frac"] = round(
             ast.cells * 20 / 32 / (ast.kernel_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 5)
 

@@ -75,7 +75,14 @@ typedef struct {
 enum {
   RH_WINDOW_UNPOLISHED = 0,         /* < 3 sequences: consensus = backbone */
   RH_WINDOW_POLISHED = 1,
-  RH_WINDOW_POLISHED_UNTRIMMED = 2  /* racon's "might be chimeric" case   */
+  RH_WINDOW_POLISHED_UNTRIMMED = 2, /* racon's "might be chimeric" case   */
+  RH_WINDOW_REJECTED = 3            /* outside the device limits (see
+                                       rh_poa_run): consensus = backbone;
+                                       the caller runs its CPU path for it,
+                                       as racon's CUDA path does for windows
+                                       its batch refuses
+                                       (src/cuda/cudabatch.cpp:205-213,
+                                       src/cuda/cudapolisher.cpp:354-381) */
 };
 
 /* Kernel launch modes. */
@@ -100,6 +107,7 @@ typedef struct {
   uint64_t input_bytes; /* packed input bytes                           */
   uint32_t slots;       /* resident waves (slots) of the last launch    */
   uint32_t launches;    /* kernel launches of the last run              */
+  uint64_t rejected;    /* windows left to the caller (RH_WINDOW_REJECTED) */
 } rh_poa_stats;
 
 /* Creates a batch on `device` with racon's scores (match > 0, mismatch and
@@ -115,8 +123,34 @@ void rh_poa_destroy(rh_poa_batch* batch);
 int32_t rh_poa_upload(rh_poa_batch* batch, uint32_t n, const rh_window* windows);
 
 /* Runs the consensus of every uploaded window (blocking). Windows whose
- * graph outgrows the default scratch slot are rerun in larger slots. */
+ * graph outgrows the default scratch slot are rerun in larger slots; windows
+ * with layers longer than 3,072 bases run in a pass of their own (wider DP
+ * rows, fewer of them in LDS). A window outside the device limits does not
+ * fail the batch: it ends as RH_WINDOW_REJECTED (rh_poa_stats.rejected
+ * counts them). The limits: a graph bound (backbone + all layer bytes) of
+ * 2^24 - 1 nodes; a layer whose bytes alone exceed the LDS of one workgroup
+ * (160 KiB on gfx950); a graph node with more than 8,192 in-edges; a
+ * column of more than 9 distinct bytes (aligned sets hold 8 entries per
+ * node). */
 int32_t rh_poa_run(rh_poa_batch* batch, int32_t mode);
+
+/* rh_poa_run in two halves, for callers that keep several batches in flight
+ * (racon's CUDA path runs one batch object per host thread,
+ * src/cuda/cudapolisher.cpp:336-350; a serving loop overlaps one batch's
+ * upload with another's kernels): rh_poa_submit queues the kernels on the
+ * batch's own stream and returns; rh_poa_wait blocks until they are done,
+ * reruns the windows that need larger slots and leaves the batch ready for
+ * rh_poa_download. submit + wait == run. A submitted batch refuses another
+ * upload or submit until its wait. */
+int32_t rh_poa_submit(rh_poa_batch* batch, int32_t mode);
+int32_t rh_poa_wait(rh_poa_batch* batch);
+
+/* Free and total device memory of `device` (admission control: racon sizes
+ * its batches from 90% of the free memory, src/cuda/cudapolisher.cpp:165-180),
+ * and the device bytes a batch object holds (inputs, outputs and scratch
+ * slots of its last run). No reference counterpart for the latter. */
+int32_t rh_poa_capacity(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes);
+uint64_t rh_poa_device_bytes(const rh_poa_batch* batch);
 
 /* Upper bound of the total consensus bytes of the uploaded batch. */
 uint64_t rh_poa_output_bound(const rh_poa_batch* batch);
@@ -172,12 +206,32 @@ typedef struct {
   uint64_t cells;       /* sum |q| * |t| (full-matrix equivalent) */
   uint64_t ops;         /* total alignment operations             */
   uint32_t launches;
+  uint64_t steps;       /* wave steps taken (one column of up to 64
+                           64-row blocks each): cells computed in the
+                           bands <= steps x 4096                   */
+  uint64_t rejected;    /* pairs left to the caller's CPU path    */
 } rh_align_stats;
 
 int32_t rh_align_create(int32_t device, rh_align_batch** batch);
 void rh_align_destroy(rh_align_batch* batch);
 int32_t rh_align_upload(rh_align_batch* batch, uint32_t n, const rh_pair* pairs);
+/* Runs every uploaded pair (blocking). Each pair's DP is computed inside a
+ * diagonal band: a whole pair starts from a distance guess (its length
+ * difference plus a fifth of its length, or the band set here) that doubles
+ * until the banded distance is within it, which makes it exact; every
+ * subproblem of the path recursion uses the band its known score allows.
+ * The path is the one of the unbanded computation. A pair the device cannot
+ * align does not fail the batch: rh_align_download reports it with edit
+ * distance -1 and no ops, and rh_align_breaking_points with count
+ * RH_PAIR_REJECTED (racon's CUDA path aligns such overlaps with edlib on the
+ * CPU, src/cuda/cudaaligner.cpp:67-71, src/cuda/cudapolisher.cpp:199-211). */
 int32_t rh_align_run(rh_align_batch* batch);
+/* First band guess (a distance bound) for every pair; 0 = automatic. racon's
+ * --cudaaligner-band-width (src/main.cpp:133-135, default 10% of the mean
+ * overlap length, src/cuda/cudapolisher.cpp:151-163) maps here; unlike
+ * GenomeWorks' banded aligner the result never depends on it. */
+int32_t rh_align_set_band(rh_align_batch* batch, uint32_t band);
+#define RH_PAIR_REJECTED 0xfffffffeu
 /* Bound on the total number of ops (sum |q| + |t|). */
 uint64_t rh_align_output_bound(const rh_align_batch* batch);
 /* Edit operations of pair i at ops + offsets[i], n_ops[i] of them, encoded

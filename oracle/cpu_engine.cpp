@@ -55,7 +55,8 @@ class CpuEngine : public PolishEngine {
   }
 
   void generate_consensus(std::vector<std::shared_ptr<Window>>& windows, bool trim,
-                          std::vector<uint8_t>& status) override {
+                          std::vector<uint8_t>& status,
+                          const std::function<void(uint64_t)>& ready) override {
     status.assign(windows.size(), kWindowUnpolished);
     std::vector<std::future<void>> futures;
     futures.reserve(windows.size());
@@ -88,9 +89,12 @@ class CpuEngine : public PolishEngine {
           },
           i));
     }
-    for (const auto& f : futures) {
-      f.wait();
+    // windows in order, reported every 256
+    for (uint64_t i = 0; i < futures.size(); ++i) {
+      futures[i].wait();
+      if (ready && ((i + 1) % 256 == 0 || i + 1 == futures.size())) ready(i + 1);
     }
+    if (ready && futures.empty()) ready(0);
   }
 
  private:

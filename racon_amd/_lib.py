@@ -25,6 +25,7 @@ RH_POA_STATIC = 2
 RH_WINDOW_UNPOLISHED = 0
 RH_WINDOW_POLISHED = 1
 RH_WINDOW_POLISHED_UNTRIMMED = 2
+RH_WINDOW_REJECTED = 3
 
 
 class RhLayer(ctypes.Structure):
@@ -61,6 +62,7 @@ class RhPoaStats(ctypes.Structure):
         ("input_bytes", ctypes.c_uint64),
         ("slots", ctypes.c_uint32),
         ("launches", ctypes.c_uint32),
+        ("rejected", ctypes.c_uint64),
     ]
 
 
@@ -91,6 +93,8 @@ class RhAlignStats(ctypes.Structure):
         ("cells", ctypes.c_uint64),
         ("ops", ctypes.c_uint64),
         ("launches", ctypes.c_uint32),
+        ("steps", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
     ]
 
 
@@ -106,6 +110,11 @@ _SIGNATURES = {
     "rh_poa_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(RhWindow)]),
     "rh_poa_run": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "rh_poa_submit": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "rh_poa_wait": (ctypes.c_int32, [ctypes.c_void_p]),
+    "rh_poa_capacity": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "rh_poa_device_bytes": (ctypes.c_uint64, [ctypes.c_void_p]),
     "rh_poa_output_bound": (ctypes.c_uint64, [ctypes.c_void_p]),
     "rh_poa_download": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.c_uint64),
@@ -123,6 +132,7 @@ _SIGNATURES = {
     "rh_align_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
                                          ctypes.POINTER(RhPair)]),
     "rh_align_run": (ctypes.c_int32, [ctypes.c_void_p]),
+    "rh_align_set_band": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
     "rh_align_output_bound": (ctypes.c_uint64, [ctypes.c_void_p]),
     "rh_align_download": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64,
                                            ctypes.POINTER(ctypes.c_uint64),
@@ -318,6 +328,17 @@ class PoaBatch:
     def run(self, mode: int = RH_POA_BATCHED) -> None:
         _check(self.lib.rh_poa_run(self.handle, mode), "rh_poa_run")
 
+    def submit(self, mode: int = RH_POA_BATCHED) -> None:
+        """Queues the kernels and returns (rh_poa_submit)."""
+        _check(self.lib.rh_poa_submit(self.handle, mode), "rh_poa_submit")
+
+    def wait(self) -> None:
+        """Blocks until a submitted run is done (rh_poa_wait)."""
+        _check(self.lib.rh_poa_wait(self.handle), "rh_poa_wait")
+
+    def device_bytes(self) -> int:
+        return int(self.lib.rh_poa_device_bytes(self.handle))
+
     def output_buffers(self, capacity: int = 0):
         """Buffers for download_into: (out, capacity, offsets, lengths, status)."""
         cap = max(1, capacity or self.lib.rh_poa_output_bound(self.handle))
@@ -410,6 +431,10 @@ class AlignBatch:
 
     def run(self) -> None:
         _check(self.lib.rh_align_run(self.handle), "rh_align_run")
+
+    def set_band(self, band: int) -> None:
+        """First band guess of every pair (0: automatic)."""
+        _check(self.lib.rh_align_set_band(self.handle, band), "rh_align_set_band")
 
     def download(self):
         cap = self.lib.rh_align_output_bound(self.handle)

@@ -125,28 +125,109 @@ struct SlotA {
   uint8_t* ops_tmp;
 };
 
-// Runs the bit-vector DP of seg over all columns. With stop >= 0 (which must
-// be the last column, tl - 1: the only column the recursion asks for),
-// writes the exact scores of that column for rows 0..ql-1 into col, from
-// each block's state after its last step. With store, keeps (P, M, score)
-// of every (column, block) for the traceback. Returns the score of the last
-// cell (row ql-1, column tl-1) when stop < 0.
-__device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const SlotA& s) {
+// Diagonal band of a (sub)problem: the cells (i, j) with lo <= i - j <= hi.
+// A path through (i, j) costs at least |i - j| + |(ql - tl) - (i - j)|, so
+// with d >= the subproblem's distance every optimal path lies inside
+// band_for(ql, tl, d). Cells outside are not computed: a strip of blocks
+// runs only over the columns where its rows meet the band, a strip that
+// starts past column 0 begins from upper bounds (vertical deltas of +1 from
+// max(row + 1, column + 1)), and a column the strip above did not compute
+// feeds it a horizontal delta of +1. Every computed score is then an upper
+// bound of the exact one, equal to it on every cell of an optimal path,
+// which is all that the traceback and the split search read (a cell they
+// look up outside the computed columns reads as kFar).
+struct Band {
+  int lo, hi;
+};
+constexpr int32_t kFar = 1 << 28;
+
+__device__ __forceinline__ Band band_for(int ql, int tl, int d) {
+  const int delta = ql - tl;
+  const int ad = delta < 0 ? -delta : delta;
+  const int slack = d >= ad ? (d - ad) / 2 : 0;
+  return Band{min(0, delta) - slack, max(0, delta) + slack};
+}
+
+// The band of the reversed segment (row ql-1-i, column tl-1-j).
+__device__ __forceinline__ Band band_reversed(Band b, int ql, int tl) {
+  return Band{(ql - tl) - b.hi, (ql - tl) - b.lo};
+}
+
+// Columns [cs, ce] that strip k (rows k * 4096 .. + 64 * nbs - 1) computes.
+__device__ __forceinline__ void strip_columns(const Seg& g, Band bd, int k, int nbs, int* cs,
+                                              int* ce) {
+  const int row_lo = k * kWave * 64;
+  const int row_hi = min(g.ql, (k * kWave + nbs) * 64) - 1;
+  *cs = max(0, row_lo - bd.hi);
+  *ce = min(g.tl - 1, row_hi - bd.lo);
+}
+
+// Runs the bit-vector DP of seg over the columns of band bd. With stop >= 0
+// (which must be the last column, tl - 1: the only column the recursion asks
+// for), writes the scores of that column for rows 0..ql-1 into col, from each
+// block's state after its last step (kFar for rows of a strip that does not
+// reach it). With store, keeps (P, M, score) of every computed (column,
+// block) for the traceback. Returns the score of the last cell (row ql-1,
+// column tl-1) when stop < 0. *steps counts the wave's steps.
+__device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const SlotA& s, Band bd,
+                       uint32_t* steps_done) {
   const int lane = lane_id();
   const int nb = (g.ql + 63) / 64;
   const int strips = (nb + kWave - 1) / kWave;
-  int last_score = 0;
+  int last_score = kFar;
+  int prev_cs = 1, prev_ce = 0;  // columns whose bottom carries the strip above stored
+  // the last row of the strip above at the column before this strip's first
+  // (join_col): this strip's first column continues it by +1 per row
+  __shared__ int join_score;
+  int join_col = -1;
   for (int k = 0; k < strips; ++k) {
     const int b = k * kWave + lane;
     const int nbs = min(kWave, nb - k * kWave);  // blocks in this strip
     const bool active_lane = lane < nbs;
+    int cs, ce;
+    strip_columns(g, bd, k, nbs, &cs, &ce);
+    cs = uniform(cs);
+    ce = uniform(ce);
+    // where the next strip starts (its join column is the one before)
+    int next_cs = 0, next_ce = -1;
+    if (k + 1 < strips) {
+      strip_columns(g, bd, k + 1, min(kWave, nb - (k + 1) * kWave), &next_cs, &next_ce);
+    }
+    const int next_join = uniform(next_cs) - 1;
+    if (cs > ce) {
+      // the band misses every row of this strip
+      if (stop >= 0 && active_lane) {
+        for (int i = 0; i < 64; ++i) {
+          const int r = b * 64 + i;
+          if (r < g.ql) col[r] = kFar;
+        }
+      }
+      prev_cs = 1;
+      prev_ce = 0;
+      join_col = -1;
+      wave_sync();
+      continue;
+    }
     Peq peq{0, 0, 0, 0, 0};
     if (active_lane) {
       peq = build_peq(g, b);
     }
     Word P = ~Word(0), M = 0;
-    int score = (b + 1) * 64;  // D at the block's last row before column 0
-    const int steps = g.tl + nbs - 1;
+    // D at the block's last row before column cs, with vertical deltas of +1
+    // down the column: exact at column -1; further right, +1 per row from the
+    // strip above's last row at column cs - 1 (an upper bound, and a column
+    // whose deltas join the carries of the strip above), or from the bound
+    // max(row, column) + 1 when the strip above computed nothing
+    int score;
+    if (cs == 0) {
+      score = (b + 1) * 64;
+    } else {
+      const int top = join_col == cs - 1 ? join_score : max(k * kWave * 64, cs);
+      score = top + (lane + 1) * 64;
+    }
+    wave_sync();  // (join_score read before this strip may write it)
+    const int steps = ce + nbs;  // lane 0 runs columns cs..ce, the last lane ends nbs - 1 later
+    *steps_done += static_cast<uint32_t>(steps - cs);
     const uint64_t store_base = static_cast<uint64_t>(k) * (g.tl + 63) * 64;
     // Lane b works on column c = st - b, the column lane b - 1 had one step
     // earlier: each lane hands its target byte and its bottom carry (hout + 1
@@ -154,18 +235,20 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
     // its byte (and, below the first strip, its carry from the strip above)
     // from 64-column chunks that every lane loads one chunk ahead.
     auto t_at = [&](int j) -> int { return j < g.tl ? g.tc(j) : 0; };
+    // (a column the strip above did not compute: +1, an upper bound)
     auto carry_at = [&](int j) -> int {
-      return k == 0 ? 1 : (j < g.tl ? static_cast<int>(s.carry[j]) : 1);
+      return k == 0 ? 1 : (j >= prev_cs && j <= prev_ce ? static_cast<int>(s.carry[j]) : 1);
     };
     // (an outer loop per chunk: a loaded chunk is only read in the chunk
     // after the one that issued its load, and no loaded value is carried
     // through the step loop, which would make every step wait for memory)
-    int t_next = t_at(lane);
-    int c_next = carry_at(lane);
+    const int st_begin = cs & ~(kWave - 1);
+    int t_next = t_at(st_begin + lane);
+    int c_next = carry_at(st_begin + lane);
     const bool carry_lane = lane == nbs - 1 && k + 1 < strips;
     bool prev_acgt = true;  // (no columns before the first chunk)
     int fwd = 1 << 8;  // (byte, hout + 1) of this lane's last step
-    for (int st0 = 0; st0 < steps; st0 += kWave) {
+    for (int st0 = st_begin; st0 < steps; st0 += kWave) {
      const int t_cur = t_next, c_cur = c_next;
      // this chunk's bytes and carries (loaded a chunk ago) are waited for
      // here, once, before the next chunk's loads go out
@@ -176,7 +259,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
      const bool mine_acgt = st0 + lane >= g.tl || is_acgt(t_cur, code2(t_cur));
      const bool chunk_acgt = __all(mine_acgt);
      const int st_end = min(steps, st0 + kWave);
-     const bool fast = chunk_acgt && prev_acgt && st0 >= nbs - 1 && st_end <= g.tl;
+     const bool fast = chunk_acgt && prev_acgt && st0 >= cs + nbs - 1 && st_end - 1 <= ce;
      prev_acgt = chunk_acgt;
      t_next = t_at(st0 + kWave + lane);
      c_next = carry_at(st0 + kWave + lane);
@@ -204,6 +287,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
        }
        if (carry_lane) {
          s.carry[c] = static_cast<int8_t>(hout);
+         if (c == next_join) join_score = score;
        }
        fwd = static_cast<int>(tc) | ((hout + 1) << 8);
       }
@@ -218,7 +302,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
       const uint8_t tc = static_cast<uint8_t>(lane == 0 ? t0 : (up & 0xff));
       const int hin = lane == 0 ? h0 : (up >> 8) - 1;
       int hout = 0;
-      if (active_lane && c >= 0 && c < g.tl) {
+      if (active_lane && c >= cs && c <= ce) {
         // rows equal to the target byte (exact byte equality, as edlib's
         // alphabet without equalities)
         const int code = code2(tc);
@@ -238,23 +322,25 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
         }
         if (carry_lane) {
           s.carry[c] = static_cast<int8_t>(hout);
+          if (c == next_join) join_score = score;
         }
       }
       fwd = static_cast<int>(tc) | ((hout + 1) << 8);
      }
     }
+    const bool reaches_end = ce == g.tl - 1;
     if (stop >= 0 && active_lane) {
-      // exact values of this block's rows at the last column
+      // this block's rows at the last column
       int v = score;
       for (int i = 63; i >= 0; --i) {
         const int r = b * 64 + i;
-        if (r < g.ql) col[r] = v;
+        if (r < g.ql) col[r] = reaches_end ? v : kFar;
         if ((P >> i) & 1) v -= 1;
         if ((M >> i) & 1) v += 1;
       }
     }
     // score of the last row (in the last strip's last block)
-    if (k == strips - 1) {
+    if (k == strips - 1 && reaches_end) {
       const int last_block = nb - 1;
       const int last_bit = (g.ql - 1) % 64;
       int v = score;
@@ -264,7 +350,10 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
       }
       last_score = __shfl(v, last_block - k * kWave, kWave);
     }
-    wave_sync();  // carries and stored columns visible to every lane
+    prev_cs = cs;
+    prev_ce = ce;
+    join_col = next_join >= cs && next_join <= ce ? next_join : -1;
+    wave_sync();  // carries, stored columns and join_score visible to every lane
   }
   return last_score;
 }
@@ -277,7 +366,7 @@ __device__ __forceinline__ Word readlane64(Word x, int l) {
 }
 
 // Traceback of a stored leaf; writes ops (forward order) at out, returns count.
-__device__ int leaf_traceback(const Seg& g, const SlotA& s, uint8_t* out) {
+__device__ int leaf_traceback(const Seg& g, const SlotA& s, uint8_t* out, Band bd) {
   // The walk is uniform (every lane follows it); the stored words it reads
   // come from a window loaded by all lanes in one round trip: lane l holds
   // column cw - l of block bw (the block of row r) and of block bw - 1, so
@@ -316,6 +405,13 @@ __device__ int leaf_traceback(const Seg& g, const SlotA& s, uint8_t* out) {
     auto val = [&](int rr, int cc) -> int {
       if (rr < 0) return cc + 1;
       if (cc < 0) return rr + 1;
+      {
+        // a column the cell's strip did not compute: off every optimal path
+        const int k = rr / (64 * kWave);
+        int cs, ce;
+        strip_columns(g, bd, k, min(kWave, nb - k * kWave), &cs, &ce);
+        if (cc < cs || cc > ce) return kFar;
+      }
       const int l = cw - cc;
       const bool above = rr / 64 < bw;
       const Word P = readlane64(above ? pa : pb, l);
@@ -407,13 +503,32 @@ __device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResul
     R.status = kAlignOk;
     return;
   }
+  // The whole pair's distance is not known: its band starts from a guess k
+  // (the batch's band width, else |Δ| + max(|q|, |t|) / 5) and doubles until
+  // the banded distance is at most k (then it is exact: with k >= the true
+  // distance every optimal path lies inside the band) or the band holds the
+  // whole matrix. Every subproblem below knows its score, and its band is
+  // the tightest one for it.
+  const int delta_top = QL > TL ? QL - TL : TL - QL;
+  int k_top = A.band0 > 0 ? max(static_cast<int>(A.band0), delta_top)
+                          : delta_top + max(QL, TL) / 5 + 64;
+  auto full_band = [](Band b, int ql, int tl) { return b.lo <= -(tl - 1) && b.hi >= ql - 1; };
+  uint32_t steps = 0;
+
   if (is_leaf(QL, TL)) {
     Seg g{Q, T, QL, TL, false};
-    best = forward(g, -1, nullptr, true, s);
-    n_ops = leaf_traceback(g, s, out);
+    Band bd;
+    while (true) {
+      bd = band_for(QL, TL, k_top);
+      best = forward(g, -1, nullptr, true, s, bd, &steps);
+      if (best <= k_top || full_band(bd, QL, TL)) break;
+      k_top *= 2;
+    }
+    n_ops = leaf_traceback(g, s, out, bd);
     R.n_ops = static_cast<uint32_t>(n_ops);
     R.edit_distance = best;
     R.status = kAlignOk;
+    R.pad = steps;
     return;
   }
   // top level: the Hirschberg split also yields the distance
@@ -435,23 +550,26 @@ __device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResul
     }
     if (is_leaf(ql, tl)) {
       Seg g{Q + tk.q0, T + tk.t0, ql, tl, false};
-      forward(g, -1, nullptr, true, s);
-      n_ops += leaf_traceback(g, s, out + n_ops);
+      const Band bd = band_for(ql, tl, tk.score);
+      forward(g, -1, nullptr, true, s, bd, &steps);
+      n_ops += leaf_traceback(g, s, out + n_ops, bd);
       continue;
     }
     const int lw = tl / 2, rw = tl - lw;
-    // forward over the left half: scores of column lw-1
     Seg gl{Q + tk.q0, T + tk.t0, ql, lw, false};
-    forward(gl, lw - 1, s.col_left, false, s);
-    // reverse over the right half: col_right[r'] = ED of the last r'+1 query
-    // bytes against t[lw..)
     Seg gr{Q + tk.q0, T + tk.t0 + lw, ql, rw, true};
-    forward(gr, rw - 1, s.col_right, false, s);
-    wave_sync();
-    // right(x) = ED(q[x..], t[lw..]) = col_right[ql-1-x] for x < ql, rw at ql
     int score = tk.score;
-    if (score < 0) {
-      // distance of the whole pair: min over all split rows
+    while (true) {
+      const Band bd = band_for(ql, tl, score >= 0 ? score : k_top);
+      // forward over the left half: scores of column lw-1
+      forward(gl, lw - 1, s.col_left, false, s, bd, &steps);
+      // reverse over the right half: col_right[r'] = ED of the last r'+1
+      // query bytes against t[lw..)
+      forward(gr, rw - 1, s.col_right, false, s, band_reversed(bd, ql, tl), &steps);
+      wave_sync();
+      if (score >= 0) break;
+      // right(x) = ED(q[x..], t[lw..]) = col_right[ql-1-x] for x < ql, rw at
+      // ql; the distance of the whole pair: min over all split rows
       int m = 0x7fffffff;
       for (int x = lane - 1; x <= ql - 1; x += kWave) {
         const int lv = x < 0 ? lw : s.col_left[x];
@@ -459,8 +577,13 @@ __device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResul
         m = min(m, lv + rv);
       }
       for (int d = 1; d < kWave; d <<= 1) m = min(m, __shfl_xor(m, d, kWave));
-      score = uniform(m);
-      best = score;
+      m = uniform(m);
+      if (m <= k_top || full_band(bd, ql, tl)) {
+        score = m;
+        best = score;
+        break;
+      }
+      k_top *= 2;
     }
     // first interior split row x in [0, ql-2]
     int split = -2;
@@ -509,6 +632,7 @@ __device__ void align_pair(const AlignParams& A, const AlignPair& pr, AlignResul
   R.n_ops = static_cast<uint32_t>(n_ops);
   R.edit_distance = best;
   R.status = kAlignOk;
+  R.pad = steps;
 }
 
 }  // namespace
@@ -526,7 +650,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void al
     if (q >= A.n_order) {
       break;
     }
-    const uint32_t i = A.order[q];
+    const uint32_t i = A.order != nullptr ? A.order[q] : q;
     AlignResult res{0, -1, kAlignNotRun, 0};
     align_pair(A, A.pairs[i], res);
     __syncthreads();
@@ -540,6 +664,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void al
 hipError_t launch_align_batch(const AlignParams& params, uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL(align_batch_kernel, dim3(grid), dim3(64), 0, stream, params);
   return hipGetLastError();
+}
+
+// Clears the work-queue counter with a kernel store (no memset node, so the
+// sequence can sit inside a captured graph).
+__global__ void clear_counter_kernel(uint32_t* counter) {
+  if (threadIdx.x == 0) *counter = 0;
+}
+
+hipError_t launch_align_batch_cleared(const AlignParams& params, uint32_t grid,
+                                      hipStream_t stream) {
+  hipLaunchKernelGGL(clear_counter_kernel, dim3(1), dim3(64), 0, stream, params.queue_head);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_align_batch(params, grid, stream);
 }
 
 // Breaking points of every aligned pair (racon overlap.cpp:226-292, the walk
@@ -558,6 +696,10 @@ __global__ __launch_bounds__(256) void breaking_points_kernel(
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const BreakSpan sp = spans[i];
+  if (results[i].status != kAlignOk) {
+    counts[i] = 0xfffffffeu;  // not aligned on the device
+    return;
+  }
   const uint8_t* op = ops + pairs[i].out_off;
   const uint32_t n_ops = results[i].n_ops;
   const uint32_t W = window_length;

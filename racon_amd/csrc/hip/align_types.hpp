@@ -29,7 +29,7 @@ struct AlignResult {
   uint32_t n_ops;
   int32_t edit_distance;
   int32_t status;  // 0 ok, < 0 error
-  uint32_t pad;
+  uint32_t pad;    // wave steps the pair took (each step: one column of up to 64 blocks)
 };
 
 enum : int32_t {
@@ -83,18 +83,32 @@ inline AlignSlotLayout align_slot_layout(const AlignSlotCaps& c) {
   return l;
 }
 
+// Slot capacities for pairs of at most max_q query and max_t target bytes:
+// the score columns and carries, and the stored columns of the largest leaf
+// (under kLeafBytes at 20 bytes per stored word) plus one block row's slack.
+inline AlignSlotCaps align_caps_for(uint32_t max_q, uint32_t max_t) {
+  AlignSlotCaps caps{};
+  caps.q = max_q < 64 ? 64 : max_q;
+  caps.t = max_t < 64 ? 64 : max_t;
+  const uint64_t max_blocks = (caps.q + 63) / 64 + 1;
+  caps.store_words = kLeafBytes / 20 + 64 * (max_blocks + 2);
+  return caps;
+}
+
 struct AlignParams {
   const AlignPair* pairs;
   const uint8_t* arena;
   uint8_t* ops;
   AlignResult* results;
-  const uint32_t* order;
+  const uint32_t* order;  // queue position -> pair index; null: identity
   uint32_t n_order;
   uint32_t* queue_head;
   uint8_t* scratch;
   uint64_t slot_bytes;
   AlignSlotCaps caps;
   AlignSlotLayout layout;
+  uint32_t band0;  // first band guess (distance bound) of a whole pair; 0: automatic
+  uint32_t pad;
 };
 
 }  // namespace rh

@@ -2329,8 +2329,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     // score width: 16-bit ring rows when every score of the layer fits
     const int32_t max_abs = max(max(abs(P.match), abs(P.mismatch)), abs(P.gap));
     const bool narrow = static_cast<int64_t>(max_abs) * (V + L + 2) < 32000;
+    // (a pass for very long layers may keep no rows in LDS at all)
     const int n_slots = min(kMaxLdsSlots, narrow ? static_cast<int>(P.ring_rows)
-                                                 : max(1, static_cast<int>(P.ring_rows) / 2));
+                                                 : static_cast<int>(P.ring_rows) / 2);
 
     bool strip_ok = false;  // the one-pass DP applies (dp_layer_strip)
 
@@ -2343,6 +2344,11 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
                      : build_rows<false>(s.p, s.L, V, n_slots, rows_lds, ring_bytes));
     strip_ok = narrow && L >= 1 && max_abs <= 4096 && max_np <= 255;
     wave_sync();
+    if (max_np > static_cast<int>(kCodePayload) + 1) {
+      // a move code holds the in-edge index in 13 bits
+      R.status = kPoaInDegree;
+      return;
+    }
 
     clk.lap(R, kPhaseRows);
     // 3. DP over the layer (column 0 folded into each row)

@@ -67,7 +67,9 @@ enum : int32_t {
   kPoaOutOverflow = -3,        // consensus longer than out_cap
   kPoaLoopLimit = -4,          // heaviest-bundle completion did not end
   kPoaBadInput = -5,           // layer end outside the graph
+  kPoaInDegree = -6,           // a node with more than 8192 in-edges
   kPoaNotRun = -100,
+  kPoaRejectedOnHost = -101,   // outside the device limits, never launched
 };
 
 // Capacities of one scratch slot.

@@ -45,6 +45,7 @@ struct rh_align_batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool uploaded = false, ran = false;
+  uint32_t band0 = 0;  // first band guess of every pair (0: per pair, automatic)
   rh_align_stats stats{};
 };
 
@@ -195,11 +196,7 @@ int32_t rh_align_run(rh_align_batch* b) {
     b->ran = true;
     return RH_OK;
   }
-  rh::AlignSlotCaps caps{};
-  caps.q = std::max<uint32_t>(b->max_q, 64);
-  caps.t = std::max<uint32_t>(b->max_t, 64);
-  const uint64_t max_blocks = (caps.q + 63) / 64 + 1;
-  caps.store_words = rh::kLeafBytes / 20 + 64 * (max_blocks + 2);
+  const rh::AlignSlotCaps caps = rh::align_caps_for(b->max_q, b->max_t);
   const rh::AlignSlotLayout layout = rh::align_slot_layout(caps);
   const uint64_t slot_bytes = rh::align_up_a(layout.total, 4096);
 
@@ -229,6 +226,7 @@ int32_t rh_align_run(rh_align_batch* b) {
   p.slot_bytes = slot_bytes;
   p.caps = caps;
   p.layout = layout;
+  p.band0 = b->band0;
 
   RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
   RH_HIP_CHECK(rh::launch_align_batch(p, static_cast<uint32_t>(slots), b->stream),
@@ -243,15 +241,27 @@ int32_t rh_align_run(rh_align_batch* b) {
                          hipMemcpyDeviceToHost),
                RH_ERR_DEVICE);
   b->stats.ops = 0;
+  b->stats.steps = 0;
+  b->stats.rejected = 0;
   for (uint32_t i = 0; i < n; ++i) {
+    b->stats.steps += b->results[i].pad;
     if (b->results[i].status != rh::kAlignOk) {
-      rh::set_error("rh_align_run: pair " + std::to_string(i) + " failed on the device (status " +
-                    std::to_string(b->results[i].status) + ")");
-      return RH_ERR_KERNEL;
+      // left to the caller's CPU path (edit distance -1 in rh_align_download)
+      b->stats.rejected++;
+      continue;
     }
     b->stats.ops += b->results[i].n_ops;
   }
   b->ran = true;
+  return RH_OK;
+}
+
+int32_t rh_align_set_band(rh_align_batch* b, uint32_t band) {
+  if (b == nullptr) {
+    rh::set_error("rh_align_set_band: null batch");
+    return RH_ERR_ARGUMENT;
+  }
+  b->band0 = band;
   return RH_OK;
 }
 
@@ -286,7 +296,8 @@ int32_t rh_align_download(rh_align_batch* b, uint8_t* ops, uint64_t capacity,
   }
   uint64_t pos = 0;
   for (size_t i = 0; i < b->pairs.size(); ++i) {
-    const uint32_t len = b->results[i].n_ops;
+    const bool ok = b->results[i].status == rh::kAlignOk;
+    const uint32_t len = ok ? b->results[i].n_ops : 0;
     if (pos + len > capacity) {
       rh::set_error("rh_align_download: output capacity too small");
       return RH_ERR_ARGUMENT;
@@ -294,7 +305,7 @@ int32_t rh_align_download(rh_align_batch* b, uint8_t* ops, uint64_t capacity,
     memcpy(ops + pos, dev.data() + b->pairs[i].out_off, len);
     offsets[i] = pos;
     n_ops[i] = len;
-    edit_distance[i] = b->results[i].edit_distance;
+    edit_distance[i] = ok ? b->results[i].edit_distance : -1;
     pos += len;
   }
   b->stats.d2h_ms = rh::ms_since(t0);

@@ -63,6 +63,10 @@ struct rh_poa_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool uploaded = false;
   bool ran = false;
+  bool submitted = false;  // between rh_poa_submit and rh_poa_wait
+  bool launched = false;   // the submit queued kernel passes
+  uint32_t lds_limit = 0;
+  std::vector<std::vector<uint32_t>> pass_orders;  // window order of each queued pass
   rh_poa_stats stats{};
 };
 
@@ -127,88 +131,172 @@ uint32_t priority_windows(const rh_poa_batch* b, const std::vector<uint32_t>& or
   return n;
 }
 
-int32_t run_pass(rh_poa_batch* b, const std::vector<uint32_t>& order, const PoaSlotCaps& caps,
-                 uint32_t max_slots, int32_t mode, double* kernel_ms) {
-  if (order.empty()) {
-    return RH_OK;
-  }
-  const rh::PoaSlotLayout layout = rh::poa_slot_layout(caps);
-  const uint64_t slot_bytes = rh::align_up(layout.total, 4096);
+// One kernel pass: slot capacities and the DP rows kept in LDS per wave.
+struct PassPlan {
+  PoaSlotCaps caps;
+  uint32_t ring;  // LDS ring rows (0: every DP row in the slot's global matrix)
+  uint32_t lds;   // dynamic LDS bytes per wave
+};
 
+// Layers up to this length run in the main pass; longer ones run in a pass
+// of their own with a wider row pitch (and a shallower LDS ring), so that one
+// long layer does not cut the LDS ring and the occupancy of every window.
+constexpr uint32_t kMainPassMaxLayer = 3072;
+
+// The plan for `ws`: graph arrays for its largest window, the row pitch of
+// its longest layer, DP rows capped at row_cap (0: exact, node bound + 1),
+// and the deepest ring (the tuned or occupancy default, at most 8 rows in
+// long passes) that fits the per-workgroup LDS limit. False when even a
+// ring of 0 rows does not fit (the layer bytes alone exceed the limit).
+bool plan_pass(const rh_poa_batch* b, const std::vector<uint32_t>& ws, uint32_t row_cap,
+               uint32_t lds_limit, PassPlan* plan) {
+  PoaSlotCaps caps{};
+  uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
+  for (uint32_t w : ws) {
+    max_nodes = std::max(max_nodes, b->windows[w].node_bound);
+    max_edges = std::max(max_edges, b->windows[w].edge_bound);
+    max_cols = std::max(max_cols, b->windows[w].max_layer_len + 1);
+  }
+  caps.nodes = max_nodes;
+  caps.edges = max_edges;
+  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols - 1 + rh::kPoaColPad, 16));
+  caps.rows = row_cap == 0 ? max_nodes + 1 : std::min<uint32_t>(max_nodes + 1, row_cap);
+  caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
+  uint32_t ring = ring_rows_for(caps.cols, b->tune_ring);
+  while (ring > 0 && rh::poa_lds_bytes(caps.cols, ring) > lds_limit) --ring;
+  if (ring == 1) ring = 0;  // one 16-bit row is no slot in 32-bit mode; keep none
+  if (rh::poa_lds_bytes(caps.cols, ring) > lds_limit) {
+    return false;
+  }
+  plan->caps = caps;
+  plan->ring = ring;
+  plan->lds = rh::poa_lds_bytes(caps.cols, ring);
+  return true;
+}
+
+// A pass ready to launch: its plan, resident slots and slot size.
+struct PassLaunch {
+  PassPlan plan;
+  uint64_t slots = 0;
+  uint64_t slot_bytes = 0;
+  rh::PoaSlotLayout layout{};
+};
+
+int32_t size_pass(rh_poa_batch* b, uint64_t n_windows, const PassPlan& plan, int32_t mode,
+                  uint64_t scratch_held, PassLaunch* out) {
+  int per_cu = 0, cus = 0;
+  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, plan.lds), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
+               RH_ERR_DEVICE);
+  uint64_t max_slots = static_cast<uint64_t>(std::max(1, per_cu)) * std::max(1, cus);
+  if (b->tune_slots != 0) {
+    max_slots = b->tune_slots;
+  }
+  out->plan = plan;
+  out->layout = rh::poa_slot_layout(plan.caps);
+  out->slot_bytes = rh::align_up(out->layout.total, 4096);
   size_t free_b = 0, total_b = 0;
   RH_HIP_CHECK(hipMemGetInfo(&free_b, &total_b), RH_ERR_DEVICE);
   // keep the slots within ~60% of free memory (plus what we already hold)
-  const uint64_t budget = static_cast<uint64_t>((free_b + b->d_scratch.bytes) * 0.6);
-  uint64_t slots = std::max<uint64_t>(1, budget / slot_bytes);
+  const uint64_t budget = static_cast<uint64_t>((free_b + scratch_held) * 0.6);
+  uint64_t slots = std::max<uint64_t>(1, budget / out->slot_bytes);
   if (mode != RH_POA_STATIC) {
     slots = std::min<uint64_t>(slots, max_slots);
   }
-  slots = std::min<uint64_t>(slots, order.size());
+  slots = std::min<uint64_t>(slots, n_windows);
   if (mode == RH_POA_PLAIN) {
     slots = 1;
   }
-  RH_HIP_CHECK(b->d_scratch.reserve(slots * slot_bytes), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_order.reserve(order.size() * sizeof(uint32_t)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(hipMemcpyAsync(b->d_order.ptr, order.data(), order.size() * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, b->stream),
-               RH_ERR_DEVICE);
-  RH_HIP_CHECK(b->d_queue.reserve(sizeof(uint32_t)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(hipMemsetAsync(b->d_queue.ptr, 0, sizeof(uint32_t), b->stream), RH_ERR_DEVICE);
+  out->slots = slots;
+  return RH_OK;
+}
 
-  PoaParams p{};
-  p.windows = b->d_windows.as<PoaWindow>();
-  p.layers = b->d_layers.as<PoaLayer>();
-  p.arena = b->d_arena.as<uint8_t>();
-  p.out = b->d_out.as<uint8_t>();
-  p.results = b->d_results.as<PoaResult>();
-  p.order = b->d_order.as<uint32_t>();
-  p.n_order = static_cast<uint32_t>(order.size());
-  p.queue_head = b->d_queue.as<uint32_t>();
-  p.scratch = b->d_scratch.as<uint8_t>();
-  p.slot_bytes = slot_bytes;
-  p.caps = caps;
-  p.layout = layout;
-  p.match = b->match;
-  p.mismatch = b->mismatch;
-  p.gap = b->gap;
-  p.n_slots = static_cast<uint32_t>(slots);
-  p.ring_rows = ring_rows_for(caps.cols, b->tune_ring);
-  p.lds_bytes = rh::poa_lds_bytes(caps.cols, p.ring_rows);
-  p.n_priority = priority_windows(b, order);
-
-  RH_HIP_CHECK(b->d_params.reserve(sizeof(PoaParams)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(hipMemcpyAsync(b->d_params.ptr, &p, sizeof(PoaParams), hipMemcpyHostToDevice,
-                              b->stream),
-               RH_ERR_DEVICE);
-  const PoaParams* dp = b->d_params.as<PoaParams>();
-
+// Queues passes back to back on the batch's stream (their orders, queue
+// counters and parameter blocks side by side; the passes share the scratch,
+// which is sized for the largest) between events ev0 and ev1. Returns
+// without waiting.
+int32_t launch_passes(rh_poa_batch* b, const std::vector<const std::vector<uint32_t>*>& orders,
+                      const std::vector<PassLaunch>& passes, int32_t mode) {
+  const size_t np = passes.size();
+  uint64_t scratch = 0, total_order = 0;
+  for (size_t k = 0; k < np; ++k) {
+    scratch = std::max(scratch, passes[k].slots * passes[k].slot_bytes);
+    total_order += orders[k]->size();
+  }
+  RH_HIP_CHECK(b->d_scratch.reserve(scratch), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_order.reserve(std::max<uint64_t>(total_order, 1) * sizeof(uint32_t)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_queue.reserve(std::max<size_t>(np, 1) * sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_params.reserve(std::max<size_t>(np, 1) * sizeof(PoaParams)), RH_ERR_MEMORY);
   RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
-  if (mode == RH_POA_PLAIN) {
-    for (uint32_t q = 0; q < order.size(); ++q) {
-      RH_HIP_CHECK(rh::launch_poa_single(dp, p.lds_bytes, q, b->stream), RH_ERR_KERNEL);
-      b->stats.launches++;
-    }
-  } else if (mode == RH_POA_STATIC) {
-    // one workgroup (and slot) per window; launches of `slots` windows when
-    // the slots of all windows do not fit in memory at once
-    for (uint64_t first = 0; first < order.size(); first += slots) {
-      const uint64_t count = std::min<uint64_t>(slots, order.size() - first);
-      RH_HIP_CHECK(rh::launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
-                                         static_cast<uint32_t>(count), b->stream),
+  if (np > 0) {
+    RH_HIP_CHECK(hipMemsetAsync(b->d_queue.ptr, 0, np * sizeof(uint32_t), b->stream),
+                 RH_ERR_DEVICE);
+  }
+  uint64_t order_at = 0;
+  for (size_t k = 0; k < np; ++k) {
+    const std::vector<uint32_t>& order = *orders[k];
+    const PassLaunch& pl = passes[k];
+    uint32_t* d_order = b->d_order.as<uint32_t>() + order_at;
+    order_at += order.size();
+    RH_HIP_CHECK(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, b->stream),
+                 RH_ERR_DEVICE);
+    PoaParams p{};
+    p.windows = b->d_windows.as<PoaWindow>();
+    p.layers = b->d_layers.as<PoaLayer>();
+    p.arena = b->d_arena.as<uint8_t>();
+    p.out = b->d_out.as<uint8_t>();
+    p.results = b->d_results.as<PoaResult>();
+    p.order = d_order;
+    p.n_order = static_cast<uint32_t>(order.size());
+    p.queue_head = b->d_queue.as<uint32_t>() + k;
+    p.scratch = b->d_scratch.as<uint8_t>();
+    p.slot_bytes = pl.slot_bytes;
+    p.caps = pl.plan.caps;
+    p.layout = pl.layout;
+    p.match = b->match;
+    p.mismatch = b->mismatch;
+    p.gap = b->gap;
+    p.n_slots = static_cast<uint32_t>(pl.slots);
+    p.ring_rows = pl.plan.ring;
+    p.lds_bytes = pl.plan.lds;
+    p.n_priority = priority_windows(b, order);
+    PoaParams* dp = b->d_params.as<PoaParams>() + k;
+    RH_HIP_CHECK(hipMemcpyAsync(dp, &p, sizeof(PoaParams), hipMemcpyHostToDevice, b->stream),
+                 RH_ERR_DEVICE);
+    if (mode == RH_POA_PLAIN) {
+      for (uint32_t q = 0; q < order.size(); ++q) {
+        RH_HIP_CHECK(rh::launch_poa_single(dp, p.lds_bytes, q, b->stream), RH_ERR_KERNEL);
+        b->stats.launches++;
+      }
+    } else if (mode == RH_POA_STATIC) {
+      // one workgroup (and slot) per window; launches of `slots` windows when
+      // the slots of all windows do not fit in memory at once
+      for (uint64_t first = 0; first < order.size(); first += pl.slots) {
+        const uint64_t count = std::min<uint64_t>(pl.slots, order.size() - first);
+        RH_HIP_CHECK(rh::launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
+                                           static_cast<uint32_t>(count), b->stream),
+                     RH_ERR_KERNEL);
+        b->stats.launches++;
+      }
+    } else {
+      RH_HIP_CHECK(rh::launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(pl.slots),
+                                        b->stream),
                    RH_ERR_KERNEL);
       b->stats.launches++;
     }
-  } else {
-    RH_HIP_CHECK(rh::launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(slots), b->stream),
-                 RH_ERR_KERNEL);
-    b->stats.launches++;
   }
   RH_HIP_CHECK(hipEventRecord(b->ev1, b->stream), RH_ERR_DEVICE);
+  return RH_OK;
+}
+
+// Waits for the passes queued by launch_passes and adds their device time.
+int32_t finish_passes(rh_poa_batch* b, double* kernel_ms) {
   RH_HIP_CHECK(hipEventSynchronize(b->ev1), RH_ERR_KERNEL);
   float ms = 0;
   RH_HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1), RH_ERR_DEVICE);
   *kernel_ms += ms;
-  b->stats.slots = static_cast<uint32_t>(slots);
   return RH_OK;
 }
 
@@ -297,6 +385,10 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   if (b == nullptr || (n > 0 && windows == nullptr)) {
     rh::set_error("rh_poa_upload: null argument");
     return RH_ERR_ARGUMENT;
+  }
+  if (b->submitted) {
+    rh::set_error("rh_poa_upload: the batch is running (call rh_poa_wait first)");
+    return RH_ERR_STATE;
   }
   RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
   auto t0 = std::chrono::steady_clock::now();
@@ -389,12 +481,12 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
     }
     const uint64_t node_bound = w.backbone_length + sum_len;
     const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
-    // DP rows are packed in 24 bits of a predecessor entry and the in-edge
-    // index (at most one in-edge per node and layer) in 13 bits of a move code
-    if (node_bound >= (1ull << 24) - 1 || edge_bound > 0x7fffffffull ||
-        w.n_layers >= (1u << 13) - 2) {
-      rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is too large");
-      return RH_ERR_ARGUMENT;
+    // DP rows are packed in 24 bits of a predecessor entry: a larger graph is
+    // refused here (RH_WINDOW_REJECTED, the caller's CPU path takes it); the
+    // in-edge index of a move code (13 bits) is checked on the device
+    if (node_bound >= (1ull << 24) - 1) {
+      b->results[i].status = rh::kPoaRejectedOnHost;
+      continue;
     }
     d.node_bound = static_cast<uint32_t>(node_bound);
     d.edge_bound = static_cast<uint32_t>(edge_bound);
@@ -449,14 +541,18 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   return RH_OK;
 }
 
-int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
+int32_t rh_poa_submit(rh_poa_batch* b, int32_t mode) {
   rh::clear_error();
   if (b == nullptr || !b->uploaded) {
-    rh::set_error("rh_poa_run: nothing uploaded");
+    rh::set_error("rh_poa_submit: nothing uploaded");
+    return RH_ERR_STATE;
+  }
+  if (b->submitted) {
+    rh::set_error("rh_poa_submit: the batch is already running (call rh_poa_wait)");
     return RH_ERR_STATE;
   }
   if (mode != RH_POA_BATCHED && mode != RH_POA_PLAIN && mode != RH_POA_STATIC) {
-    rh::set_error("rh_poa_run: unknown mode");
+    rh::set_error("rh_poa_submit: unknown mode");
     return RH_ERR_ARGUMENT;
   }
   RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
@@ -464,73 +560,95 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   b->stats.kernel_ms = 0;
   b->stats.launches = 0;
   b->stats.retried = 0;
+  b->stats.rejected = 0;
+  b->ran = false;
+  b->pass_orders.clear();
+  b->submitted = true;
+  if (n == 0) {
+    return RH_OK;
+  }
+  // fresh results: every kernel window starts as not-run, the ones refused
+  // at upload keep their status
+  for (uint32_t w = 0; w < n; ++w) {
+    if (b->results[w].status != rh::kPoaRejectedOnHost) b->results[w] = make_result();
+  }
+  int lds_limit = 0;
+  RH_HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                     b->device),
+               RH_ERR_DEVICE);
+  b->lds_limit = static_cast<uint32_t>(lds_limit);
+
+  // bins by layer length: the main pass and the long-layer pass (windows in
+  // the cost order of the queue either way, in upload order for STATIC)
+  std::vector<uint32_t> bins[2];
+  for (uint32_t w : b->gpu_order) {
+    bins[b->windows[w].max_layer_len <= kMainPassMaxLayer ? 0 : 1].push_back(w);
+  }
+  const uint32_t row_cap = b->tune_rows ? b->tune_rows : 4096;
+  std::vector<PassLaunch> passes;
+  for (auto& ws : bins) {
+    if (ws.empty()) continue;
+    if (mode == RH_POA_STATIC) std::sort(ws.begin(), ws.end());
+    PassPlan plan{};
+    if (!plan_pass(b, ws, row_cap, b->lds_limit, &plan)) {
+      for (uint32_t w : ws) b->results[w].status = rh::kPoaRejectedOnHost;
+      continue;
+    }
+    PassLaunch pl;
+    const int32_t rc = size_pass(b, ws.size(), plan, mode, b->d_scratch.bytes, &pl);
+    if (rc != RH_OK) return rc;
+    passes.push_back(pl);
+    b->pass_orders.push_back(std::move(ws));
+  }
+  RH_HIP_CHECK(hipMemcpyAsync(b->d_results.ptr, b->results.data(), n * sizeof(PoaResult),
+                              hipMemcpyHostToDevice, b->stream),
+               RH_ERR_DEVICE);
+  std::vector<const std::vector<uint32_t>*> orders;
+  for (const auto& o : b->pass_orders) orders.push_back(&o);
+  if (!passes.empty()) {
+    b->stats.slots = static_cast<uint32_t>(passes[0].slots);
+    const int32_t rc = launch_passes(b, orders, passes, mode);
+    if (rc != RH_OK) return rc;
+  }
+  b->launched = !passes.empty();
+  return RH_OK;
+}
+
+int32_t rh_poa_wait(rh_poa_batch* b) {
+  rh::clear_error();
+  if (b == nullptr || !b->submitted) {
+    rh::set_error("rh_poa_wait: nothing submitted");
+    return RH_ERR_STATE;
+  }
+  b->submitted = false;
+  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  const uint32_t n = static_cast<uint32_t>(b->windows.size());
   if (n == 0) {
     b->ran = true;
     return RH_OK;
   }
-  // fresh results: every kernel window starts as not-run
-  std::vector<PoaResult> init(n, make_result());
-  RH_HIP_CHECK(hipMemcpyAsync(b->d_results.ptr, init.data(), n * sizeof(PoaResult),
-                              hipMemcpyHostToDevice, b->stream),
-               RH_ERR_DEVICE);
-
-  // default slot: graph arrays for the largest window, DP rows capped
-  PoaSlotCaps caps{};
-  uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
-  for (uint32_t w : b->gpu_order) {
-    max_nodes = std::max(max_nodes, b->windows[w].node_bound);
-    max_edges = std::max(max_edges, b->windows[w].edge_bound);
-    max_cols = std::max(max_cols, b->windows[w].max_layer_len + 1);
-  }
-  caps.nodes = max_nodes;
-  caps.edges = max_edges;
-  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols - 1 + rh::kPoaColPad, 16));
-  caps.rows = std::min<uint32_t>(max_nodes + 1, b->tune_rows ? b->tune_rows : 4096);
-  caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
-
-  const uint32_t lds = rh::poa_lds_bytes(caps.cols, ring_rows_for(caps.cols, b->tune_ring));
-  if (lds > 64u * 1024u) {
-    rh::set_error("rh_poa_run: layers longer than the LDS ring allows");
-    return RH_ERR_ARGUMENT;
-  }
-  int per_cu = 0, cus = 0;
-  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, lds), RH_ERR_DEVICE);
-  RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
-               RH_ERR_DEVICE);
-  uint32_t resident = static_cast<uint32_t>(std::max(1, per_cu) * std::max(1, cus));
-  if (b->tune_slots != 0) {
-    resident = b->tune_slots;
-  }
-
   double kernel_ms = 0;
-  std::vector<uint32_t> upload_order;
-  if (mode == RH_POA_STATIC) {
-    upload_order = b->gpu_order;
-    std::sort(upload_order.begin(), upload_order.end());
-  }
-  int32_t rc = run_pass(b, mode == RH_POA_STATIC ? upload_order : b->gpu_order, caps, resident,
-                        mode, &kernel_ms);
-  if (rc != RH_OK) {
-    return rc;
+  if (b->launched) {
+    const int32_t rc = finish_passes(b, &kernel_ms);
+    if (rc != RH_OK) return rc;
   }
   RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
                          hipMemcpyDeviceToHost),
                RH_ERR_DEVICE);
 
-  // windows that outgrew the default slot: rerun with exact bounds
-  std::vector<uint32_t> retry;
+  // windows that outgrew their slot: rerun with exact bounds (rows = node
+  // bound + 1), in one pass per layer-length bin
+  std::vector<uint32_t> retry[2];
   for (uint32_t w : b->gpu_order) {
     if (is_retry_status(b->results[w].status)) {
-      retry.push_back(w);
+      retry[b->windows[w].max_layer_len <= kMainPassMaxLayer ? 0 : 1].push_back(w);
     }
   }
-  if (!retry.empty()) {
-    b->stats.retried = retry.size();
-    PoaSlotCaps big = caps;
+  if (!retry[0].empty() || !retry[1].empty()) {
+    b->stats.retried = retry[0].size() + retry[1].size();
     uint64_t extra_out = 0;
-    for (uint32_t w : retry) {
-      big.rows = std::max(big.rows, b->windows[w].node_bound + 1);
-      extra_out += rh::align_up(b->windows[w].node_bound, 16);
+    for (const auto& ws : retry) {
+      for (uint32_t w : ws) extra_out += rh::align_up(b->windows[w].node_bound, 16);
     }
     // bigger output regions for the retried windows, appended to d_out
     const uint64_t old_out = b->out_bytes;
@@ -541,18 +659,36 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
     b->d_out.release();
     b->d_out = grown;
     uint64_t off = old_out;
-    for (uint32_t w : retry) {
-      b->windows[w].out_off = off;
-      b->windows[w].out_cap = b->windows[w].node_bound;
-      off += rh::align_up(b->windows[w].node_bound, 16);
-      RH_HIP_CHECK(hipMemcpy(b->d_windows.as<PoaWindow>() + w, &b->windows[w],
-                             sizeof(PoaWindow), hipMemcpyHostToDevice),
-                   RH_ERR_DEVICE);
+    for (const auto& ws : retry) {
+      for (uint32_t w : ws) {
+        b->windows[w].out_off = off;
+        b->windows[w].out_cap = b->windows[w].node_bound;
+        off += rh::align_up(b->windows[w].node_bound, 16);
+        RH_HIP_CHECK(hipMemcpy(b->d_windows.as<PoaWindow>() + w, &b->windows[w],
+                               sizeof(PoaWindow), hipMemcpyHostToDevice),
+                     RH_ERR_DEVICE);
+      }
     }
     b->out_bytes = off;
-    rc = run_pass(b, retry, big, resident, RH_POA_BATCHED, &kernel_ms);
-    if (rc != RH_OK) {
-      return rc;
+    std::vector<PassLaunch> passes;
+    std::vector<const std::vector<uint32_t>*> orders;
+    for (const auto& ws : retry) {
+      if (ws.empty()) continue;
+      PassPlan plan{};
+      if (!plan_pass(b, ws, 0, b->lds_limit, &plan)) {
+        continue;  // stays in its overflow status: rejected below
+      }
+      PassLaunch pl;
+      const int32_t rc = size_pass(b, ws.size(), plan, RH_POA_BATCHED, b->d_scratch.bytes, &pl);
+      if (rc != RH_OK) return rc;
+      passes.push_back(pl);
+      orders.push_back(&ws);
+    }
+    if (!passes.empty()) {
+      int32_t rc = launch_passes(b, orders, passes, RH_POA_BATCHED);
+      if (rc != RH_OK) return rc;
+      rc = finish_passes(b, &kernel_ms);
+      if (rc != RH_OK) return rc;
     }
     RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
                            hipMemcpyDeviceToHost),
@@ -560,17 +696,47 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   }
   b->stats.kernel_ms = kernel_ms;
   b->stats.cells = 0;
-  for (uint32_t w : b->gpu_order) {
+  for (uint32_t w = 0; w < n; ++w) {
+    if (b->shallow[w]) continue;
     const PoaResult& r = b->results[w];
-    if (r.status != rh::kPoaPolished && r.status != rh::kPoaPolishedUntrimmed) {
-      rh::set_error("rh_poa_run: window " + std::to_string(w) + " failed on the device (status " +
-                    std::to_string(r.status) + ")");
-      return RH_ERR_KERNEL;
+    if (r.status == rh::kPoaPolished || r.status == rh::kPoaPolishedUntrimmed) {
+      b->stats.cells += r.cells;
+    } else {
+      b->stats.rejected++;  // outside the device limits: racon's CPU path takes it
     }
-    b->stats.cells += r.cells;
   }
   b->ran = true;
   return RH_OK;
+}
+
+int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
+  const int32_t rc = rh_poa_submit(b, mode);
+  if (rc != RH_OK) {
+    if (b != nullptr) b->submitted = false;
+    return rc;
+  }
+  return rh_poa_wait(b);
+}
+
+int32_t rh_poa_capacity(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes) {
+  rh::clear_error();
+  if (free_bytes == nullptr || total_bytes == nullptr) {
+    rh::set_error("rh_poa_capacity: null argument");
+    return RH_ERR_ARGUMENT;
+  }
+  RH_HIP_CHECK(hipSetDevice(device), RH_ERR_DEVICE);
+  size_t f = 0, t = 0;
+  RH_HIP_CHECK(hipMemGetInfo(&f, &t), RH_ERR_DEVICE);
+  *free_bytes = f;
+  *total_bytes = t;
+  return RH_OK;
+}
+
+uint64_t rh_poa_device_bytes(const rh_poa_batch* b) {
+  if (b == nullptr) return 0;
+  return b->d_arena.bytes + b->d_windows.bytes + b->d_layers.bytes + b->d_out.bytes +
+         b->d_results.bytes + b->d_order.bytes + b->d_queue.bytes + b->d_scratch.bytes +
+         b->d_params.bytes;
 }
 
 uint64_t rh_poa_output_bound(const rh_poa_batch* b) {
@@ -579,7 +745,8 @@ uint64_t rh_poa_output_bound(const rh_poa_batch* b) {
   }
   uint64_t total = 0;
   for (size_t i = 0; i < b->windows.size(); ++i) {
-    total += b->shallow[i] ? b->backbone_len[i] : b->windows[i].out_cap;
+    const bool host_rejected = b->results[i].status == rh::kPoaRejectedOnHost;
+    total += b->shallow[i] || host_rejected ? b->backbone_len[i] : b->windows[i].out_cap;
   }
   return total;
 }
@@ -607,10 +774,16 @@ int32_t rh_poa_download(rh_poa_batch* b, char* out, uint64_t capacity, uint64_t*
     const uint8_t* src;
     uint32_t len;
     uint8_t st;
+    const int32_t rs = b->results[i].status;
     if (b->shallow[i]) {
       src = b->arena.data() + b->backbone_off[i];
       len = b->backbone_len[i];
       st = RH_WINDOW_UNPOLISHED;
+    } else if (rs != rh::kPoaPolished && rs != rh::kPoaPolishedUntrimmed) {
+      // outside the device limits: the backbone, for the caller's CPU path
+      src = b->arena.data() + b->backbone_off[i];
+      len = b->backbone_len[i];
+      st = RH_WINDOW_REJECTED;
     } else {
       src = dev_out.data() + b->windows[i].out_off;
       len = b->results[i].length;

@@ -83,9 +83,13 @@ class PolishEngine {
   }
 
   // Generates the consensus of every window (set through
-  // Window::set_consensus) and fills status[i] with kWindow* codes.
+  // Window::set_consensus) and fills status[i] with kWindow* codes. ready(n)
+  // is called (from the calling thread, n increasing, the last call with
+  // windows.size()) once windows [0, n) and their status are final, so the
+  // caller can emit finished targets while later windows still run.
   virtual void generate_consensus(std::vector<std::shared_ptr<Window>>& windows,
-                                  bool trim, std::vector<uint8_t>& status) = 0;
+                                  bool trim, std::vector<uint8_t>& status,
+                                  const std::function<void(uint64_t)>& ready) = 0;
 };
 
 // Defined once per executable: the product links the HIP engine, the oracle

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -38,6 +39,8 @@ class HipEngine : public PolishEngine {
     rh_align_batch* batch = nullptr;
     int32_t rc = rh_align_create(0, &batch);
     if (rc != RH_OK) die("rh_align_create", rc);
+    rc = rh_align_set_band(batch, config_.aligner_band_width);
+    if (rc != RH_OK) die("rh_align_set_band", rc);
     // chunks of about 1 GiB of sequence
     const uint64_t kChunkBytes = 1ull << 30;
     uint64_t first = 0;
@@ -67,6 +70,14 @@ class HipEngine : public PolishEngine {
                              distance.data());
       if (rc != RH_OK) die("rh_align_download", rc);
       for (uint64_t i = 0; i < pairs.size(); ++i) {
+        if (distance[i] < 0) {
+          fprintf(stderr,
+                  "[racon::HipEngine] error: overlap %lu (%u x %u) could not be aligned on the "
+                  "device; no CPU path in this build\n",
+                  static_cast<unsigned long>(first + i), pairs[i].query_length,
+                  pairs[i].target_length);
+          exit(1);
+        }
         done(first + i, ops.data() + offsets[i], n_ops[i]);
       }
       first = last;
@@ -82,6 +93,8 @@ class HipEngine : public PolishEngine {
     rh_align_batch* batch = nullptr;
     int32_t rc = rh_align_create(0, &batch);
     if (rc != RH_OK) die("rh_align_create", rc);
+    rc = rh_align_set_band(batch, config_.aligner_band_width);
+    if (rc != RH_OK) die("rh_align_set_band", rc);
     const uint64_t kChunkBytes = 1ull << 30;
     uint64_t first = 0;
     std::vector<rh_pair> pairs;
@@ -112,6 +125,14 @@ class HipEngine : public PolishEngine {
                                     counts.data(), points.data(), points.size() / 2);
       if (rc != RH_OK) die("rh_align_breaking_points", rc);
       for (uint32_t i = 0; i < n; ++i) {
+        if (counts[i] == RH_PAIR_REJECTED) {
+          fprintf(stderr,
+                  "[racon::HipEngine] error: overlap %lu (%u x %u) could not be aligned on the "
+                  "device; no CPU path in this build\n",
+                  static_cast<unsigned long>(first + i), pairs[i].query_length,
+                  pairs[i].target_length);
+          exit(1);
+        }
         done(first + i, points.data() + 2 * offsets[i], counts[i]);
       }
       first = last;
@@ -120,69 +141,110 @@ class HipEngine : public PolishEngine {
     return true;
   }
 
+  // Windows go to the device in chunks of about 1 GiB of sequence, on two
+  // batch objects: while one chunk's kernel runs, the next chunk is packed
+  // and copied on the other batch's stream. Results are set in window order
+  // and reported through ready() chunk by chunk.
   void generate_consensus(std::vector<std::shared_ptr<Window>>& windows, bool trim,
-                          std::vector<uint8_t>& status) override {
+                          std::vector<uint8_t>& status,
+                          const std::function<void(uint64_t)>& ready) override {
     status.assign(windows.size(), kWindowUnpolished);
-    rh_poa_batch* batch = nullptr;
-    int32_t rc = rh_poa_create(0, config_.match, config_.mismatch, config_.gap, trim ? 1 : 0,
-                               &batch);
-    if (rc != RH_OK) die("rh_poa_create", rc);
     const uint64_t kChunkBytes = 1ull << 30;
-    std::vector<rh_window> batch_windows;
-    std::vector<std::vector<rh_layer>> layers;
+    std::vector<std::pair<uint64_t, uint64_t>> chunks;
+    for (uint64_t first = 0; first < windows.size();) {
+      uint64_t last = first, bytes = 0;
+      while (last < windows.size() && (last == first || bytes < kChunkBytes)) {
+        const Window& w = *windows[last];
+        for (uint32_t k = 0; k < w.size(); ++k) bytes += 2ull * w.sequence(k).second;
+        ++last;
+      }
+      chunks.emplace_back(first, last);
+      first = last;
+    }
+    struct Slot {
+      rh_poa_batch* batch = nullptr;
+      std::vector<rh_window> windows;
+      std::vector<rh_layer> layers;
+    };
+    Slot slots[2];
+    const uint32_t n_batches = chunks.size() > 1 ? 2 : 1;
+    for (uint32_t k = 0; k < n_batches; ++k) {
+      const int32_t rc = rh_poa_create(0, config_.match, config_.mismatch, config_.gap,
+                                       trim ? 1 : 0, &slots[k].batch);
+      if (rc != RH_OK) die("rh_poa_create", rc);
+    }
+    auto upload = [&](uint64_t c) {
+      Slot& sl = slots[c % n_batches];
+      const uint64_t first = chunks[c].first, last = chunks[c].second;
+      sl.windows.clear();
+      sl.layers.clear();
+      for (uint64_t i = first; i < last; ++i) {
+        const Window& w = *windows[i];
+        for (uint32_t k = 1; k < w.size(); ++k) {
+          sl.layers.push_back(rh_layer{w.sequence(k).first, w.quality(k).first,
+                                       w.sequence(k).second, w.position(k).first,
+                                       w.position(k).second});
+        }
+      }
+      uint64_t at = 0;
+      for (uint64_t i = first; i < last; ++i) {
+        const Window& w = *windows[i];
+        sl.windows.push_back(rh_window{w.sequence(0).first, w.quality(0).first,
+                                       w.sequence(0).second, w.size() - 1,
+                                       sl.layers.data() + at,
+                                       w.type() == WindowType::kTGS ? 1u : 0u});
+        at += w.size() - 1;
+      }
+      return rh_poa_upload(sl.batch, static_cast<uint32_t>(sl.windows.size()),
+                           sl.windows.data());
+    };
     std::vector<char> out;
     std::vector<uint64_t> offsets;
     std::vector<uint32_t> lengths;
     std::vector<uint8_t> codes;
-    uint64_t first = 0;
-    while (first < windows.size()) {
-      uint64_t last = first, bytes = 0;
-      batch_windows.clear();
-      layers.clear();
-      while (last < windows.size() && (last == first || bytes < kChunkBytes)) {
-        const Window& w = *windows[last];
-        std::vector<rh_layer> ls;
-        ls.reserve(w.size() - 1);
-        for (uint32_t k = 1; k < w.size(); ++k) {
-          ls.push_back(rh_layer{w.sequence(k).first, w.quality(k).first, w.sequence(k).second,
-                                w.position(k).first, w.position(k).second});
-          bytes += 2ull * w.sequence(k).second;
-        }
-        bytes += 2ull * w.sequence(0).second;
-        layers.push_back(std::move(ls));
-        ++last;
-      }
-      for (uint64_t i = first; i < last; ++i) {
-        const Window& w = *windows[i];
-        const auto& ls = layers[i - first];
-        batch_windows.push_back(rh_window{w.sequence(0).first, w.quality(0).first,
-                                          w.sequence(0).second,
-                                          static_cast<uint32_t>(ls.size()), ls.data(),
-                                          w.type() == WindowType::kTGS ? 1u : 0u});
-      }
-      rc = rh_poa_upload(batch, static_cast<uint32_t>(batch_windows.size()),
-                         batch_windows.data());
-      if (rc != RH_OK) die("rh_poa_upload", rc);
-      rc = rh_poa_run(batch, RH_POA_BATCHED);
-      if (rc != RH_OK) die("rh_poa_run", rc);
+    int32_t rc = chunks.empty() ? RH_OK : upload(0);
+    if (rc != RH_OK) die("rh_poa_upload", rc);
+    for (uint64_t c = 0; c < chunks.size(); ++c) {
+      rh_poa_batch* batch = slots[c % n_batches].batch;
+      // the kernel of chunk c on a helper thread, the next upload here
+      int32_t run_rc = RH_OK;
+      std::thread runner([&] { run_rc = rh_poa_run(batch, RH_POA_BATCHED); });
+      int32_t up_rc = RH_OK;
+      if (c + 1 < chunks.size()) up_rc = upload(c + 1);
+      runner.join();
+      if (run_rc != RH_OK) die("rh_poa_run", run_rc);
+      if (up_rc != RH_OK) die("rh_poa_upload", up_rc);
+      const uint64_t first = chunks[c].first, last = chunks[c].second;
+      const uint64_t n = last - first;
       out.resize(rh_poa_output_bound(batch));
-      offsets.resize(batch_windows.size());
-      lengths.resize(batch_windows.size());
-      codes.resize(batch_windows.size());
+      offsets.resize(n);
+      lengths.resize(n);
+      codes.resize(n);
       rc = rh_poa_download(batch, out.data(), out.size(), offsets.data(), lengths.data(),
                            codes.data());
       if (rc != RH_OK) die("rh_poa_download", rc);
       for (uint64_t i = first; i < last; ++i) {
         const uint64_t k = i - first;
+        if (codes[k] == RH_WINDOW_REJECTED) {
+          // racon's CUDA path would polish it on the CPU; this binary has
+          // only the device path, so it stops rather than print the backbone
+          fprintf(stderr,
+                  "[racon::HipEngine] error: window %lu of target %lu (rank %u, %u layers) is "
+                  "outside the device limits; no CPU path in this build\n",
+                  static_cast<unsigned long>(i), static_cast<unsigned long>(windows[i]->id()),
+                  windows[i]->rank(), windows[i]->size() - 1);
+          exit(1);
+        }
         windows[i]->set_consensus(std::string(out.data() + offsets[k], lengths[k]));
         status[i] = codes[k] == RH_WINDOW_UNPOLISHED
                         ? kWindowUnpolished
                         : (codes[k] == RH_WINDOW_POLISHED_UNTRIMMED ? kWindowPolishedChimeric
                                                                    : kWindowPolished);
       }
-      first = last;
+      if (ready) ready(last);
     }
-    rh_poa_destroy(batch);
+    if (ready && chunks.empty()) ready(0);
+    for (uint32_t k = 0; k < n_batches; ++k) rh_poa_destroy(slots[k].batch);
   }
 
  private:

@@ -122,7 +122,6 @@ int main(int argc, char** argv) {
     exit(1);
   }
 
-  std::vector<std::unique_ptr<racon::Sequence>> polished_sequences;
   racon::PolisherTimings timings;
   {
     auto polisher = racon::createPolisher(
@@ -139,13 +138,16 @@ int main(int argc, char** argv) {
       polisher->dump_windows();
       return 0;
     }
-    polisher->polish(polished_sequences, drop_unpolished_sequences);
+    // each target is written as soon as its windows are polished (the bytes
+    // of racon's output loop, main.cpp:164-166, in the same order)
+    polisher->polish_streamed(drop_unpolished_sequences,
+                              [](std::unique_ptr<racon::Sequence> s) {
+                                fprintf(stdout, ">%s\n%s\n", s->name().c_str(),
+                                        s->data().c_str());
+                              });
     timings = polisher->timings();
   }
-
-  for (const auto& it : polished_sequences) {
-    fprintf(stdout, ">%s\n%s\n", it->name().c_str(), it->data().c_str());
-  }
+  fflush(stdout);
 
   if (!report_path.empty()) {
     FILE* f = fopen(report_path.c_str(), "w");

@@ -519,45 +519,55 @@ void Polisher::find_overlap_breaking_points(
 
 void Polisher::polish(std::vector<std::unique_ptr<Sequence>>& dst,
                       bool drop_unpolished_sequences) {
-  // polisher.cpp:486-548
+  polish_streamed(drop_unpolished_sequences,
+                  [&](std::unique_ptr<Sequence> s) { dst.emplace_back(std::move(s)); });
+}
+
+void Polisher::polish_streamed(bool drop_unpolished_sequences,
+                               const std::function<void(std::unique_ptr<Sequence>)>& sink) {
+  // polisher.cpp:486-548, with the stitching done target by target as soon
+  // as the engine reports a target's windows final (same sequences, same
+  // order, same tags)
   logger_->log();
   auto t_consensus = std::chrono::steady_clock::now();
+  double stitch_s = 0;
 
   std::vector<uint8_t> status;
-  engine_->generate_consensus(windows_, trim_, status);
-  timings_.consensus = seconds_since(t_consensus);
-  auto t_stitch = std::chrono::steady_clock::now();
-
-  for (uint64_t i = 0; i < windows_.size(); ++i) {
-    if (status[i] == kWindowPolishedChimeric) {
-      windows_[i]->warn_chimeric();
-    }
-  }
-
   std::string polished_data;
   uint32_t num_polished_windows = 0;
-  for (uint64_t i = 0; i < windows_.size(); ++i) {
-    num_polished_windows += status[i] != kWindowUnpolished ? 1 : 0;
-    polished_data += windows_[i]->consensus();
-
-    if (i == windows_.size() - 1 || windows_[i + 1]->rank() == 0) {
-      double polished_ratio =
-          num_polished_windows / static_cast<double>(windows_[i]->rank() + 1);
-
-      if (!drop_unpolished_sequences || polished_ratio > 0) {
-        std::string tags = type_ == PolisherType::kF ? "r" : "";
-        tags += " LN:i:" + std::to_string(polished_data.size());
-        tags += " RC:i:" + std::to_string(targets_coverages_[windows_[i]->id()]);
-        tags += " XC:f:" + std::to_string(polished_ratio);
-        dst.emplace_back(createSequence(
-            sequences_[windows_[i]->id()]->name() + tags, polished_data));
+  uint64_t next = 0;  // first window not yet stitched
+  auto stitch = [&](uint64_t ready) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (; next < ready; ++next) {
+      const uint64_t i = next;
+      if (status[i] == kWindowPolishedChimeric) {
+        windows_[i]->warn_chimeric();
       }
-      num_polished_windows = 0;
-      polished_data.clear();
+      num_polished_windows += status[i] != kWindowUnpolished ? 1 : 0;
+      polished_data += windows_[i]->consensus();
+
+      if (i == windows_.size() - 1 || windows_[i + 1]->rank() == 0) {
+        double polished_ratio =
+            num_polished_windows / static_cast<double>(windows_[i]->rank() + 1);
+
+        if (!drop_unpolished_sequences || polished_ratio > 0) {
+          std::string tags = type_ == PolisherType::kF ? "r" : "";
+          tags += " LN:i:" + std::to_string(polished_data.size());
+          tags += " RC:i:" + std::to_string(targets_coverages_[windows_[i]->id()]);
+          tags += " XC:f:" + std::to_string(polished_ratio);
+          sink(createSequence(sequences_[windows_[i]->id()]->name() + tags, polished_data));
+        }
+        num_polished_windows = 0;
+        polished_data.clear();
+      }
+      windows_[i].reset();
     }
-    windows_[i].reset();
-  }
-  timings_.stitch = seconds_since(t_stitch);
+    stitch_s += seconds_since(t0);
+  };
+  engine_->generate_consensus(windows_, trim_, status, stitch);
+  stitch(windows_.size());
+  timings_.stitch = stitch_s;
+  timings_.consensus = seconds_since(t_consensus) - stitch_s;
 
   logger_->log("[racon::Polisher::polish] generated consensus");
 

@@ -68,6 +68,11 @@ class Polisher {
   virtual void initialize();
   virtual void polish(std::vector<std::unique_ptr<Sequence>>& dst,
                       bool drop_unpolished_sequences);
+  // polish() with every finished target handed to sink as soon as its
+  // windows are done (racon's stitching, src/polisher.cpp:506-538, streamed
+  // per target): the same sequences in the same order.
+  void polish_streamed(bool drop_unpolished_sequences,
+                       const std::function<void(std::unique_ptr<Sequence>)>& sink);
 
   const PolisherTimings& timings() const { return timings_; }
 

@@ -91,3 +91,30 @@ def test_target_side_other_bytes(length):
         (q, t.lower() if length == 600 else _plant(t, [63, 127], b"acgt")),
     ]
     check(pairs)
+
+
+@pytest.mark.parametrize("band", [0, 1, 37, 100000])
+def test_band_guess_does_not_change_the_path(band):
+    # the first band guess of a pair doubles until the banded distance fits
+    # it (exact from then on): a guess of 1 forces many doublings, a huge one
+    # computes the whole matrix; every path must be the oracle's
+    rng = np.random.default_rng(200 + band)
+    pairs = [make_pair(rng, int(n), error=0.12) for n in (90, 1500, 5000, 9000)]
+    t = random_seq(rng, 6000)
+    pairs.append((mutate(rng, t[:4500], 0.2), t))          # length difference 1500+
+    pairs.append((random_seq(rng, 2500), random_seq(rng, 3000)))  # unrelated: d ~ 0.55 L
+    pairs.append((mutate(rng, t, 0.3), t))
+    b = _lib.AlignBatch()
+    b.set_band(band)
+    b.upload(pairs)
+    b.run()
+    ops, dist = b.download()
+    st = b.stats()
+    b.close()
+    assert st.rejected == 0
+    full = sum(((len(q) + 63) // 64 + 63) // 64 * (len(t) + 63) for q, t in pairs)
+    print(f"band {band}: {st.steps} wave steps ({full} per unbanded pass)", flush=True)
+    for i, (q, t) in enumerate(pairs):
+        d, oops, _ = oracle_py.edlib_align(q, t)
+        assert dist[i] == d, f"pair {i} distance"
+        assert ops[i] == oops, f"pair {i} path"

@@ -143,3 +143,28 @@ def test_small_dp_row_capacity_reruns():
         bb, bq, layers, tgs = oracle_args(w)
         cons, ost, _, _ = oracle_py.window_consensus(bb, bq, layers, tgs, True, 3, -5, -4)
         assert (out[i], status[i]) == (cons, ost), f"window {i}"
+
+
+def test_submit_wait_two_batches_in_flight():
+    # rh_poa_submit queues a batch's kernels on its own stream; two batches
+    # in flight at once, waited for in the other order, give the oracle's
+    # consensus; a running batch refuses another upload
+    wa = make_windows(70, 12, length=500, depth=20)
+    wb = make_windows(71, 9, length=400, depth=15, quality=False)
+    ba = _lib.PoaBatch(3, -5, -4, trim=True)
+    bb = _lib.PoaBatch(3, -5, -4, trim=True)
+    ba.upload(wa)
+    bb.upload(wb)
+    ba.submit()
+    bb.submit()
+    with pytest.raises(RuntimeError):
+        ba.upload(wa)
+    bb.wait()
+    ba.wait()
+    for batch, wins in ((ba, wa), (bb, wb)):
+        out, status = batch.download()
+        assert batch.device_bytes() > 0
+        batch.close()
+        for i, w in enumerate(wins):
+            cons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
+            assert (out[i], status[i]) == (cons, ost), f"window {i}"
