@@ -266,6 +266,52 @@ int32_t rh_align_breaking_points(rh_align_batch* batch, uint32_t window_length,
                                  const rh_break_span* spans, uint64_t* offsets, uint32_t* counts,
                                  uint32_t* points, uint64_t capacity);
 
+/* ---- allocation-free overlap alignment ----
+ *
+ * No counterpart in racon. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the kernels of rh_align_run with every input and output in
+ * caller-owned device memory, and the kernels' scratch (per-wave slots, the
+ * work-queue counter) in one fixed static device array of
+ * RH_NOALLOC_SCRATCH_BYTES, which each device the library runs on holds
+ * until the process ends. The call allocates nothing; it launches on
+ * `stream` (a hipStream_t; NULL = the null stream) and returns without
+ * waiting. Every call on one device shares that device's scratch: calls that
+ * overlap in time on one device race, so order them on one stream.
+ *
+ * pairs[i] (device memory): the query and target at arena + query_offset /
+ * arena + target_offset (edlib's orientation, as rh_pair), and room for
+ * query_length + target_length ops at ops + ops_offset. max_query_length and
+ * max_target_length bound every pair; they size the slots, and a pair
+ * beyond them ends with status RH_ALIGN_PAIR_TOO_LONG. Sizes whose slot
+ * does not fit the scratch are refused with RH_ERR_ARGUMENT (about 40 kbp x
+ * 40 kbp pairs take a 2.3 MiB slot). results[i] (device memory) receives the
+ * op count, the edit distance and a status (0 = aligned). Ops are coded as
+ * in rh_align_download. */
+#define RH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+enum { RH_ALIGN_PAIR_OK = 0, RH_ALIGN_PAIR_TOO_LONG = -3 };
+
+typedef struct {
+  uint64_t query_offset;
+  uint64_t target_offset;
+  uint64_t ops_offset;
+  uint32_t query_length;
+  uint32_t target_length;
+} rh_dev_pair;
+
+typedef struct {
+  uint32_t n_ops;
+  int32_t edit_distance;
+  int32_t status;
+  uint32_t reserved;
+} rh_dev_align_result;
+
+int32_t rh_align_run_noalloc(int32_t device, void* stream, uint32_t n, const rh_dev_pair* pairs,
+                             const uint8_t* arena, uint32_t max_query_length,
+                             uint32_t max_target_length, uint8_t* ops,
+                             rh_dev_align_result* results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -147,7 +147,18 @@ _SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_uint32),
                                                   ctypes.POINTER(ctypes.c_uint32),
                                                   ctypes.c_uint64]),
+    # device pointers are passed as integers (c_void_p)
+    "rh_align_run_noalloc": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
 }
+
+# rh_dev_pair / rh_dev_align_result of include/racon_hip.h as numpy dtypes
+DEV_PAIR_FIELDS = [("query_offset", "<u8"), ("target_offset", "<u8"), ("ops_offset", "<u8"),
+                   ("query_length", "<u4"), ("target_length", "<u4")]
+DEV_ALIGN_RESULT_FIELDS = [("n_ops", "<u4"), ("edit_distance", "<i4"), ("status", "<i4"),
+                           ("reserved", "<u4")]
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _lib: Optional[ctypes.CDLL] = None
 
