@@ -622,21 +622,35 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
   return best_row;
 }
 
+// Strip rows in LDS (the one-pass DP below) are laid out quad-major: quad q
+// (strip columns j0 + 4q .. j0 + 4q + 3) of lane k sits at element
+// jq + q * qs with jq = 4k and qs = 4 * (lanes with columns <= L), so that
+// consecutive lanes touch consecutive 8-byte quads (no LDS bank conflicts);
+// column 0 sits at the row's last element. H[p][j0 - 1], the last column
+// of the previous lane's strip, comes from that lane with a DPP shift.
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t x) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x138, 0xf, 0xf, false));  // wave_shr:1
+}
+
 // Predecessor row p (entry e) at the lane's strip: hp[u] = H[p][j0 - 1 + u],
 // u = 0..W. Row 0 is the virtual start row (j * gap).
 template <int W>
 __device__ __forceinline__ void load_strip(const int16_t* ring, int pitch, const Slot& s, int e,
-                                           int j0, int jr, int L, int32_t gap, int32_t* hp) {
+                                           int j0, int jq, int qs, int L, int32_t gap,
+                                           int32_t* hp) {
   const int p = e & kPredRowMask;
   const int sl = static_cast<unsigned>(e) >> kPredRowBits;
   if (p == 0) {
 #pragma unroll
     for (int u = 0; u <= W; ++u) hp[u] = (j0 - 1 + u) * gap;
   } else if (sl < kSlotGlobal) {
-    const int16_t* row = ring + sl * pitch + kColShift;
-    hp[0] = row[jr - 1];
+    const int16_t* row = ring + sl * pitch;
 #pragma unroll
-    for (int q = 0; q < W / 4; ++q) Quad<int16_t>::load(row + jr + 4 * q, hp + 1 + 4 * q);
+    for (int q = 0; q < W / 4; ++q) Quad<int16_t>::load(row + jq + q * qs, hp + 1 + 4 * q);
+    const int32_t c0 = row[pitch - 1];
+    const int32_t prev = static_cast<int32_t>(from_prev_lane(static_cast<uint32_t>(hp[W])));
+    hp[0] = lane_id() == 0 ? c0 : prev;
   } else {
     const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
 #pragma unroll
@@ -655,8 +669,8 @@ __device__ __forceinline__ void load_strip(const int16_t* ring, int pitch, const
 // (key or score c0, code vertical). Returns H[row][L].
 template <int W, bool kKeys>
 __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* bv, int32_t c0,
-                                              uint32_t pay0, int lane, int j0, bool live,
-                                              int L, int32_t gap, int own, int pitch,
+                                              uint32_t pay0, int lane, int j0, int jq, int qs,
+                                              bool live, int L, int32_t gap, int own, int pitch,
                                               int16_t* ring, int32_t* Hg, uint16_t* Cr,
                                               int end_lane, int end_u) {
   auto score = [](int32_t x) { return kKeys ? (x >> 8) : x; };
@@ -692,9 +706,9 @@ __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* 
       *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = v;
     }
     if (own < kSlotGlobal) {
-      int16_t* row = ring + own * pitch + kColShift;
+      int16_t* row = ring + own * pitch;
 #pragma unroll
-      for (int q = 0; q < W / 4; ++q) Quad<int16_t>::store(row + j0 + 4 * q, h + 4 * q);
+      for (int q = 0; q < W / 4; ++q) Quad<int16_t>::store(row + jq + q * qs, h + 4 * q);
     }
   }
   if (own == kSlotGlobal) {
@@ -704,7 +718,7 @@ __device__ __forceinline__ int32_t finish_row(const int32_t* bd, const int32_t* 
     }
   }
   if (lane == 0) {
-    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(h0);
+    if (own < kSlotGlobal) ring[own * pitch + pitch - 1] = static_cast<int16_t>(h0);
     if (own == kSlotGlobal) Hg[0] = h0;
     Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | pred(c0));
   }
@@ -796,7 +810,7 @@ __device__ __forceinline__ void packed_strip_init(PackedStrip<W>& ps, const uint
 // virtual start row (H = j * gap, so G = 0).
 template <int W>
 __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, const Slot& s, int e,
-                                               int j0, int jr, int L, int32_t gap,
+                                               int j0, int jq, int qs, int L, int32_t gap,
                                                const PackedStrip<W>& ps, uint32_t* pp) {
   const int p = e & kPredRowMask;
   const int sl = static_cast<unsigned>(e) >> kPredRowBits;
@@ -805,14 +819,16 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
     for (int q = 0; q < W / 2; ++q) pp[q] = 0u;
     return 0u;
   } else if (sl < kSlotGlobal) {
-    const int16_t* row = ring + sl * pitch + kColShift;
+    const int16_t* row = ring + sl * pitch;
 #pragma unroll
     for (int q = 0; q < W / 4; ++q) {
-      const uint2 v = *reinterpret_cast<const uint2*>(row + jr + 4 * q);
+      const uint2 v = *reinterpret_cast<const uint2*>(row + jq + q * qs);
       pp[2 * q] = v.x;
       pp[2 * q + 1] = v.y;
     }
-    return static_cast<uint16_t>(row[jr - 1]);
+    const uint32_t c0 = static_cast<uint16_t>(row[pitch - 1]);
+    const uint32_t prev = from_prev_lane(pp[W / 2 - 1]) >> 16;
+    return lane_id() == 0 ? c0 : prev;
   } else {
     const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
     int32_t hv[W + 1];
@@ -893,9 +909,10 @@ __device__ __forceinline__ void packed_compute(const PackedStrip<W>& ps, const u
 // them) and H[row][L] as the return value.
 template <int W, bool kMulti>
 __device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t k0, uint32_t pay0,
-                                                int lane, int j0, bool live, int L, int32_t gap,
-                                                int own, int pitch, int16_t* ring, int32_t* Hg,
-                                                uint16_t* Cr, int end_lane, int end_u) {
+                                                int lane, int j0, int jq, int qs, bool live, int L,
+                                                int32_t gap, int own, int pitch, int16_t* ring,
+                                                int32_t* Hg, uint16_t* Cr, int end_lane,
+                                                int end_u) {
   constexpr int H2 = W / 2;
   if (live) {
 #pragma unroll
@@ -903,10 +920,10 @@ __device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t 
       *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(o.cd[2 * q], o.cd[2 * q + 1]);
     }
     if (own < kSlotGlobal) {
-      int16_t* row = ring + own * pitch + kColShift;
+      int16_t* row = ring + own * pitch;
 #pragma unroll
       for (int q = 0; q < W / 4; ++q) {
-        *reinterpret_cast<uint2*>(row + j0 + 4 * q) = make_uint2(o.h[2 * q], o.h[2 * q + 1]);
+        *reinterpret_cast<uint2*>(row + jq + q * qs) = make_uint2(o.h[2 * q], o.h[2 * q + 1]);
       }
     }
   }
@@ -917,7 +934,7 @@ __device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t 
     }
   }
   if (lane == 0) {
-    if (own < kSlotGlobal) ring[own * pitch + kColShift] = static_cast<int16_t>(o.h0);
+    if (own < kSlotGlobal) ring[own * pitch + pitch - 1] = static_cast<int16_t>(o.h0);
     if (own == kSlotGlobal) Hg[0] = o.h0;
     Cr[pitch - 1] = static_cast<uint16_t>(
         kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu)) : (kCodeVert | pay0));
@@ -953,13 +970,14 @@ template <int W, bool kMulti>
 __device__ __forceinline__ int32_t packed_row(const PackedStrip<W>& ps, const uint32_t* pp,
                                               uint32_t hp0, const uint32_t* kk, uint32_t k0,
                                               uint32_t b, uint32_t pay0, int lane,
-                                              int j0, bool live, int L, int32_t gap, int own,
-                                              int pitch, int16_t* ring, int32_t* Hg, uint16_t* Cr,
-                                              int end_lane, int end_u) {
+                                              int j0, int jq, int qs, bool live, int L,
+                                              int32_t gap, int own, int pitch, int16_t* ring,
+                                              int32_t* Hg, uint16_t* Cr, int end_lane,
+                                              int end_u) {
   PackedOut<W> o;
   packed_compute<W, kMulti>(ps, pp, hp0, kk, k0, b, pay0, lane, gap, o);
-  return packed_store<W, kMulti>(o, k0, pay0, lane, j0, live, L, gap, own, pitch, ring, Hg, Cr,
-                                 end_lane, end_u);
+  return packed_store<W, kMulti>(o, k0, pay0, lane, j0, jq, qs, live, L, gap, own, pitch, ring,
+                                 Hg, Cr, end_lane, end_u);
 }
 
 // DP rows of one layer with 16-bit scores, every row in one pass of the
@@ -991,6 +1009,11 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
   const int j0 = 1 + lane * W;
   const bool live = j0 <= L;
   const int jr = live ? j0 : 1;  // idle lanes read columns 0..W (in bounds)
+  // quad-major LDS rows (see from_prev_lane): this lane's first quad and the
+  // quad stride; idle lanes read lane 0's quads
+  const int n_live = (L + W - 1) / W;
+  const int jq = (live ? lane : 0) * 4;
+  const int qs = n_live * 4;
   uint32_t sw[W / 4];            // layer bytes of the strip
 #pragma unroll
   for (int q = 0; q < W / 4; ++q) sw[q] = *reinterpret_cast<const uint32_t*>(seq + (jr - 1) + 4 * q);
@@ -1088,16 +1111,16 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
               const uint32_t delta2 = static_cast<uint32_t>(r + 1 - (e2 & kPredRowMask));
               const uint32_t pay2 = delta2 <= kCodePayload ? delta2 : kCodeViaK;
               uint32_t pa[W / 2], pb[W / 2];
-              const uint32_t ha0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pa);
-              const uint32_t hb0 = load_pairs<W>(ring, pitch, s, e2, j0, jr, L, gap, ps, pb);
+              const uint32_t ha0 = load_pairs<W>(ring, pitch, s, e, j0, jq, qs, L, gap, ps, pa);
+              const uint32_t hb0 = load_pairs<W>(ring, pitch, s, e2, j0, jq, qs, L, gap, ps, pb);
               PackedOut<W> oa, ob;
               packed_compute<W, false>(ps, pa, ha0, pa, 0u, b, pay0, lane, gap, oa);
               packed_compute<W, false>(ps, pb, hb0, pb, 0u, static_cast<uint32_t>(meta2 & 0xff),
                                        pay2, lane, gap, ob);
-              const int32_t h_a = packed_store<W, false>(oa, 0u, pay0, lane, j0, live, L, gap,
+              const int32_t h_a = packed_store<W, false>(oa, 0u, pay0, lane, j0, jq, qs, live, L, gap,
                                                          own, pitch, ring, Hg, Cr, end_lane,
                                                          end_u);
-              h_last = packed_store<W, false>(ob, 0u, pay2, lane, j0, live, L, gap, own2, pitch,
+              h_last = packed_store<W, false>(ob, 0u, pay2, lane, j0, jq, qs, live, L, gap, own2, pitch,
                                               ring, s.H() + static_cast<int64_t>(r + 1) * pitch,
                                               s.code() + static_cast<int64_t>(r + 1) * pitch,
                                               end_lane, end_u);
@@ -1111,21 +1134,21 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         }
         if (!paired) {
           uint32_t pp[W / 2];
-          const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jr, L, gap, ps, pp);
-          h_last = packed_row<W, false>(ps, pp, hp0, pp, 0u, b, pay0, lane, j0, live, L, gap,
+          const uint32_t hp0 = load_pairs<W>(ring, pitch, s, e, j0, jq, qs, L, gap, ps, pp);
+          h_last = packed_row<W, false>(ps, pp, hp0, pp, 0u, b, pay0, lane, j0, jq, qs, live, L, gap,
                                         own, pitch, ring, Hg, Cr, end_lane, end_u);
         }
       } else if (kPacked) {
         // best predecessor score of every column and the first predecessor
         // reaching it (a later one replaces it only when strictly better)
         uint32_t mx[W / 2], kx[W / 2];
-        uint32_t m0 = load_pairs<W>(ring, pitch, s, pred_at(0), j0, jr, L, gap, ps, mx);
+        uint32_t m0 = load_pairs<W>(ring, pitch, s, pred_at(0), j0, jq, qs, L, gap, ps, mx);
         uint32_t k0 = 0;
 #pragma unroll
         for (int q = 0; q < W / 2; ++q) kx[q] = 0;
         for (int k = 1; k < np; ++k) {
           uint32_t pp[W / 2];
-          const uint32_t hk = load_pairs<W>(ring, pitch, s, pred_at(k), j0, jr, L, gap, ps, pp);
+          const uint32_t hk = load_pairs<W>(ring, pitch, s, pred_at(k), j0, jq, qs, L, gap, ps, pp);
           const uint32_t ks = static_cast<uint32_t>(k) * 0x10001u;
 #pragma unroll
           for (int q = 0; q < W / 2; ++q) {
@@ -1137,20 +1160,20 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
           k0 = pk_mad(pk_min_u(pk_sub(m, m0), ps.one), pk_sub(ks, k0), k0);
           m0 = m;
         }
-        h_last = packed_row<W, true>(ps, mx, m0, kx, k0, b, 0u, lane, j0, live, L, gap, own, pitch,
+        h_last = packed_row<W, true>(ps, mx, m0, kx, k0, b, 0u, lane, j0, jq, qs, live, L, gap, own, pitch,
                                      ring, Hg, Cr, end_lane, end_u);
       } else if (np == 1) {
         // move-code payload of the single predecessor: its row delta
         const uint32_t delta = static_cast<uint32_t>(r - (pred_at(0) & kPredRowMask));
         const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
         int32_t hp[W + 1];
-        load_strip<W>(ring, pitch, s, pred_at(0), j0, jr, L, gap, hp);
+        load_strip<W>(ring, pitch, s, pred_at(0), j0, jq, qs, L, gap, hp);
 #pragma unroll
         for (int u = 0; u < W; ++u) {
           bd[u] = hp[u] + sc(u);
           bv[u] = hp[u + 1] + gap;
         }
-        h_last = finish_row<W, false>(bd, bv, hp[0] + gap, pay0, lane, j0, live, L, gap, own,
+        h_last = finish_row<W, false>(bd, bv, hp[0] + gap, pay0, lane, j0, jq, qs, live, L, gap, own,
                                       pitch, ring, Hg, Cr, end_lane, end_u);
       } else {
         int32_t c0k = kNegKey;
@@ -1161,7 +1184,7 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
         }
         for (int k = 0; k < np; ++k) {
           int32_t hp[W + 1];
-          load_strip<W>(ring, pitch, s, pred_at(k), j0, jr, L, gap, hp);
+          load_strip<W>(ring, pitch, s, pred_at(k), j0, jq, qs, L, gap, hp);
           const int32_t tag = 255 - k;
 #pragma unroll
           for (int u = 0; u < W; ++u) {
@@ -1170,7 +1193,7 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
           }
           c0k = max(c0k, (hp[0] + gap) * 256 | tag);
         }
-        h_last = finish_row<W, true>(bd, bv, c0k, 0u, lane, j0, live, L, gap, own, pitch, ring,
+        h_last = finish_row<W, true>(bd, bv, c0k, 0u, lane, j0, jq, qs, live, L, gap, own, pitch, ring,
                                      Hg, Cr, end_lane, end_u);
       }
       account(h_last, tail_row, tail_meta);
