@@ -290,6 +290,9 @@ def main() -> int:
     ap.add_argument("--kernel-only", action="store_true",
                     help="profiling runs: time only rh_poa_run (inputs resident), skip the "
                          "comparators and the CPU baseline")
+    ap.add_argument("--align-only", action="store_true",
+                    help="profiling runs: only the overlap aligner, `steps` launches over the "
+                         "config's overlaps (inputs resident)")
     ap.add_argument("--profile", default=os.path.join(ROOT, "profiles", "poa_cfg2_counters.txt"),
                     help="committed rocprofv3 summary with the POA kernel's HBM bytes per launch")
     ap.add_argument("--serving", default="", metavar="RATES",
@@ -340,6 +343,24 @@ def main() -> int:
     m, x, g = 3, -5, -4
     if args.serving:
         print(json.dumps(serving_bench(windows, args, (m, x, g))), flush=True)
+        return 0
+    if args.align_only:
+        ab = racon_amd.AlignBatch()
+        ab.upload(pairs)
+        times = []
+        for i in range(args.warmup + args.steps):
+            ab.run()
+            if i >= args.warmup:
+                times.append(ab.stats().kernel_ms)
+        ast = ab.stats()
+        ab.close()
+        ms = sum(times) / len(times)
+        log(f"align: {len(pairs)} overlaps, kernel {ms:.1f} ms per launch ({times})")
+        print(json.dumps({"metric": "overlap alignment kernel", "unit": "ms", "kernel_ms": ms,
+                          "pairs": len(pairs), "cells_full_matrix": int(ast.cells),
+                          "wave_steps": int(ast.steps),
+                          "gcups_full_matrix": round(ast.cells / (ms / 1e3) / 1e9, 1)}),
+              flush=True)
         return 0
 
     # the caller's rh_window array (racon's Window objects), built once
