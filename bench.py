@@ -172,7 +172,7 @@ def serving_bench(windows, args, scores) -> dict:
             n_req = max(6, int(lam * args.serving_seconds))
             reqs = [make_request() for _ in range(n_req)]
             gaps = [rng.expovariate(lam) for _ in range(n_req)]
-            launches0 = len(srv.launch_log)
+            st0 = srv.stats()
             handles = []
             t0 = time.perf_counter()
             due = t0
@@ -182,20 +182,24 @@ def serving_bench(windows, args, scores) -> dict:
                 handles.append(srv.submit(ws))
             for h in handles:
                 h.wait()
-            t1 = max(h.done for h in handles)
+            st1 = srv.stats()
+            first = min(h.arrival for h in handles)
+            span = max(h.done for h in handles) - first
             lat = [h.latency for h in handles]
             n_win = sum(len(ws) for ws in reqs)
-            log_slice = srv.launch_log[launches0:]
             row = {"rate": lam, "requests": n_req, "windows": n_win,
-                   "windows_per_s": round(n_win / (t1 - t0), 1),
+                   "windows_per_s": round(n_win / span, 1),
                    "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
                    "mean_wait_s": round(sum(h.admitted - h.arrival for h in handles) / n_req, 4),
-                   "launches": len(log_slice),
-                   "mixed_launches": sum(1 for _, _, k in log_slice if k > 1)}
+                   "launches": st1["launches"] - st0["launches"],
+                   "mixed_launches": st1["shared_launches"] - st0["shared_launches"],
+                   "kernel_s": round((st1["kernel_ms"] - st0["kernel_ms"]) / 1e3, 3)}
             log(f"serving rate {lam}/s: {n_req} requests, {n_win} windows, "
                 f"p50 {row['p50_s']} s, p99 {row['p99_s']} s, {row['windows_per_s']} windows/s, "
                 f"{row['mixed_launches']}/{row['launches']} launches mixed")
             report["rates"].append(row)
+        report["server"] = {k: srv.stats()[k] for k in ("pool_bytes", "scratch_bytes",
+                                                         "window_capacity")}
     return report
 
 
@@ -367,20 +371,14 @@ def main() -> int:
     n_windows = len(windows)
     arr, keep = pack_windows(windows)
 
-    # ---- autotune the LDS ring depth on the resident batch ----
+    # ---- the library's autotuner: LDS ring depth (and so waves per CU) per bin ----
     batch = racon_amd.PoaBatch(m, x, g, trim=True)
     batch.upload_packed(arr, n_windows, keep)
-    tuned = {}
-    best_ring = 0
+    tuned = None
     if not args.no_tune:
         batch.run(racon_amd.RH_POA_BATCHED)  # first launch: one-time costs
-        for ring in (0, 4, 8):
-            batch.set_tuning(ring_rows=ring)
-            batch.run(racon_amd.RH_POA_BATCHED)
-            tuned[ring] = batch.stats().kernel_ms
-            log(f"tune ring_rows={ring or 'auto'}: kernel {tuned[ring]:.1f} ms")
-        best_ring = min(tuned, key=tuned.get)
-        batch.set_tuning(ring_rows=best_ring)
+        tuned = batch.autotune()
+        log(f"autotune: {tuned}")
 
     # ---- the polish phase: pack + H2D + kernels + D2H per step ----
     bufs = batch.output_buffers(2 * batch.lib.rh_poa_output_bound(batch.handle))
@@ -540,8 +538,7 @@ def main() -> int:
         "poa_wave_busy_frac": round(busy_frac, 4),
         "poa_queue": sched,
         "poa_resident_waves": st.slots,
-        "autotune_kernel_ms": {str(k): round(v, 1) for k, v in tuned.items()},
-        "ring_rows": best_ring,
+        "autotune": tuned,
     }
     result.update(extra)
     print(json.dumps(result), flush=True)

@@ -169,6 +169,20 @@ int32_t rh_poa_get_stats(const rh_poa_batch* batch, rh_poa_stats* stats);
 int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows,
                           uint32_t ring_rows);
 
+/* Autotuner: times the uploaded batch's kernel per layer-length bin (main
+ * pass, long-layer pass) for each candidate depth of the DP row ring in LDS
+ * (which also sets the waves per CU: occupancy follows the LDS bytes) and
+ * keeps the fastest per bin for the batch's later runs. Results are the
+ * same for every setting; call rh_poa_run afterwards. No reference
+ * counterpart (GenomeWorks' BatchConfig is fixed, cudabatch.cpp:56-59). */
+typedef struct {
+  uint32_t ring_rows[2]; /* chosen ring rows per bin (main, long layers)      */
+  double kernel_ms[2];   /* kernel time of the chosen setting (0: empty bin)   */
+  double default_ms[2];  /* kernel time with the library default               */
+  uint32_t candidates;   /* settings timed                                     */
+} rh_poa_tuning;
+int32_t rh_poa_autotune(rh_poa_batch* batch, rh_poa_tuning* tuning);
+
 /* Profiling: device clock ticks of the last run summed over windows, per
  * kernel phase (subgraph set, subgraph order, DP row map, DP column 0, DP
  * rows, traceback, insertion, graph order, consensus, setup); n <= 10.
@@ -183,6 +197,85 @@ int32_t rh_poa_phase_ticks(const rh_poa_batch* batch, uint64_t* ticks, uint32_t 
  * studying the work queue's cost order. No reference counterpart. */
 int32_t rh_poa_window_costs(const rh_poa_batch* batch, uint64_t* cells, uint64_t* ticks,
                             uint64_t n);
+
+/* ---- serving: continuous batching of polishing requests ----
+ *
+ * No reference counterpart (racon polishes one job per process; its CUDA
+ * path splits 90% of free device memory over fixed batches,
+ * src/cuda/cudapolisher.cpp:228-240). A server owns a device pool of
+ * `budget_bytes` in which admitted requests stay resident (packed windows,
+ * layer records and consensus space, plus window and result records), and
+ * `executors` HIP streams, each running the persistent POA kernel over the
+ * next windows of the admitted requests in admission order (at most
+ * `launch_windows` per launch, largest estimated cost first). A request is
+ * admitted first come, first served when its bytes fit in the pool; requests
+ * that arrive while kernels run are admitted as soon as completed requests
+ * free their bytes and join the next launch of any executor, so kernels of
+ * several executors (and windows of several requests) share the device. A
+ * window's consensus depends on that window alone: a request's results are
+ * the same alone and in any mix. */
+typedef struct rh_server rh_server;
+
+enum { RH_TIMEOUT = 1 }; /* rh_server_wait: not done yet (not an error) */
+
+typedef struct {
+  uint32_t state;          /* 0 waiting, 1 admitted, 2 done               */
+  uint32_t windows;        /* windows of the request                      */
+  uint64_t device_bytes;   /* pool bytes the request holds while admitted */
+  uint64_t output_bound;   /* bytes rh_server_fetch needs                 */
+  uint32_t launches;       /* kernel launches that carried its windows    */
+  uint32_t shared_launches;/* ... of them shared with other requests      */
+  double arrival_s;        /* submit, seconds since the server started    */
+  double admitted_s;       /* admission (pool bytes reserved, uploaded)   */
+  double done_s;           /* last window back (0 while not done)         */
+} rh_request_info;
+
+typedef struct {
+  uint64_t pool_bytes;      /* budget: resident request bytes             */
+  uint64_t pool_in_use;     /* bytes held by admitted requests            */
+  uint64_t window_capacity; /* resident window records                    */
+  uint64_t scratch_bytes;   /* executors' scratch slots and launch buffers*/
+  uint64_t submitted, admitted, completed;
+  uint64_t launches, shared_launches, windows_launched, retried, rejected;
+  double kernel_ms;         /* summed device time of all launches         */
+} rh_server_stats;
+
+/* Creates a server on `device` with racon's scores and trim switch. */
+int32_t rh_server_create(int32_t device, int8_t match, int8_t mismatch, int8_t gap,
+                         int32_t trim, uint64_t budget_bytes, uint32_t executors,
+                         uint32_t launch_windows, rh_server** server);
+
+/* Sizes every executor's scratch for `n` windows like the largest expected
+ * (runs them once on each executor, outputs discarded), so that served
+ * launches do not grow (reallocate) device memory. Call before serving. */
+int32_t rh_server_warm(rh_server* server, uint32_t n, const rh_window* windows);
+
+/* Submits a request (windows as for rh_poa_upload; the inputs are copied
+ * before the call returns). RH_ERR_ARGUMENT if its bytes exceed the whole
+ * pool. The request id is written to *request. */
+int32_t rh_server_submit(rh_server* server, uint32_t n, const rh_window* windows,
+                         uint64_t* request);
+
+/* hold != 0: executors start no new launch (admission goes on) until called
+ * with hold = 0; for a caller that submits several requests at one instant
+ * and wants them in the same launches. */
+int32_t rh_server_hold(rh_server* server, int32_t hold);
+
+/* Waits up to timeout_ms (< 0: no limit) for the request: RH_OK when done,
+ * RH_TIMEOUT if not yet, or the error that ended it. */
+int32_t rh_server_wait(rh_server* server, uint64_t request, int32_t timeout_ms);
+
+int32_t rh_server_request_info(rh_server* server, uint64_t request, rh_request_info* info);
+
+/* Copies a done request's results (as rh_poa_download) and forgets it. */
+int32_t rh_server_fetch(rh_server* server, uint64_t request, char* out, uint64_t capacity,
+                        uint64_t* offsets, uint32_t* lengths, uint8_t* status);
+
+int32_t rh_server_get_stats(rh_server* server, rh_server_stats* stats);
+
+/* Finishes every submitted request, then stops the executors and frees the
+ * device memory. */
+void rh_server_destroy(rh_server* server);
 
 /* ---- (a) overlap alignment ---- */
 

@@ -12,6 +12,7 @@
 
 #include "edlib_cpu.hpp"
 #include "engine.hpp"
+#include "serve.hpp"
 #include "poa_cpu.hpp"
 #include "thread_pool.hpp"
 #include "window.hpp"
@@ -107,5 +108,13 @@ class CpuEngine : public PolishEngine {
 std::unique_ptr<PolishEngine> createEngine(const EngineConfig& config) {
   return std::unique_ptr<PolishEngine>(new CpuEngine(config));
 }
+
+// --serve runs jobs on the device serving loop; the oracle build has none.
+std::vector<ServeJob> read_jobs(const std::string&) {
+  fprintf(stderr, "[racon::serve] error: --serve needs the device build!\n");
+  exit(1);
+}
+
+int serve_jobs(const std::vector<ServeJob>&, const ServeOptions&) { return 1; }
 
 }  // namespace racon

@@ -98,6 +98,50 @@ class RhAlignStats(ctypes.Structure):
     ]
 
 
+class RhPoaTuning(ctypes.Structure):
+    _fields_ = [
+        ("ring_rows", ctypes.c_uint32 * 2),
+        ("kernel_ms", ctypes.c_double * 2),
+        ("default_ms", ctypes.c_double * 2),
+        ("candidates", ctypes.c_uint32),
+    ]
+
+
+RH_TIMEOUT = 1
+
+
+class RhRequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("state", ctypes.c_uint32),
+        ("windows", ctypes.c_uint32),
+        ("device_bytes", ctypes.c_uint64),
+        ("output_bound", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+        ("shared_launches", ctypes.c_uint32),
+        ("arrival_s", ctypes.c_double),
+        ("admitted_s", ctypes.c_double),
+        ("done_s", ctypes.c_double),
+    ]
+
+
+class RhServerStats(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_uint64),
+        ("pool_in_use", ctypes.c_uint64),
+        ("window_capacity", ctypes.c_uint64),
+        ("scratch_bytes", ctypes.c_uint64),
+        ("submitted", ctypes.c_uint64),
+        ("admitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("shared_launches", ctypes.c_uint64),
+        ("windows_launched", ctypes.c_uint64),
+        ("retried", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+
 # Every entry point of include/racon_hip.h with its ctypes signature.
 _SIGNATURES = {
     "rh_last_error": (ctypes.c_char_p, []),
@@ -123,10 +167,30 @@ _SIGNATURES = {
     "rh_poa_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaStats)]),
     "rh_poa_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.c_uint32]),
+    "rh_poa_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaTuning)]),
     "rh_poa_phase_ticks": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "rh_poa_window_costs": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]),
+    "rh_server_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int8, ctypes.c_int8,
+                                          ctypes.c_int8, ctypes.c_int32, ctypes.c_uint64,
+                                          ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "rh_server_warm": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                        ctypes.POINTER(RhWindow)]),
+    "rh_server_submit": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.POINTER(RhWindow),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "rh_server_hold": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "rh_server_wait": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32]),
+    "rh_server_request_info": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64,
+                                                ctypes.POINTER(RhRequestInfo)]),
+    "rh_server_fetch": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_char_p,
+                                         ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                         ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_uint8)]),
+    "rh_server_get_stats": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhServerStats)]),
+    "rh_server_destroy": (None, [ctypes.c_void_p]),
     "rh_align_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "rh_align_destroy": (None, [ctypes.c_void_p]),
     "rh_align_upload": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
@@ -322,6 +386,14 @@ class PoaBatch:
     def set_tuning(self, slots: int = 0, rows: int = 0, ring_rows: int = 0) -> None:
         _check(self.lib.rh_poa_set_tuning(self.handle, slots, rows, ring_rows),
                "rh_poa_set_tuning")
+
+    def autotune(self) -> dict:
+        """rh_poa_autotune on the uploaded batch: the fastest LDS ring depth
+        per layer-length bin, kept for this batch's later runs."""
+        t = RhPoaTuning()
+        _check(self.lib.rh_poa_autotune(self.handle, ctypes.byref(t)), "rh_poa_autotune")
+        return {"ring_rows": list(t.ring_rows), "kernel_ms": list(t.kernel_ms),
+                "default_ms": list(t.default_ms), "candidates": t.candidates}
 
     def upload(self, windows: Sequence[Window]) -> None:
         arr, keep = pack_windows(windows)

@@ -237,8 +237,8 @@ int32_t rh_align_run(rh_align_batch* b) {
   RH_HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1), RH_ERR_DEVICE);
   b->stats.kernel_ms = ms;
   b->stats.launches = 1;
-  RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(rh::AlignResult),
-                         hipMemcpyDeviceToHost),
+  RH_HIP_CHECK(rh::copy_sync(b->results.data(), b->d_results.ptr, n * sizeof(rh::AlignResult),
+                             hipMemcpyDeviceToHost, b->stream),
                RH_ERR_DEVICE);
   b->stats.ops = 0;
   b->stats.steps = 0;
@@ -291,7 +291,8 @@ int32_t rh_align_download(rh_align_batch* b, uint8_t* ops, uint64_t capacity,
   auto t0 = std::chrono::steady_clock::now();
   std::vector<uint8_t> dev(b->ops_bytes);
   if (b->ops_bytes > 0) {
-    RH_HIP_CHECK(hipMemcpy(dev.data(), b->d_ops.ptr, b->ops_bytes, hipMemcpyDeviceToHost),
+    RH_HIP_CHECK(rh::copy_sync(dev.data(), b->d_ops.ptr, b->ops_bytes, hipMemcpyDeviceToHost,
+                               b->stream),
                  RH_ERR_DEVICE);
   }
   uint64_t pos = 0;

@@ -19,6 +19,7 @@ namespace rh {
 
 void set_error(const std::string& message);
 void clear_error();
+const char* last_error();
 
 #define RH_HIP_CHECK(expr, code)                                                   \
   do {                                                                             \
@@ -83,6 +84,17 @@ inline void run_copies(uint8_t* base, const std::vector<ArenaCopy>& copies, uint
     pool.emplace_back(run, c, std::min(nc, c + chunk));
   }
   for (auto& t : pool) t.join();
+}
+
+// A copy on `stream`, waited for. Unlike hipMemcpy (the null stream, which
+// waits for every other stream of the device) it does not wait for kernels
+// that other batch objects run at the same time.
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                            hipStream_t stream) {
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return e;
 }
 
 // Device memory buffer that grows on demand (never shrinks).

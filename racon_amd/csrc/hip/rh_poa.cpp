@@ -15,8 +15,7 @@
 #include <thread>
 #include <vector>
 
-#include "poa_types.hpp"
-#include "rh_common.hpp"
+#include "poa_host.hpp"
 
 namespace rh {
 
@@ -36,62 +35,7 @@ void set_error(const std::string& message) { g_error = message; }
 void clear_error() { g_error.clear(); }
 const char* last_error() { return g_error.c_str(); }
 
-}  // namespace rh
-
-struct rh_poa_batch {
-  int32_t device = 0;
-  int32_t match = 3, mismatch = -5, gap = -4;
-  bool trim = true;
-  uint32_t tune_slots = 0;
-  uint32_t tune_rows = 0;
-  uint32_t tune_ring = 0;
-
-  // host copies of the packed batch
-  rh::HostArena arena;  // sized without zero-filling
-  std::vector<rh::PoaWindow> windows;
-  std::vector<rh::PoaLayer> layers;
-  std::vector<uint32_t> gpu_order;        // windows that need the kernel
-  std::vector<uint8_t> shallow;           // 1: < 3 sequences
-  std::vector<uint64_t> backbone_off;     // arena offset of each backbone
-  std::vector<uint32_t> backbone_len;
-  std::vector<rh::PoaResult> results;
-  uint64_t out_bytes = 0;
-
-  rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results, d_order, d_queue,
-      d_scratch, d_params;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool uploaded = false;
-  bool ran = false;
-  bool submitted = false;  // between rh_poa_submit and rh_poa_wait
-  bool launched = false;   // the submit queued kernel passes
-  uint32_t lds_limit = 0;
-  std::vector<std::vector<uint32_t>> pass_orders;  // window order of each queued pass
-  rh_poa_stats stats{};
-};
-
 namespace {
-
-using rh::PoaLayer;
-using rh::PoaParams;
-using rh::PoaResult;
-using rh::PoaSlotCaps;
-using rh::PoaWindow;
-
-PoaResult make_result() {
-  PoaResult r{};
-  r.status = rh::kPoaNotRun;
-  return r;
-}
-
-bool is_retry_status(int32_t s) {
-  return s == rh::kPoaNodeOverflow || s == rh::kPoaRowOverflow || s == rh::kPoaOutOverflow;
-}
-
-// Estimated cost of a window: layers x backbone x graph growth.
-uint64_t window_cost(const PoaWindow& w) {
-  return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
-}
 
 // DP rows kept in LDS (16-bit scores): as many as fit while 16 waves share a
 // CU's 160 KiB (12 or 8 waves for long layers), at most 8; a tuned value
@@ -103,7 +47,7 @@ uint32_t ring_rows_for(uint32_t cols, uint32_t tuned) {
   const uint64_t lds_per_cu = 160u * 1024u;
   for (uint32_t waves : {16u, 12u, 8u}) {
     const uint64_t per_wave = lds_per_cu / waves;
-    const uint64_t fixed = rh::align_up(cols, 16);
+    const uint64_t fixed = align_up(cols, 16);
     if (per_wave <= fixed) continue;
     const uint64_t rows = (per_wave - fixed) / (2ull * cols);
     if (rows >= 4 || waves == 8u) {
@@ -116,7 +60,7 @@ uint32_t ring_rows_for(uint32_t cols, uint32_t tuned) {
 // Windows at the front of the cost-ordered queue whose estimated cost is at
 // least 1.5 times the mean: the ones that can outlast the rest of the batch
 // on a single wave (2x and 1.25x measured slower on a 10k-window batch).
-uint32_t priority_windows(const rh_poa_batch* b, const std::vector<uint32_t>& order) {
+uint32_t priority_windows(const PoaWindow* host, const std::vector<uint32_t>& order) {
 #ifdef RH_POA_NO_PRIORITY
   return 0;
 #endif
@@ -124,81 +68,210 @@ uint32_t priority_windows(const rh_poa_batch* b, const std::vector<uint32_t>& or
     return 0;
   }
   long double total = 0;
-  for (uint32_t w : order) total += static_cast<long double>(window_cost(b->windows[w]));
+  for (uint32_t w : order) total += static_cast<long double>(window_cost(host[w]));
   const long double mean = total / order.size();
   uint32_t n = 0;
-  while (n < order.size() && window_cost(b->windows[order[n]]) >= 1.5L * mean) ++n;
+  while (n < order.size() && window_cost(host[order[n]]) >= 1.5L * mean) ++n;
   return n;
 }
 
-// One kernel pass: slot capacities and the DP rows kept in LDS per wave.
-struct PassPlan {
-  PoaSlotCaps caps;
-  uint32_t ring;  // LDS ring rows (0: every DP row in the slot's global matrix)
-  uint32_t lds;   // dynamic LDS bytes per wave
-};
+}  // namespace
 
-// Layers up to this length run in the main pass; longer ones run in a pass
-// of their own with a wider row pitch (and a shallower LDS ring), so that one
-// long layer does not cut the LDS ring and the occupancy of every window.
-constexpr uint32_t kMainPassMaxLayer = 3072;
+uint64_t window_cost(const PoaWindow& w) {
+  return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
+}
 
-// The plan for `ws`: graph arrays for its largest window, the row pitch of
-// its longest layer, DP rows capped at row_cap (0: exact, node bound + 1),
-// and the deepest ring (the tuned or occupancy default, at most 8 rows in
-// long passes) that fits the per-workgroup LDS limit. False when even a
-// ring of 0 rows does not fit (the layer bytes alone exceed the limit).
-bool plan_pass(const rh_poa_batch* b, const std::vector<uint32_t>& ws, uint32_t row_cap,
-               uint32_t lds_limit, PassPlan* plan) {
+bool is_retry_status(int32_t s) {
+  return s == kPoaNodeOverflow || s == kPoaRowOverflow || s == kPoaOutOverflow;
+}
+
+int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out) {
+  PackedWindows& P = *out;
+  P.arena.clear();
+  P.windows.assign(n, PoaWindow{});
+  P.layers.clear();
+  P.gpu_order.clear();
+  P.shallow.assign(n, 0);
+  P.host_rejected.assign(n, 0);
+  P.out_bytes = 0;
+
+  uint64_t total = 0, n_layers = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rh_window& w = windows[i];
+    if (w.backbone == nullptr || w.backbone_quality == nullptr || w.backbone_length == 0 ||
+        (w.n_layers > 0 && w.layers == nullptr)) {
+      set_error("window " + std::to_string(i) + " is malformed");
+      return RH_ERR_ARGUMENT;
+    }
+    total += 2ull * w.backbone_length;
+    for (uint32_t k = 0; k < w.n_layers; ++k) {
+      total += 2ull * w.layers[k].length;
+    }
+    n_layers += w.n_layers;
+  }
+  P.layers.reserve(n_layers);
+
+  // offsets first (same layout as appending in order), bytes copied after
+  std::vector<ArenaCopy> copies;
+  copies.reserve(total / 256 + 2ull * n);
+  uint64_t cur = 0;
+  auto place = [&](const char* src, uint32_t len) {
+    copies.push_back(ArenaCopy{src, cur, len});
+    const uint64_t at = cur;
+    cur += len;
+    return at;
+  };
+  uint64_t out_off = 0;
+  std::vector<uint32_t> rank;
+  for (uint32_t i = 0; i < n; ++i) {
+    const rh_window& w = windows[i];
+    PoaWindow& d = P.windows[i];
+    d.backbone_off = place(w.backbone, w.backbone_length);
+    d.backbone_qual = place(w.backbone_quality, w.backbone_length);
+    d.backbone_len = w.backbone_length;
+    d.n_layers = w.n_layers;
+    d.layer_first = static_cast<uint32_t>(P.layers.size());
+    d.offset = static_cast<uint32_t>(0.01 * w.backbone_length);
+    d.flags = (w.tgs ? 1u : 0u) | (trim ? 2u : 0u);
+
+    for (uint32_t k = 0; k < w.n_layers; ++k) {
+      const rh_layer& l = w.layers[k];
+      if (l.sequence == nullptr || l.length == 0 || l.begin >= l.end ||
+          l.end > w.backbone_length) {
+        set_error("window " + std::to_string(i) + " layer " + std::to_string(k) +
+                  " is invalid");
+        return RH_ERR_ARGUMENT;
+      }
+    }
+    if (w.n_layers + 1 < 3) {
+      P.shallow[i] = 1;
+      d.n_layers = 0;
+      continue;
+    }
+    // racon's layer order: std::sort of indices 1..n by begin (window.cpp:78-85)
+    rank.resize(w.n_layers + 1);
+    std::iota(rank.begin(), rank.end(), 0u);
+    std::sort(rank.begin() + 1, rank.end(), [&](uint32_t lhs, uint32_t rhs) {
+      return w.layers[lhs - 1].begin < w.layers[rhs - 1].begin;
+    });
+    uint64_t sum_len = 0;
+    uint32_t max_len = 0;
+    for (uint32_t k = 1; k <= w.n_layers; ++k) {
+      const rh_layer& l = w.layers[rank[k] - 1];
+      PoaLayer pl{};
+      pl.seq_off = place(l.sequence, l.length);
+      if (l.quality != nullptr) {
+        pl.qual_off = place(l.quality, l.length);
+      } else {
+        pl.qual_off = kNoQuality;
+      }
+      pl.len = l.length;
+      pl.begin = l.begin;
+      pl.end = l.end;
+      P.layers.push_back(pl);
+      sum_len += l.length;
+      max_len = std::max(max_len, l.length);
+    }
+    const uint64_t node_bound = w.backbone_length + sum_len;
+    const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
+    // DP rows are packed in 24 bits of a predecessor entry: a larger graph is
+    // refused here (RH_WINDOW_REJECTED, the caller's CPU path takes it); the
+    // in-edge index of a move code (13 bits) is checked on the device
+    if (node_bound >= (1ull << 24) - 1) {
+      P.host_rejected[i] = 1;
+      continue;
+    }
+    d.node_bound = static_cast<uint32_t>(node_bound);
+    d.edge_bound = static_cast<uint32_t>(edge_bound);
+    d.max_layer_len = max_len;
+    d.out_cap = static_cast<uint32_t>(
+        std::min<uint64_t>(node_bound, 2ull * w.backbone_length + 1024));
+    d.out_off = out_off;
+    out_off += align_up(d.out_cap, 16);
+    P.gpu_order.push_back(i);
+  }
+  P.out_bytes = out_off;
+  P.arena.resize(cur);
+  run_copies(P.arena.data(), copies, cur);
+  // largest estimated cost first (longest-processing-time order)
+  std::stable_sort(P.gpu_order.begin(), P.gpu_order.end(), [&](uint32_t x, uint32_t y) {
+    return window_cost(P.windows[x]) > window_cost(P.windows[y]);
+  });
+  return RH_OK;
+}
+
+hipError_t PoaLane::init(int32_t dev) {
+  device = dev;
+  hipError_t e = hipSetDevice(dev);
+  if (e == hipSuccess) e = hipStreamCreate(&stream);
+  if (e == hipSuccess) e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  int limit = 0;
+  if (e == hipSuccess) {
+    e = hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  }
+  lds_limit = static_cast<uint32_t>(limit);
+  return e;
+}
+
+void PoaLane::release() {
+  (void)hipSetDevice(device);
+  if (stream != nullptr) (void)hipStreamSynchronize(stream);
+  d_scratch.release();
+  d_order.release();
+  d_queue.release();
+  d_params.release();
+  if (ev0 != nullptr) (void)hipEventDestroy(ev0);
+  if (ev1 != nullptr) (void)hipEventDestroy(ev1);
+  if (stream != nullptr) (void)hipStreamDestroy(stream);
+  ev0 = ev1 = nullptr;
+  stream = nullptr;
+}
+
+bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t row_cap,
+               uint32_t lds_limit, uint32_t tune_ring, PassPlan* plan) {
   PoaSlotCaps caps{};
   uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
   for (uint32_t w : ws) {
-    max_nodes = std::max(max_nodes, b->windows[w].node_bound);
-    max_edges = std::max(max_edges, b->windows[w].edge_bound);
-    max_cols = std::max(max_cols, b->windows[w].max_layer_len + 1);
+    max_nodes = std::max(max_nodes, host[w].node_bound);
+    max_edges = std::max(max_edges, host[w].edge_bound);
+    max_cols = std::max(max_cols, host[w].max_layer_len + 1);
   }
   caps.nodes = max_nodes;
   caps.edges = max_edges;
-  caps.cols = static_cast<uint32_t>(rh::align_up(max_cols - 1 + rh::kPoaColPad, 16));
+  caps.cols = static_cast<uint32_t>(align_up(max_cols - 1 + kPoaColPad, 16));
   caps.rows = row_cap == 0 ? max_nodes + 1 : std::min<uint32_t>(max_nodes + 1, row_cap);
-  caps.stack = 2 * (max_edges + (rh::kMaxAligned + 1) * max_nodes) + 64;
-  uint32_t ring = ring_rows_for(caps.cols, b->tune_ring);
-  while (ring > 0 && rh::poa_lds_bytes(caps.cols, ring) > lds_limit) --ring;
+  caps.stack = 2 * (max_edges + (kMaxAligned + 1) * max_nodes) + 64;
+  uint32_t ring = ring_rows_for(caps.cols, tune_ring);
+  while (ring > 0 && poa_lds_bytes(caps.cols, ring) > lds_limit) --ring;
   if (ring == 1) ring = 0;  // one 16-bit row is no slot in 32-bit mode; keep none
-  if (rh::poa_lds_bytes(caps.cols, ring) > lds_limit) {
+  if (poa_lds_bytes(caps.cols, ring) > lds_limit) {
     return false;
   }
   plan->caps = caps;
   plan->ring = ring;
-  plan->lds = rh::poa_lds_bytes(caps.cols, ring);
+  plan->lds = poa_lds_bytes(caps.cols, ring);
   return true;
 }
 
-// A pass ready to launch: its plan, resident slots and slot size.
-struct PassLaunch {
-  PassPlan plan;
-  uint64_t slots = 0;
-  uint64_t slot_bytes = 0;
-  rh::PoaSlotLayout layout{};
-};
-
-int32_t size_pass(rh_poa_batch* b, uint64_t n_windows, const PassPlan& plan, int32_t mode,
-                  uint64_t scratch_held, PassLaunch* out) {
+int32_t size_pass(PoaLane& lane, uint64_t n_windows, const PassPlan& plan, int32_t mode,
+                  const PoaConfig& c, PassLaunch* out) {
   int per_cu = 0, cus = 0;
-  RH_HIP_CHECK(rh::poa_occupancy(&per_cu, plan.lds), RH_ERR_DEVICE);
-  RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
+  RH_HIP_CHECK(poa_occupancy(&per_cu, plan.lds), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lane.device),
                RH_ERR_DEVICE);
   uint64_t max_slots = static_cast<uint64_t>(std::max(1, per_cu)) * std::max(1, cus);
-  if (b->tune_slots != 0) {
-    max_slots = b->tune_slots;
+  max_slots = std::max<uint64_t>(1, max_slots / std::max<uint32_t>(1, c.slot_share));
+  if (c.tune_slots != 0) {
+    max_slots = c.tune_slots;
   }
   out->plan = plan;
-  out->layout = rh::poa_slot_layout(plan.caps);
-  out->slot_bytes = rh::align_up(out->layout.total, 4096);
+  out->layout = poa_slot_layout(plan.caps);
+  out->slot_bytes = align_up(out->layout.total, 4096);
   size_t free_b = 0, total_b = 0;
   RH_HIP_CHECK(hipMemGetInfo(&free_b, &total_b), RH_ERR_DEVICE);
-  // keep the slots within ~60% of free memory (plus what we already hold)
-  const uint64_t budget = static_cast<uint64_t>((free_b + scratch_held) * 0.6);
+  // keep the slots within ~60% of free memory (plus what the lane holds)
+  const uint64_t budget = static_cast<uint64_t>((free_b + lane.d_scratch.bytes) * 0.6);
   uint64_t slots = std::max<uint64_t>(1, budget / out->slot_bytes);
   if (mode != RH_POA_STATIC) {
     slots = std::min<uint64_t>(slots, max_slots);
@@ -211,93 +284,159 @@ int32_t size_pass(rh_poa_batch* b, uint64_t n_windows, const PassPlan& plan, int
   return RH_OK;
 }
 
-// Queues passes back to back on the batch's stream (their orders, queue
-// counters and parameter blocks side by side; the passes share the scratch,
-// which is sized for the largest) between events ev0 and ev1. Returns
-// without waiting.
-int32_t launch_passes(rh_poa_batch* b, const std::vector<const std::vector<uint32_t>*>& orders,
-                      const std::vector<PassLaunch>& passes, int32_t mode) {
+int32_t plan_bins(PoaLane& lane, const PoaWindow* host, const std::vector<uint32_t>& ws,
+                  uint32_t row_cap, int32_t mode, const PoaConfig& c,
+                  std::vector<std::vector<uint32_t>>* orders, std::vector<PassLaunch>* passes,
+                  std::vector<uint32_t>* refused) {
+  std::vector<uint32_t> bins[2];
+  for (uint32_t w : ws) {
+    bins[layer_bin(host[w])].push_back(w);
+  }
+  for (int k = 0; k < 2; ++k) {
+    std::vector<uint32_t>& bin = bins[k];
+    if (bin.empty()) continue;
+    if (mode == RH_POA_STATIC) std::sort(bin.begin(), bin.end());
+    PassPlan plan{};
+    const uint32_t ring = c.bin_ring[k] != 0 ? c.bin_ring[k] : c.tune_ring;
+    if (!plan_pass(host, bin, row_cap, lane.lds_limit, ring, &plan)) {
+      refused->insert(refused->end(), bin.begin(), bin.end());
+      continue;
+    }
+    PassLaunch pl;
+    const int32_t rc = size_pass(lane, bin.size(), plan, mode, c, &pl);
+    if (rc != RH_OK) return rc;
+    passes->push_back(pl);
+    orders->push_back(std::move(bin));
+  }
+  return RH_OK;
+}
+
+int32_t launch_passes(PoaLane& lane, const PoaTables& t, const PoaConfig& c,
+                      const std::vector<const std::vector<uint32_t>*>& orders,
+                      const std::vector<PassLaunch>& passes, int32_t mode, uint32_t* launches) {
   const size_t np = passes.size();
   uint64_t scratch = 0, total_order = 0;
   for (size_t k = 0; k < np; ++k) {
     scratch = std::max(scratch, passes[k].slots * passes[k].slot_bytes);
     total_order += orders[k]->size();
   }
-  RH_HIP_CHECK(b->d_scratch.reserve(scratch), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_order.reserve(std::max<uint64_t>(total_order, 1) * sizeof(uint32_t)),
+  RH_HIP_CHECK(lane.d_scratch.reserve(scratch), RH_ERR_MEMORY);
+  RH_HIP_CHECK(lane.d_order.reserve(std::max<uint64_t>(total_order, 1) * sizeof(uint32_t)),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_queue.reserve(std::max<size_t>(np, 1) * sizeof(uint32_t)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_params.reserve(std::max<size_t>(np, 1) * sizeof(PoaParams)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(hipEventRecord(b->ev0, b->stream), RH_ERR_DEVICE);
+  RH_HIP_CHECK(lane.d_queue.reserve(std::max<size_t>(np, 1) * sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(lane.d_params.reserve(std::max<size_t>(np, 1) * sizeof(PoaParams)),
+               RH_ERR_MEMORY);
+  RH_HIP_CHECK(hipEventRecord(lane.ev0, lane.stream), RH_ERR_DEVICE);
   if (np > 0) {
-    RH_HIP_CHECK(hipMemsetAsync(b->d_queue.ptr, 0, np * sizeof(uint32_t), b->stream),
+    RH_HIP_CHECK(hipMemsetAsync(lane.d_queue.ptr, 0, np * sizeof(uint32_t), lane.stream),
                  RH_ERR_DEVICE);
   }
   uint64_t order_at = 0;
   for (size_t k = 0; k < np; ++k) {
     const std::vector<uint32_t>& order = *orders[k];
     const PassLaunch& pl = passes[k];
-    uint32_t* d_order = b->d_order.as<uint32_t>() + order_at;
+    uint32_t* d_order = lane.d_order.as<uint32_t>() + order_at;
     order_at += order.size();
     RH_HIP_CHECK(hipMemcpyAsync(d_order, order.data(), order.size() * sizeof(uint32_t),
-                                hipMemcpyHostToDevice, b->stream),
+                                hipMemcpyHostToDevice, lane.stream),
                  RH_ERR_DEVICE);
     PoaParams p{};
-    p.windows = b->d_windows.as<PoaWindow>();
-    p.layers = b->d_layers.as<PoaLayer>();
-    p.arena = b->d_arena.as<uint8_t>();
-    p.out = b->d_out.as<uint8_t>();
-    p.results = b->d_results.as<PoaResult>();
+    p.windows = t.windows;
+    p.layers = t.layers;
+    p.arena = t.arena;
+    p.out = t.out;
+    p.results = t.results;
     p.order = d_order;
     p.n_order = static_cast<uint32_t>(order.size());
-    p.queue_head = b->d_queue.as<uint32_t>() + k;
-    p.scratch = b->d_scratch.as<uint8_t>();
+    p.queue_head = lane.d_queue.as<uint32_t>() + k;
+    p.scratch = lane.d_scratch.as<uint8_t>();
     p.slot_bytes = pl.slot_bytes;
     p.caps = pl.plan.caps;
     p.layout = pl.layout;
-    p.match = b->match;
-    p.mismatch = b->mismatch;
-    p.gap = b->gap;
+    p.match = c.match;
+    p.mismatch = c.mismatch;
+    p.gap = c.gap;
     p.n_slots = static_cast<uint32_t>(pl.slots);
     p.ring_rows = pl.plan.ring;
     p.lds_bytes = pl.plan.lds;
-    p.n_priority = priority_windows(b, order);
-    PoaParams* dp = b->d_params.as<PoaParams>() + k;
-    RH_HIP_CHECK(hipMemcpyAsync(dp, &p, sizeof(PoaParams), hipMemcpyHostToDevice, b->stream),
+    p.n_priority = priority_windows(t.host, order);
+    PoaParams* dp = lane.d_params.as<PoaParams>() + k;
+    RH_HIP_CHECK(hipMemcpyAsync(dp, &p, sizeof(PoaParams), hipMemcpyHostToDevice, lane.stream),
                  RH_ERR_DEVICE);
     if (mode == RH_POA_PLAIN) {
       for (uint32_t q = 0; q < order.size(); ++q) {
-        RH_HIP_CHECK(rh::launch_poa_single(dp, p.lds_bytes, q, b->stream), RH_ERR_KERNEL);
-        b->stats.launches++;
+        RH_HIP_CHECK(launch_poa_single(dp, p.lds_bytes, q, lane.stream), RH_ERR_KERNEL);
+        ++*launches;
       }
     } else if (mode == RH_POA_STATIC) {
       // one workgroup (and slot) per window; launches of `slots` windows when
       // the slots of all windows do not fit in memory at once
       for (uint64_t first = 0; first < order.size(); first += pl.slots) {
         const uint64_t count = std::min<uint64_t>(pl.slots, order.size() - first);
-        RH_HIP_CHECK(rh::launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
-                                           static_cast<uint32_t>(count), b->stream),
+        RH_HIP_CHECK(launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
+                                       static_cast<uint32_t>(count), lane.stream),
                      RH_ERR_KERNEL);
-        b->stats.launches++;
+        ++*launches;
       }
     } else {
-      RH_HIP_CHECK(rh::launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(pl.slots),
-                                        b->stream),
+      RH_HIP_CHECK(launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(pl.slots),
+                                    lane.stream),
                    RH_ERR_KERNEL);
-      b->stats.launches++;
+      ++*launches;
     }
   }
-  RH_HIP_CHECK(hipEventRecord(b->ev1, b->stream), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipEventRecord(lane.ev1, lane.stream), RH_ERR_DEVICE);
   return RH_OK;
 }
 
-// Waits for the passes queued by launch_passes and adds their device time.
-int32_t finish_passes(rh_poa_batch* b, double* kernel_ms) {
-  RH_HIP_CHECK(hipEventSynchronize(b->ev1), RH_ERR_KERNEL);
+int32_t finish_passes(PoaLane& lane, double* kernel_ms) {
+  RH_HIP_CHECK(hipEventSynchronize(lane.ev1), RH_ERR_KERNEL);
   float ms = 0;
-  RH_HIP_CHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipEventElapsedTime(&ms, lane.ev0, lane.ev1), RH_ERR_DEVICE);
   *kernel_ms += ms;
   return RH_OK;
+}
+
+}  // namespace rh
+
+struct rh_poa_batch {
+  bool trim = true;
+  rh::PoaConfig config;
+  rh::PoaLane lane;
+
+  rh::PackedWindows packed;  // host copy of the batch
+  std::vector<rh::PoaResult> results;
+
+  rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results;
+  bool uploaded = false;
+  bool ran = false;
+  bool submitted = false;  // between rh_poa_submit and rh_poa_wait
+  bool launched = false;   // the submit queued kernel passes
+  std::vector<std::vector<uint32_t>> pass_orders;  // window order of each queued pass
+  rh_poa_stats stats{};
+
+  rh::PoaTables tables() const {
+    rh::PoaTables t;
+    t.host = packed.windows.data();
+    t.windows = d_windows.as<rh::PoaWindow>();
+    t.layers = d_layers.as<rh::PoaLayer>();
+    t.arena = d_arena.as<uint8_t>();
+    t.out = d_out.as<uint8_t>();
+    t.results = d_results.as<rh::PoaResult>();
+    return t;
+  }
+};
+
+namespace {
+
+using rh::PassLaunch;
+using rh::PoaResult;
+using rh::PoaWindow;
+
+PoaResult make_result() {
+  PoaResult r{};
+  r.status = rh::kPoaNotRun;
+  return r;
 }
 
 }  // namespace
@@ -330,16 +469,14 @@ int32_t rh_poa_create(int32_t device, int8_t match, int8_t mismatch, int8_t gap,
     rh::set_error("rh_poa_create: no such device");
     return RH_ERR_DEVICE;
   }
-  RH_HIP_CHECK(hipSetDevice(device), RH_ERR_DEVICE);
   rh_poa_batch* b = new rh_poa_batch();
-  b->device = device;
-  b->match = match;
-  b->mismatch = mismatch;
-  b->gap = gap;
+  b->config.match = match;
+  b->config.mismatch = mismatch;
+  b->config.gap = gap;
   b->trim = trim != 0;
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
-      hipEventCreate(&b->ev1) != hipSuccess) {
+  if (b->lane.init(device) != hipSuccess) {
     rh::set_error("rh_poa_create: stream/event creation failed");
+    b->lane.release();
     delete b;
     return RH_ERR_DEVICE;
   }
@@ -351,21 +488,12 @@ void rh_poa_destroy(rh_poa_batch* b) {
   if (b == nullptr) {
     return;
   }
-  (void)hipSetDevice(b->device);
-  if (b->stream != nullptr) {
-    (void)hipStreamSynchronize(b->stream);
-  }
+  b->lane.release();
   b->d_arena.release();
   b->d_windows.release();
   b->d_layers.release();
   b->d_out.release();
   b->d_results.release();
-  b->d_order.release();
-  b->d_queue.release();
-  b->d_scratch.release();
-  if (b->ev0 != nullptr) (void)hipEventDestroy(b->ev0);
-  if (b->ev1 != nullptr) (void)hipEventDestroy(b->ev1);
-  if (b->stream != nullptr) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
@@ -374,9 +502,9 @@ int32_t rh_poa_set_tuning(rh_poa_batch* b, uint32_t slots, uint32_t rows, uint32
     rh::set_error("rh_poa_set_tuning: null batch");
     return RH_ERR_ARGUMENT;
   }
-  b->tune_slots = slots;
-  b->tune_rows = rows;
-  b->tune_ring = ring_rows;
+  b->config.tune_slots = slots;
+  b->config.tune_rows = rows;
+  b->config.tune_ring = ring_rows;
   return RH_OK;
 }
 
@@ -390,151 +518,52 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
     rh::set_error("rh_poa_upload: the batch is running (call rh_poa_wait first)");
     return RH_ERR_STATE;
   }
-  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipSetDevice(b->lane.device), RH_ERR_DEVICE);
   auto t0 = std::chrono::steady_clock::now();
-  b->arena.clear();
-  b->windows.assign(n, PoaWindow{});
-  b->layers.clear();
-  b->gpu_order.clear();
-  b->shallow.assign(n, 0);
-  b->backbone_off.assign(n, 0);
-  b->backbone_len.assign(n, 0);
-  b->results.assign(n, make_result());
   b->stats = rh_poa_stats{};
-
-  uint64_t total = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const rh_window& w = windows[i];
-    if (w.backbone == nullptr || w.backbone_quality == nullptr || w.backbone_length == 0 ||
-        (w.n_layers > 0 && w.layers == nullptr)) {
-      rh::set_error("rh_poa_upload: window " + std::to_string(i) + " is malformed");
-      return RH_ERR_ARGUMENT;
-    }
-    total += 2ull * w.backbone_length;
-    for (uint32_t k = 0; k < w.n_layers; ++k) {
-      total += 2ull * w.layers[k].length;
-    }
+  b->uploaded = false;
+  const int32_t rc = rh::pack_windows(n, windows, b->trim, &b->packed);
+  if (rc != RH_OK) {
+    rh::set_error(std::string("rh_poa_upload: ") + rh::last_error());
+    return rc;
   }
-
-  // offsets first (same layout as appending in order), bytes copied after
-  std::vector<rh::ArenaCopy> copies;
-  copies.reserve(total / 256 + 2ull * n);
-  uint64_t cur = 0;
-  auto place = [&](const char* src, uint32_t len) {
-    copies.push_back(rh::ArenaCopy{src, cur, len});
-    const uint64_t at = cur;
-    cur += len;
-    return at;
-  };
-  uint64_t out_off = 0;
-  std::vector<uint32_t> rank;
+  const rh::PackedWindows& P = b->packed;
+  b->results.assign(n, make_result());
   for (uint32_t i = 0; i < n; ++i) {
-    const rh_window& w = windows[i];
-    PoaWindow& d = b->windows[i];
-    d.backbone_off = place(w.backbone, w.backbone_length);
-    b->backbone_off[i] = d.backbone_off;
-    b->backbone_len[i] = w.backbone_length;
-    d.backbone_qual = place(w.backbone_quality, w.backbone_length);
-    d.backbone_len = w.backbone_length;
-    d.n_layers = w.n_layers;
-    d.layer_first = static_cast<uint32_t>(b->layers.size());
-    d.offset = static_cast<uint32_t>(0.01 * w.backbone_length);
-    d.flags = (w.tgs ? 1u : 0u) | (b->trim ? 2u : 0u);
-
-    for (uint32_t k = 0; k < w.n_layers; ++k) {
-      const rh_layer& l = w.layers[k];
-      if (l.sequence == nullptr || l.length == 0 || l.begin >= l.end ||
-          l.end > w.backbone_length) {
-        rh::set_error("rh_poa_upload: window " + std::to_string(i) + " layer " +
-                      std::to_string(k) + " is invalid");
-        return RH_ERR_ARGUMENT;
-      }
-    }
-    if (w.n_layers + 1 < 3) {
-      b->shallow[i] = 1;
-      d.n_layers = 0;
-      continue;
-    }
-    // racon's layer order: std::sort of indices 1..n by begin (window.cpp:78-85)
-    rank.resize(w.n_layers + 1);
-    std::iota(rank.begin(), rank.end(), 0u);
-    std::sort(rank.begin() + 1, rank.end(), [&](uint32_t lhs, uint32_t rhs) {
-      return w.layers[lhs - 1].begin < w.layers[rhs - 1].begin;
-    });
-    uint64_t sum_len = 0;
-    uint32_t max_len = 0;
-    for (uint32_t k = 1; k <= w.n_layers; ++k) {
-      const rh_layer& l = w.layers[rank[k] - 1];
-      PoaLayer pl{};
-      pl.seq_off = place(l.sequence, l.length);
-      if (l.quality != nullptr) {
-        pl.qual_off = place(l.quality, l.length);
-      } else {
-        pl.qual_off = rh::kNoQuality;
-      }
-      pl.len = l.length;
-      pl.begin = l.begin;
-      pl.end = l.end;
-      b->layers.push_back(pl);
-      sum_len += l.length;
-      max_len = std::max(max_len, l.length);
-    }
-    const uint64_t node_bound = w.backbone_length + sum_len;
-    const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
-    // DP rows are packed in 24 bits of a predecessor entry: a larger graph is
-    // refused here (RH_WINDOW_REJECTED, the caller's CPU path takes it); the
-    // in-edge index of a move code (13 bits) is checked on the device
-    if (node_bound >= (1ull << 24) - 1) {
-      b->results[i].status = rh::kPoaRejectedOnHost;
-      continue;
-    }
-    d.node_bound = static_cast<uint32_t>(node_bound);
-    d.edge_bound = static_cast<uint32_t>(edge_bound);
-    d.max_layer_len = max_len;
-    d.out_cap = static_cast<uint32_t>(
-        std::min<uint64_t>(node_bound, 2ull * w.backbone_length + 1024));
-    d.out_off = out_off;
-    out_off += rh::align_up(d.out_cap, 16);
-    b->gpu_order.push_back(i);
+    if (P.host_rejected[i]) b->results[i].status = rh::kPoaRejectedOnHost;
   }
-  b->out_bytes = out_off;
-  b->arena.resize(cur);
-  rh::run_copies(b->arena.data(), copies, cur);
-  // largest estimated cost first (longest-processing-time order)
-  std::stable_sort(b->gpu_order.begin(), b->gpu_order.end(), [&](uint32_t x, uint32_t y) {
-    return window_cost(b->windows[x]) > window_cost(b->windows[y]);
-  });
   b->stats.pack_ms = rh::ms_since(t0);
   b->stats.windows = n;
-  b->stats.gpu_windows = b->gpu_order.size();
-  b->stats.input_bytes = b->arena.size();
+  b->stats.gpu_windows = P.gpu_order.size();
+  b->stats.input_bytes = P.arena.size();
 
   auto t1 = std::chrono::steady_clock::now();
-  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(b->arena.size(), 1)), RH_ERR_MEMORY);
+  hipStream_t st = b->lane.stream;
+  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(P.arena.size(), 1)), RH_ERR_MEMORY);
   RH_HIP_CHECK(b->d_windows.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaWindow)),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_layers.reserve(std::max<size_t>(b->layers.size(), 1) * sizeof(PoaLayer)),
+  RH_HIP_CHECK(b->d_layers.reserve(std::max<size_t>(P.layers.size(), 1) * sizeof(rh::PoaLayer)),
                RH_ERR_MEMORY);
   RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaResult)),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(b->out_bytes, 1)), RH_ERR_MEMORY);
-  if (!b->arena.empty()) {
-    RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, b->arena.data(), b->arena.size(),
-                                hipMemcpyHostToDevice, b->stream),
+  RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(P.out_bytes, 1)), RH_ERR_MEMORY);
+  if (!P.arena.empty()) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, P.arena.data(), P.arena.size(),
+                                hipMemcpyHostToDevice, st),
                  RH_ERR_DEVICE);
   }
   if (n > 0) {
-    RH_HIP_CHECK(hipMemcpyAsync(b->d_windows.ptr, b->windows.data(), n * sizeof(PoaWindow),
-                                hipMemcpyHostToDevice, b->stream),
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_windows.ptr, P.windows.data(), n * sizeof(PoaWindow),
+                                hipMemcpyHostToDevice, st),
                  RH_ERR_DEVICE);
   }
-  if (!b->layers.empty()) {
-    RH_HIP_CHECK(hipMemcpyAsync(b->d_layers.ptr, b->layers.data(),
-                                b->layers.size() * sizeof(PoaLayer), hipMemcpyHostToDevice,
-                                b->stream),
+  if (!P.layers.empty()) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_layers.ptr, P.layers.data(),
+                                P.layers.size() * sizeof(rh::PoaLayer), hipMemcpyHostToDevice,
+                                st),
                  RH_ERR_DEVICE);
   }
-  RH_HIP_CHECK(hipStreamSynchronize(b->stream), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipStreamSynchronize(st), RH_ERR_DEVICE);
   b->stats.h2d_ms = rh::ms_since(t1);
   b->uploaded = true;
   b->ran = false;
@@ -555,8 +584,8 @@ int32_t rh_poa_submit(rh_poa_batch* b, int32_t mode) {
     rh::set_error("rh_poa_submit: unknown mode");
     return RH_ERR_ARGUMENT;
   }
-  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
-  const uint32_t n = static_cast<uint32_t>(b->windows.size());
+  RH_HIP_CHECK(hipSetDevice(b->lane.device), RH_ERR_DEVICE);
+  const uint32_t n = static_cast<uint32_t>(b->packed.windows.size());
   b->stats.kernel_ms = 0;
   b->stats.launches = 0;
   b->stats.retried = 0;
@@ -572,42 +601,24 @@ int32_t rh_poa_submit(rh_poa_batch* b, int32_t mode) {
   for (uint32_t w = 0; w < n; ++w) {
     if (b->results[w].status != rh::kPoaRejectedOnHost) b->results[w] = make_result();
   }
-  int lds_limit = 0;
-  RH_HIP_CHECK(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                     b->device),
-               RH_ERR_DEVICE);
-  b->lds_limit = static_cast<uint32_t>(lds_limit);
-
   // bins by layer length: the main pass and the long-layer pass (windows in
   // the cost order of the queue either way, in upload order for STATIC)
-  std::vector<uint32_t> bins[2];
-  for (uint32_t w : b->gpu_order) {
-    bins[b->windows[w].max_layer_len <= kMainPassMaxLayer ? 0 : 1].push_back(w);
-  }
-  const uint32_t row_cap = b->tune_rows ? b->tune_rows : 4096;
+  const uint32_t row_cap = b->config.tune_rows ? b->config.tune_rows : 4096;
   std::vector<PassLaunch> passes;
-  for (auto& ws : bins) {
-    if (ws.empty()) continue;
-    if (mode == RH_POA_STATIC) std::sort(ws.begin(), ws.end());
-    PassPlan plan{};
-    if (!plan_pass(b, ws, row_cap, b->lds_limit, &plan)) {
-      for (uint32_t w : ws) b->results[w].status = rh::kPoaRejectedOnHost;
-      continue;
-    }
-    PassLaunch pl;
-    const int32_t rc = size_pass(b, ws.size(), plan, mode, b->d_scratch.bytes, &pl);
-    if (rc != RH_OK) return rc;
-    passes.push_back(pl);
-    b->pass_orders.push_back(std::move(ws));
-  }
+  std::vector<uint32_t> refused;
+  int32_t rc = rh::plan_bins(b->lane, b->packed.windows.data(), b->packed.gpu_order, row_cap,
+                             mode, b->config, &b->pass_orders, &passes, &refused);
+  if (rc != RH_OK) return rc;
+  for (uint32_t w : refused) b->results[w].status = rh::kPoaRejectedOnHost;
   RH_HIP_CHECK(hipMemcpyAsync(b->d_results.ptr, b->results.data(), n * sizeof(PoaResult),
-                              hipMemcpyHostToDevice, b->stream),
+                              hipMemcpyHostToDevice, b->lane.stream),
                RH_ERR_DEVICE);
   std::vector<const std::vector<uint32_t>*> orders;
   for (const auto& o : b->pass_orders) orders.push_back(&o);
   if (!passes.empty()) {
     b->stats.slots = static_cast<uint32_t>(passes[0].slots);
-    const int32_t rc = launch_passes(b, orders, passes, mode);
+    rc = rh::launch_passes(b->lane, b->tables(), b->config, orders, passes, mode,
+                           &b->stats.launches);
     if (rc != RH_OK) return rc;
   }
   b->launched = !passes.empty();
@@ -621,83 +632,74 @@ int32_t rh_poa_wait(rh_poa_batch* b) {
     return RH_ERR_STATE;
   }
   b->submitted = false;
-  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
-  const uint32_t n = static_cast<uint32_t>(b->windows.size());
+  RH_HIP_CHECK(hipSetDevice(b->lane.device), RH_ERR_DEVICE);
+  std::vector<PoaWindow>& windows = b->packed.windows;
+  const uint32_t n = static_cast<uint32_t>(windows.size());
   if (n == 0) {
     b->ran = true;
     return RH_OK;
   }
   double kernel_ms = 0;
   if (b->launched) {
-    const int32_t rc = finish_passes(b, &kernel_ms);
+    const int32_t rc = rh::finish_passes(b->lane, &kernel_ms);
     if (rc != RH_OK) return rc;
   }
-  RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
-                         hipMemcpyDeviceToHost),
+  RH_HIP_CHECK(rh::copy_sync(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
+                             hipMemcpyDeviceToHost, b->lane.stream),
                RH_ERR_DEVICE);
 
   // windows that outgrew their slot: rerun with exact bounds (rows = node
   // bound + 1), in one pass per layer-length bin
-  std::vector<uint32_t> retry[2];
-  for (uint32_t w : b->gpu_order) {
-    if (is_retry_status(b->results[w].status)) {
-      retry[b->windows[w].max_layer_len <= kMainPassMaxLayer ? 0 : 1].push_back(w);
-    }
+  std::vector<uint32_t> retry;
+  for (uint32_t w : b->packed.gpu_order) {
+    if (rh::is_retry_status(b->results[w].status)) retry.push_back(w);
   }
-  if (!retry[0].empty() || !retry[1].empty()) {
-    b->stats.retried = retry[0].size() + retry[1].size();
+  if (!retry.empty()) {
+    b->stats.retried = retry.size();
     uint64_t extra_out = 0;
-    for (const auto& ws : retry) {
-      for (uint32_t w : ws) extra_out += rh::align_up(b->windows[w].node_bound, 16);
-    }
+    for (uint32_t w : retry) extra_out += rh::align_up(windows[w].node_bound, 16);
     // bigger output regions for the retried windows, appended to d_out
-    const uint64_t old_out = b->out_bytes;
+    const uint64_t old_out = b->packed.out_bytes;
     rh::DeviceBuffer grown;
     RH_HIP_CHECK(grown.reserve(old_out + extra_out), RH_ERR_MEMORY);
-    RH_HIP_CHECK(hipMemcpy(grown.ptr, b->d_out.ptr, old_out, hipMemcpyDeviceToDevice),
+    RH_HIP_CHECK(rh::copy_sync(grown.ptr, b->d_out.ptr, old_out, hipMemcpyDeviceToDevice,
+                               b->lane.stream),
                  RH_ERR_DEVICE);
     b->d_out.release();
     b->d_out = grown;
     uint64_t off = old_out;
-    for (const auto& ws : retry) {
-      for (uint32_t w : ws) {
-        b->windows[w].out_off = off;
-        b->windows[w].out_cap = b->windows[w].node_bound;
-        off += rh::align_up(b->windows[w].node_bound, 16);
-        RH_HIP_CHECK(hipMemcpy(b->d_windows.as<PoaWindow>() + w, &b->windows[w],
-                               sizeof(PoaWindow), hipMemcpyHostToDevice),
-                     RH_ERR_DEVICE);
-      }
+    for (uint32_t w : retry) {
+      windows[w].out_off = off;
+      windows[w].out_cap = windows[w].node_bound;
+      off += rh::align_up(windows[w].node_bound, 16);
+      RH_HIP_CHECK(rh::copy_sync(b->d_windows.as<PoaWindow>() + w, &windows[w],
+                                 sizeof(PoaWindow), hipMemcpyHostToDevice, b->lane.stream),
+                   RH_ERR_DEVICE);
     }
-    b->out_bytes = off;
+    b->packed.out_bytes = off;
+    std::vector<std::vector<uint32_t>> retry_orders;
     std::vector<PassLaunch> passes;
-    std::vector<const std::vector<uint32_t>*> orders;
-    for (const auto& ws : retry) {
-      if (ws.empty()) continue;
-      PassPlan plan{};
-      if (!plan_pass(b, ws, 0, b->lds_limit, &plan)) {
-        continue;  // stays in its overflow status: rejected below
-      }
-      PassLaunch pl;
-      const int32_t rc = size_pass(b, ws.size(), plan, RH_POA_BATCHED, b->d_scratch.bytes, &pl);
-      if (rc != RH_OK) return rc;
-      passes.push_back(pl);
-      orders.push_back(&ws);
-    }
+    std::vector<uint32_t> refused;  // stay in their overflow status: rejected below
+    int32_t rc = rh::plan_bins(b->lane, windows.data(), retry, 0, RH_POA_BATCHED, b->config,
+                               &retry_orders, &passes, &refused);
+    if (rc != RH_OK) return rc;
     if (!passes.empty()) {
-      int32_t rc = launch_passes(b, orders, passes, RH_POA_BATCHED);
+      std::vector<const std::vector<uint32_t>*> orders;
+      for (const auto& o : retry_orders) orders.push_back(&o);
+      rc = rh::launch_passes(b->lane, b->tables(), b->config, orders, passes, RH_POA_BATCHED,
+                             &b->stats.launches);
       if (rc != RH_OK) return rc;
-      rc = finish_passes(b, &kernel_ms);
+      rc = rh::finish_passes(b->lane, &kernel_ms);
       if (rc != RH_OK) return rc;
     }
-    RH_HIP_CHECK(hipMemcpy(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
-                           hipMemcpyDeviceToHost),
+    RH_HIP_CHECK(rh::copy_sync(b->results.data(), b->d_results.ptr, n * sizeof(PoaResult),
+                               hipMemcpyDeviceToHost, b->lane.stream),
                  RH_ERR_DEVICE);
   }
   b->stats.kernel_ms = kernel_ms;
   b->stats.cells = 0;
   for (uint32_t w = 0; w < n; ++w) {
-    if (b->shallow[w]) continue;
+    if (b->packed.shallow[w]) continue;
     const PoaResult& r = b->results[w];
     if (r.status == rh::kPoaPolished || r.status == rh::kPoaPolishedUntrimmed) {
       b->stats.cells += r.cells;
@@ -718,6 +720,66 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
   return rh_poa_wait(b);
 }
 
+int32_t rh_poa_autotune(rh_poa_batch* b, rh_poa_tuning* out) {
+  rh::clear_error();
+  if (b == nullptr || out == nullptr || !b->uploaded) {
+    rh::set_error("rh_poa_autotune: null argument or nothing uploaded");
+    return RH_ERR_ARGUMENT;
+  }
+  if (b->submitted) {
+    rh::set_error("rh_poa_autotune: the batch is running (call rh_poa_wait first)");
+    return RH_ERR_STATE;
+  }
+  RH_HIP_CHECK(hipSetDevice(b->lane.device), RH_ERR_DEVICE);
+  *out = rh_poa_tuning{};
+  std::vector<uint32_t> bins[2];
+  for (uint32_t w : b->packed.gpu_order) bins[rh::layer_bin(b->packed.windows[w])].push_back(w);
+  // DP rows kept in LDS per wave: deeper rings keep more rows on chip but
+  // leave room for fewer waves per CU (occupancy follows the LDS bytes)
+  static const uint32_t kMainRings[] = {0, 4, 6, 8, 10, 12, 16};
+  static const uint32_t kLongRings[] = {0, 2, 3, 4, 6, 8};
+  const uint32_t row_cap = b->config.tune_rows ? b->config.tune_rows : 4096;
+  for (int k = 0; k < 2; ++k) {
+    if (bins[k].empty()) continue;
+    const uint32_t* rings = k == 0 ? kMainRings : kLongRings;
+    const size_t n_rings = k == 0 ? sizeof(kMainRings) / 4 : sizeof(kLongRings) / 4;
+    double best = -1;
+    uint32_t best_ring = 0;
+    std::vector<uint32_t> tried;
+    for (size_t r = 0; r < n_rings; ++r) {
+      rh::PoaConfig c = b->config;
+      c.bin_ring[k] = rings[r];
+      std::vector<std::vector<uint32_t>> orders;
+      std::vector<PassLaunch> passes;
+      std::vector<uint32_t> refused;
+      int32_t rc = rh::plan_bins(b->lane, b->packed.windows.data(), bins[k], row_cap,
+                                 RH_POA_BATCHED, c, &orders, &passes, &refused);
+      if (rc != RH_OK) return rc;
+      if (passes.empty()) break;  // the bin does not fit the LDS limit at all
+      const uint32_t ring = passes[0].plan.ring;  // after the LDS limit
+      if (std::find(tried.begin(), tried.end(), ring) != tried.end()) continue;
+      tried.push_back(ring);
+      std::vector<const std::vector<uint32_t>*> ptrs{&orders[0]};
+      double ms = 0;
+      rc = rh::launch_passes(b->lane, b->tables(), c, ptrs, passes, RH_POA_BATCHED,
+                             &b->stats.launches);
+      if (rc == RH_OK) rc = rh::finish_passes(b->lane, &ms);
+      if (rc != RH_OK) return rc;
+      out->candidates++;
+      if (r == 0) out->default_ms[k] = ms;
+      if (best < 0 || ms < best) {
+        best = ms;
+        best_ring = rings[r];
+        out->ring_rows[k] = ring;
+      }
+    }
+    b->config.bin_ring[k] = best_ring;
+    out->kernel_ms[k] = best < 0 ? 0 : best;
+  }
+  b->ran = false;  // the timing runs leave no complete results: run the batch
+  return RH_OK;
+}
+
 int32_t rh_poa_capacity(int32_t device, uint64_t* free_bytes, uint64_t* total_bytes) {
   rh::clear_error();
   if (free_bytes == nullptr || total_bytes == nullptr) {
@@ -735,18 +797,18 @@ int32_t rh_poa_capacity(int32_t device, uint64_t* free_bytes, uint64_t* total_by
 uint64_t rh_poa_device_bytes(const rh_poa_batch* b) {
   if (b == nullptr) return 0;
   return b->d_arena.bytes + b->d_windows.bytes + b->d_layers.bytes + b->d_out.bytes +
-         b->d_results.bytes + b->d_order.bytes + b->d_queue.bytes + b->d_scratch.bytes +
-         b->d_params.bytes;
+         b->d_results.bytes + b->lane.device_bytes();
 }
 
 uint64_t rh_poa_output_bound(const rh_poa_batch* b) {
   if (b == nullptr) {
     return 0;
   }
+  const rh::PackedWindows& P = b->packed;
   uint64_t total = 0;
-  for (size_t i = 0; i < b->windows.size(); ++i) {
+  for (size_t i = 0; i < P.windows.size(); ++i) {
     const bool host_rejected = b->results[i].status == rh::kPoaRejectedOnHost;
-    total += b->shallow[i] || host_rejected ? b->backbone_len[i] : b->windows[i].out_cap;
+    total += P.shallow[i] || host_rejected ? P.windows[i].backbone_len : P.windows[i].out_cap;
   }
   return total;
 }
@@ -762,33 +824,34 @@ int32_t rh_poa_download(rh_poa_batch* b, char* out, uint64_t capacity, uint64_t*
     rh::set_error("rh_poa_download: null argument");
     return RH_ERR_ARGUMENT;
   }
-  RH_HIP_CHECK(hipSetDevice(b->device), RH_ERR_DEVICE);
+  RH_HIP_CHECK(hipSetDevice(b->lane.device), RH_ERR_DEVICE);
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<uint8_t> dev_out(b->out_bytes);
-  if (b->out_bytes > 0) {
-    RH_HIP_CHECK(hipMemcpy(dev_out.data(), b->d_out.ptr, b->out_bytes, hipMemcpyDeviceToHost),
+  const rh::PackedWindows& P = b->packed;
+  std::vector<uint8_t> dev_out(P.out_bytes);
+  if (P.out_bytes > 0) {
+    RH_HIP_CHECK(rh::copy_sync(dev_out.data(), b->d_out.ptr, P.out_bytes,
+                               hipMemcpyDeviceToHost, b->lane.stream),
                  RH_ERR_DEVICE);
   }
   uint64_t pos = 0;
-  for (size_t i = 0; i < b->windows.size(); ++i) {
+  for (size_t i = 0; i < P.windows.size(); ++i) {
     const uint8_t* src;
     uint32_t len;
     uint8_t st;
     const int32_t rs = b->results[i].status;
-    if (b->shallow[i]) {
-      src = b->arena.data() + b->backbone_off[i];
-      len = b->backbone_len[i];
+    if (P.shallow[i]) {
+      src = P.arena.data() + P.windows[i].backbone_off;
+      len = P.windows[i].backbone_len;
       st = RH_WINDOW_UNPOLISHED;
     } else if (rs != rh::kPoaPolished && rs != rh::kPoaPolishedUntrimmed) {
       // outside the device limits: the backbone, for the caller's CPU path
-      src = b->arena.data() + b->backbone_off[i];
-      len = b->backbone_len[i];
+      src = P.arena.data() + P.windows[i].backbone_off;
+      len = P.windows[i].backbone_len;
       st = RH_WINDOW_REJECTED;
     } else {
-      src = dev_out.data() + b->windows[i].out_off;
+      src = dev_out.data() + P.windows[i].out_off;
       len = b->results[i].length;
-      st = b->results[i].status == rh::kPoaPolishedUntrimmed ? RH_WINDOW_POLISHED_UNTRIMMED
-                                                              : RH_WINDOW_POLISHED;
+      st = rs == rh::kPoaPolishedUntrimmed ? RH_WINDOW_POLISHED_UNTRIMMED : RH_WINDOW_POLISHED;
     }
     if (pos + len > capacity) {
       rh::set_error("rh_poa_download: output capacity too small");
@@ -812,12 +875,12 @@ int32_t rh_poa_phase_ticks(const rh_poa_batch* b, uint64_t* ticks, uint32_t n,
   }
   if (ticks_per_second != nullptr) {
     int khz = 0;
-    RH_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device),
+    RH_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->lane.device),
                  RH_ERR_DEVICE);
     *ticks_per_second = static_cast<uint64_t>(khz) * 1000ull;
   }
   for (uint32_t k = 0; k < n; ++k) ticks[k] = 0;
-  for (uint32_t w : b->gpu_order) {
+  for (uint32_t w : b->packed.gpu_order) {
     for (uint32_t k = 0; k < n && k < static_cast<uint32_t>(rh::kPoaPhases); ++k) {
       ticks[k] += b->results[w].phase[k];
     }

@@ -46,6 +46,9 @@ struct EngineConfig {
   uint32_t aligner_batches = 0;
   bool banded = false;
   uint32_t aligner_band_width = 0;
+  // a device serving loop shared by several polishers (racon --serve): the
+  // windows go to it as requests instead of to batches of their own
+  void* poa_server = nullptr;
 };
 
 // Per-window outcome of generate_consensus.

@@ -149,6 +149,10 @@ class HipEngine : public PolishEngine {
                           std::vector<uint8_t>& status,
                           const std::function<void(uint64_t)>& ready) override {
     status.assign(windows.size(), kWindowUnpolished);
+    if (config_.poa_server != nullptr) {
+      serve_consensus(static_cast<rh_server*>(config_.poa_server), windows, status, ready);
+      return;
+    }
     const uint64_t kChunkBytes = 1ull << 30;
     std::vector<std::pair<uint64_t, uint64_t>> chunks;
     for (uint64_t first = 0; first < windows.size();) {
@@ -225,21 +229,7 @@ class HipEngine : public PolishEngine {
       if (rc != RH_OK) die("rh_poa_download", rc);
       for (uint64_t i = first; i < last; ++i) {
         const uint64_t k = i - first;
-        if (codes[k] == RH_WINDOW_REJECTED) {
-          // racon's CUDA path would polish it on the CPU; this binary has
-          // only the device path, so it stops rather than print the backbone
-          fprintf(stderr,
-                  "[racon::HipEngine] error: window %lu of target %lu (rank %u, %u layers) is "
-                  "outside the device limits; no CPU path in this build\n",
-                  static_cast<unsigned long>(i), static_cast<unsigned long>(windows[i]->id()),
-                  windows[i]->rank(), windows[i]->size() - 1);
-          exit(1);
-        }
-        windows[i]->set_consensus(std::string(out.data() + offsets[k], lengths[k]));
-        status[i] = codes[k] == RH_WINDOW_UNPOLISHED
-                        ? kWindowUnpolished
-                        : (codes[k] == RH_WINDOW_POLISHED_UNTRIMMED ? kWindowPolishedChimeric
-                                                                   : kWindowPolished);
+        take_result(windows, i, out.data() + offsets[k], lengths[k], codes[k], status);
       }
       if (ready) ready(last);
     }
@@ -248,6 +238,99 @@ class HipEngine : public PolishEngine {
   }
 
  private:
+  // Sets window i's consensus and status from a device result, or stops on
+  // a window the device refused (this binary has no CPU path).
+  static void take_result(std::vector<std::shared_ptr<Window>>& windows, uint64_t i,
+                          const char* data, uint32_t length, uint8_t code,
+                          std::vector<uint8_t>& status) {
+    if (code == RH_WINDOW_REJECTED) {
+      // racon's CUDA path would polish it on the CPU; this binary has only
+      // the device path, so it stops rather than print the backbone
+      fprintf(stderr,
+              "[racon::HipEngine] error: window %lu of target %lu (rank %u, %u layers) is "
+              "outside the device limits; no CPU path in this build\n",
+              static_cast<unsigned long>(i), static_cast<unsigned long>(windows[i]->id()),
+              windows[i]->rank(), windows[i]->size() - 1);
+      exit(1);
+    }
+    windows[i]->set_consensus(std::string(data, length));
+    status[i] = code == RH_WINDOW_UNPOLISHED
+                    ? kWindowUnpolished
+                    : (code == RH_WINDOW_POLISHED_UNTRIMMED ? kWindowPolishedChimeric
+                                                            : kWindowPolished);
+  }
+
+  // The windows as requests of a shared serving loop: chunks of at most a
+  // third of its pool, all submitted at once (the server admits them as its
+  // pool frees), then collected in order.
+  void serve_consensus(rh_server* server, std::vector<std::shared_ptr<Window>>& windows,
+                       std::vector<uint8_t>& status, const std::function<void(uint64_t)>& ready) {
+    rh_server_stats st{};
+    int32_t rc = rh_server_get_stats(server, &st);
+    if (rc != RH_OK) die("rh_server_get_stats", rc);
+    const uint64_t chunk_bytes = std::max<uint64_t>(1, st.pool_bytes / 3);
+    struct Chunk {
+      uint64_t first, last, request;
+    };
+    std::vector<Chunk> chunks;
+    std::vector<rh_window> ws;
+    std::vector<rh_layer> ls;
+    for (uint64_t first = 0; first < windows.size();) {
+      uint64_t last = first, bytes = 0;
+      while (last < windows.size() && (last == first || bytes < chunk_bytes)) {
+        const Window& w = *windows[last];
+        for (uint32_t k = 0; k < w.size(); ++k) bytes += 3ull * w.sequence(k).second + 64;
+        ++last;
+      }
+      ws.clear();
+      ls.clear();
+      for (uint64_t i = first; i < last; ++i) {
+        const Window& w = *windows[i];
+        for (uint32_t k = 1; k < w.size(); ++k) {
+          ls.push_back(rh_layer{w.sequence(k).first, w.quality(k).first, w.sequence(k).second,
+                                w.position(k).first, w.position(k).second});
+        }
+      }
+      uint64_t at = 0;
+      for (uint64_t i = first; i < last; ++i) {
+        const Window& w = *windows[i];
+        ws.push_back(rh_window{w.sequence(0).first, w.quality(0).first, w.sequence(0).second,
+                               w.size() - 1, ls.data() + at,
+                               w.type() == WindowType::kTGS ? 1u : 0u});
+        at += w.size() - 1;
+      }
+      Chunk c{first, last, 0};
+      rc = rh_server_submit(server, static_cast<uint32_t>(ws.size()), ws.data(), &c.request);
+      if (rc != RH_OK) die("rh_server_submit", rc);
+      chunks.push_back(c);
+      first = last;
+    }
+    std::vector<char> out;
+    std::vector<uint64_t> offsets;
+    std::vector<uint32_t> lengths;
+    std::vector<uint8_t> codes;
+    for (const Chunk& c : chunks) {
+      rc = rh_server_wait(server, c.request, -1);
+      if (rc != RH_OK) die("rh_server_wait", rc);
+      rh_request_info info{};
+      rc = rh_server_request_info(server, c.request, &info);
+      if (rc != RH_OK) die("rh_server_request_info", rc);
+      const uint64_t n = c.last - c.first;
+      out.resize(std::max<uint64_t>(1, info.output_bound));
+      offsets.resize(n);
+      lengths.resize(n);
+      codes.resize(n);
+      rc = rh_server_fetch(server, c.request, out.data(), out.size(), offsets.data(),
+                           lengths.data(), codes.data());
+      if (rc != RH_OK) die("rh_server_fetch", rc);
+      for (uint64_t k = 0; k < n; ++k) {
+        take_result(windows, c.first + k, out.data() + offsets[k], lengths[k], codes[k], status);
+      }
+      if (ready) ready(c.last);
+    }
+    if (ready && chunks.empty()) ready(0);
+  }
+
   EngineConfig config_;
 };
 

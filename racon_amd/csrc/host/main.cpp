@@ -7,6 +7,11 @@
 // output): --report <path> writes the phase timings as JSON;
 // --dump-overlaps <path> writes the overlap alignment inputs;
 // --dump-windows <path> writes the windows after initialization and exits.
+//
+// --serve <job list> polishes many jobs in one process on one device (each
+// job's output file holds what racon prints for it alone): see serve.hpp;
+// --serve-budget <GiB>, --serve-executors <n> and --serve-jobs <n> size the
+// device pool, the kernel streams and the jobs prepared at once.
 #include <getopt.h>
 
 #include <cstdio>
@@ -16,6 +21,7 @@
 
 #include "polisher.hpp"
 #include "sequence.hpp"
+#include "serve.hpp"
 
 #ifndef RACON_VERSION
 #define RACON_VERSION "v1.4.17"
@@ -27,6 +33,10 @@ static const int32_t CUDAALIGNER_BAND_WIDTH_INPUT_CODE = 10001;
 static const int32_t REPORT_INPUT_CODE = 10002;
 static const int32_t DUMP_WINDOWS_CODE = 10003;
 static const int32_t DUMP_OVERLAPS_CODE = 10004;
+static const int32_t SERVE_CODE = 10005;
+static const int32_t SERVE_BUDGET_CODE = 10006;
+static const int32_t SERVE_EXECUTORS_CODE = 10007;
+static const int32_t SERVE_JOBS_CODE = 10008;
 
 static struct option options[] = {
     {"include-unpolished", no_argument, 0, 'u'},
@@ -48,6 +58,10 @@ static struct option options[] = {
     {"report", required_argument, 0, REPORT_INPUT_CODE},
     {"dump-windows", required_argument, 0, DUMP_WINDOWS_CODE},
     {"dump-overlaps", required_argument, 0, DUMP_OVERLAPS_CODE},
+    {"serve", required_argument, 0, SERVE_CODE},
+    {"serve-budget", required_argument, 0, SERVE_BUDGET_CODE},
+    {"serve-executors", required_argument, 0, SERVE_EXECUTORS_CODE},
+    {"serve-jobs", required_argument, 0, SERVE_JOBS_CODE},
     {0, 0, 0, 0}};
 
 static void help();
@@ -74,6 +88,9 @@ int main(int argc, char** argv) {
   bool cuda_banded_alignment = false;
   std::string report_path;
   std::string dump_windows_path, dump_overlaps_path;
+  std::string serve_path;
+  double serve_budget_gib = 8.0;
+  uint32_t serve_executors = 4, serve_jobs = 4;
 
   std::string optstring = "ufw:q:e:m:x:g:t:hbc::";
 
@@ -108,12 +125,35 @@ int main(int argc, char** argv) {
       case REPORT_INPUT_CODE: report_path = optarg; break;
       case DUMP_WINDOWS_CODE: dump_windows_path = optarg; break;
       case DUMP_OVERLAPS_CODE: dump_overlaps_path = optarg; break;
+      case SERVE_CODE: serve_path = optarg; break;
+      case SERVE_BUDGET_CODE: serve_budget_gib = atof(optarg); break;
+      case SERVE_EXECUTORS_CODE: serve_executors = atoi(optarg); break;
+      case SERVE_JOBS_CODE: serve_jobs = atoi(optarg); break;
       default: exit(1);
     }
   }
 
   for (int32_t i = optind; i < argc; ++i) {
     input_paths.emplace_back(argv[i]);
+  }
+
+  if (!serve_path.empty()) {
+    racon::ServeOptions so;
+    so.type = type == 0 ? racon::PolisherType::kC : racon::PolisherType::kF;
+    so.window_length = window_length;
+    so.quality_threshold = quality_threshold;
+    so.error_threshold = error_threshold;
+    so.trim = trim;
+    so.match = match;
+    so.mismatch = mismatch;
+    so.gap = gap;
+    so.num_threads = num_threads;
+    so.drop_unpolished = drop_unpolished_sequences;
+    so.aligner_band_width = cudaaligner_band_width;
+    so.budget_bytes = static_cast<uint64_t>(serve_budget_gib * (1ull << 30));
+    so.executors = serve_executors;
+    so.concurrent_jobs = serve_jobs;
+    return racon::serve_jobs(racon::read_jobs(serve_path), so);
   }
 
   if (input_paths.size() < 3) {

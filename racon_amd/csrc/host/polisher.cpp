@@ -54,7 +54,7 @@ std::unique_ptr<Polisher> createPolisher(
     double quality_threshold, double error_threshold, bool trim, int8_t match,
     int8_t mismatch, int8_t gap, uint32_t num_threads, uint32_t cuda_batches,
     bool cuda_banded_alignment, uint32_t cudaaligner_batches,
-    uint32_t cudaaligner_band_width) {
+    uint32_t cudaaligner_band_width, void* poa_server) {
   // polisher.cpp:62-133 (messages kept verbatim)
   if (type != PolisherType::kC && type != PolisherType::kF) {
     fprintf(stderr, "[racon::createPolisher] error: invalid polisher type!\n");
@@ -100,6 +100,7 @@ std::unique_ptr<Polisher> createPolisher(
   config.aligner_batches = cudaaligner_batches;
   config.banded = cuda_banded_alignment;
   config.aligner_band_width = cudaaligner_band_width;
+  config.poa_server = poa_server;
 
   return std::unique_ptr<Polisher>(new Polisher(
       sequences_path, sformat, overlaps_path, oformat, target_path, tformat, type,

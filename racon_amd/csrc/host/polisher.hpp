@@ -39,7 +39,7 @@ std::unique_ptr<Polisher> createPolisher(
     double quality_threshold, double error_threshold, bool trim, int8_t match,
     int8_t mismatch, int8_t gap, uint32_t num_threads, uint32_t cuda_batches = 0,
     bool cuda_banded_alignment = false, uint32_t cudaaligner_batches = 0,
-    uint32_t cudaaligner_band_width = 0);
+    uint32_t cudaaligner_band_width = 0, void* poa_server = nullptr);
 
 // Phase timings of the last run (seconds), for reports.
 struct PolisherTimings {

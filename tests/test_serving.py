@@ -1,155 +1,21 @@
-"""Serving loop: admission by budget, co-batching, per-request reassembly.
+"""Serving loop (rh_server_* through racon_amd.serving): admission by device
+budget, first come first served, windows of several requests in one launch,
+and every request's result the same alone and in the mix.
 
-The scheduler runs on the CPU with a stand-in executor (a deterministic
-function of each window, with a delay so launches overlap); the GPU test
-checks that every request's consensus is the same served alone and in a mix.
+CPU tests check the argument handling of the C ABI (no device needed); the
+GPU tests run the native server.
 """
+import ctypes
+import os
+import subprocess
 import sys
-import threading
-import time
 
+import numpy as np
 import pytest
 
 from racon_amd import _lib
-from racon_amd.serving import PolishServer, percentile, window_bytes
+from racon_amd.serving import PolishServer, percentile
 from synth import make_windows, oracle_args
-
-
-def fake_consensus(w):
-    return w.backbone[::-1] + bytes([65 + len(w.layers) % 26])
-
-
-class FakeExecutor:
-    def __init__(self, delay=0.01, fail=False):
-        self.delay = delay
-        self.fail = fail
-        self.calls = 0
-
-    def __call__(self, windows):
-        self.calls += 1
-        time.sleep(self.delay)
-        if self.fail:
-            raise RuntimeError("device rejected the launch")
-        return [fake_consensus(w) for w in windows], [1] * len(windows)
-
-
-def requests_of(seed, sizes, length=60, depth=4):
-    return [make_windows(seed + i, n, length=length, depth=depth) for i, n in enumerate(sizes)]
-
-
-def test_window_bytes_counts_sequences_and_qualities():
-    w = make_windows(1, 1, length=50, depth=3)[0]
-    expect = len(w.backbone) + len(w.backbone_quality) + sum(
-        len(l.sequence) + len(l.quality) for l in w.layers)
-    assert window_bytes(w) == expect
-
-
-def test_mix_matches_alone_and_launches_mix_requests():
-    reqs = requests_of(100, [5, 1, 9, 3, 7, 2])
-    with PolishServer(10**9, executors=2, launch_windows=6,
-                      executor_factory=lambda: FakeExecutor(0.05)) as srv:
-        handles = []
-        threads = []
-
-        def client(ws, delay):
-            time.sleep(delay)
-            handles.append(srv.submit(ws))
-
-        for k, ws in enumerate(reqs):
-            t = threading.Thread(target=client, args=(ws, 0.005 * k))
-            t.start()
-            threads.append(t)
-        for t in threads:
-            t.join()
-        for h in handles:
-            h.wait(10)
-    assert len(handles) == len(reqs)
-    for h in handles:
-        assert h.consensus == [fake_consensus(w) for w in h.windows]
-        assert h.status == [1] * len(h.windows)
-        assert h.done >= h.admitted >= h.arrival
-    assert srv.in_use() == 0
-    assert sum(n for _, n, _ in srv.launch_log) == sum(len(r) for r in reqs)
-    assert max(n for _, n, _ in srv.launch_log) <= 6
-    # windows of different requests share launches
-    assert any(k > 1 for _, _, k in srv.launch_log)
-
-
-def test_budget_admits_in_arrival_order():
-    reqs = requests_of(200, [4, 4, 1, 4])
-    sizes = [sum(window_bytes(w) for w in ws) for ws in reqs]
-    budget = max(sizes) + min(sizes) - 1  # never two of the big ones at once
-    with PolishServer(budget, executors=2, launch_windows=2,
-                      executor_factory=lambda: FakeExecutor(0.01)) as srv:
-        hs = [srv.submit(ws) for ws in reqs]
-        assert srv.in_use() <= budget
-        for h in hs:
-            h.wait(10)
-    # first come, first served: no request is admitted before an earlier one
-    for a, b in zip(hs, hs[1:]):
-        assert b.admitted >= a.admitted
-    # request 1 waited for request 0's bytes to come back
-    assert hs[1].admitted >= hs[0].done
-    for h in hs:
-        assert h.consensus == [fake_consensus(w) for w in h.windows]
-
-
-def test_submit_many_shares_the_first_launch():
-    reqs = requests_of(700, [2, 2, 3])
-    with PolishServer(10**9, executors=1, launch_windows=5,
-                      executor_factory=lambda: FakeExecutor(0.01)) as srv:
-        hs = srv.submit_many(reqs)
-        for h in hs:
-            h.wait(10)
-    assert [n for _, n, _ in srv.launch_log] == [5, 2]
-    assert srv.launch_log[0][2] == 3
-    assert hs[2].launches == {0, 1}
-    for h in hs:
-        assert h.consensus == [fake_consensus(w) for w in h.windows]
-
-
-def test_warm_runs_every_executor_once():
-    made = []
-
-    def factory():
-        made.append(FakeExecutor(0))
-        return made[-1]
-
-    with PolishServer(10**6, executors=3, executor_factory=factory) as srv:
-        srv.warm(make_windows(800, 2, length=40, depth=2))
-        assert [ex.calls for ex in made] == [1, 1, 1]
-        srv.submit(make_windows(801, 1, length=40, depth=2)).wait(10)
-    assert sum(ex.calls for ex in made) == 4
-    assert srv.launch_log and srv.launch_log[0][0] == 0
-
-
-def test_request_over_budget_is_refused():
-    ws = make_windows(300, 3, length=80, depth=4)
-    with PolishServer(window_bytes(ws[0]), executors=1,
-                      executor_factory=lambda: FakeExecutor(0)) as srv:
-        with pytest.raises(ValueError, match="exceeds the budget"):
-            srv.submit(ws)
-        srv.submit(ws[:1]).wait(10)
-
-
-def test_empty_request_and_close_drains():
-    with PolishServer(10**6, executors=1, executor_factory=lambda: FakeExecutor(0.01)) as srv:
-        empty = srv.submit([])
-        busy = srv.submit(make_windows(400, 3, length=40, depth=2))
-    assert empty.wait(0).consensus == []
-    assert busy.wait(0).consensus == [fake_consensus(w) for w in busy.windows]
-    with pytest.raises(RuntimeError, match="closed"):
-        srv.submit([])
-
-
-def test_executor_failure_reaches_every_request_of_the_launch():
-    with PolishServer(10**6, executors=1, launch_windows=8,
-                      executor_factory=lambda: FakeExecutor(0, fail=True)) as srv:
-        hs = [srv.submit(ws) for ws in requests_of(500, [2, 3])]
-        for h in hs:
-            with pytest.raises(RuntimeError, match="rejected"):
-                h.wait(10)
-    assert srv.in_use() == 0
 
 
 def test_percentile_nearest_rank():
@@ -160,6 +26,33 @@ def test_percentile_nearest_rank():
     assert percentile([7.0], 99) == 7.0
 
 
+def test_server_arguments_are_checked():
+    lib = _lib.load_library()
+    h = ctypes.c_void_p()
+    # no executors, and a budget below 1 MiB: refused before any device call
+    assert lib.rh_server_create(0, 3, -5, -4, 1, 1 << 30, 0, 2048, ctypes.byref(h)) == -1
+    assert b"executors" in lib.rh_last_error()
+    assert lib.rh_server_create(0, 3, -5, -4, 1, 1000, 2, 2048, ctypes.byref(h)) == -1
+    assert not h.value
+    with pytest.raises(ValueError):
+        PolishServer(0)
+
+
+@pytest.mark.skipif(_lib.load_library().rh_device_count() > 0, reason="a device is present")
+def test_server_without_device_fails_loudly():
+    with pytest.raises(RuntimeError, match="rh_server_create"):
+        PolishServer(1 << 30)
+
+
+def batch_alone(ws):
+    b = _lib.PoaBatch(3, -5, -4, trim=True)
+    b.upload(ws)
+    b.run()
+    out = b.download()
+    b.close()
+    return out
+
+
 @pytest.mark.gpu
 def test_gpu_mix_matches_alone():
     sys.path.insert(0, _lib.PKG_DIR + "/../oracle")
@@ -167,27 +60,128 @@ def test_gpu_mix_matches_alone():
 
     reqs = [make_windows(600, 3, length=200, depth=10),
             make_windows(601, 6, length=300, depth=14),
-            make_windows(602, 1, length=120, depth=2),   # < 3 sequences
+            make_windows(602, 2, length=120, depth=1),   # < 3 sequences: no kernel
             make_windows(603, 4, length=500, depth=20)]
-    alone = []
-    for ws in reqs:
-        b = _lib.PoaBatch(3, -5, -4, trim=True)
-        b.upload(ws)
-        b.run()
-        alone.append(b.download())
-        b.close()
-    with PolishServer(10**9, executors=2, launch_windows=5) as srv:
-        # one instant: the first launch takes request 0 and part of request 1
+    alone = [batch_alone(ws) for ws in reqs]
+    with PolishServer(1 << 30, executors=2, launch_windows=5) as srv:
+        # back to back: the first launch takes request 0 and part of request 1
         hs = srv.submit_many(reqs[:3])
         hs.append(srv.submit(reqs[3]))  # arrives while launches are in flight
         for h in hs:
             h.wait(120)
-    assert srv.launch_log[0][2] == 2
+        st = srv.stats()
+    print(f"stats {st}", flush=True)
+    assert st["completed"] == 4 and st["pool_in_use"] == 0
+    assert st["shared_launches"] >= 1
+    assert st["windows_launched"] == 3 + 6 + 4  # the shallow windows need no kernel
     for k, h in enumerate(hs):
-        print(f"request {k}: {len(h.windows)} windows, launches {sorted(h.launches)}",
-              flush=True)
+        print(f"request {k}: {len(h.windows)} windows, {h.launches} launches "
+              f"({h.shared_launches} shared)", flush=True)
         assert (h.consensus, h.status) == (alone[k][0], alone[k][1]), f"request {k}"
+        assert h.done >= h.admitted >= h.arrival
+    assert hs[0].shared_launches >= 1
     # and the mix against the oracle on the small request
     for w, cons, st in zip(reqs[0], hs[0].consensus, hs[0].status):
         ocons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
         assert (cons, st) == (ocons, ost)
+
+
+@pytest.mark.gpu
+def test_gpu_budget_admits_in_arrival_order():
+    reqs = [make_windows(610 + k, n, length=500, depth=30) for k, n in enumerate([60, 60, 4, 60])]
+    with PolishServer(1 << 30, executors=1) as probe:
+        sizes = [probe.submit(ws).wait(120).device_bytes for ws in reqs]
+    # a pool that holds the largest request but never two of the big ones
+    pool = max(sizes) + min(sizes) + 4096
+    budget = pool * 32 // 31 + 4096
+    with PolishServer(budget, executors=2, launch_windows=16) as srv:
+        hs = [srv.submit(ws) for ws in reqs]
+        for h in hs:
+            h.wait(120)
+        assert srv.stats()["pool_in_use"] == 0
+    print("sizes", sizes, "admitted", [h.admitted for h in hs], "done", [h.done for h in hs],
+          flush=True)
+    for a, b in zip(hs, hs[1:]):
+        assert b.admitted >= a.admitted  # first come, first served
+    assert hs[1].admitted >= hs[0].done  # request 1 waited for request 0's bytes
+    for ws, h in zip(reqs, hs):
+        assert (h.consensus, h.status) == batch_alone(ws)
+
+
+@pytest.mark.gpu
+def test_gpu_request_over_budget_is_refused_and_empty_requests_complete():
+    big = make_windows(620, 40, length=500, depth=30)
+    with PolishServer(1 << 20, executors=1) as srv:
+        with pytest.raises(RuntimeError, match="exceeds the server's budget"):
+            srv.submit(big)
+        empty = srv.submit([])
+        shallow = srv.submit(make_windows(621, 2, length=80, depth=1))
+        small = srv.submit(big[:1])
+        assert empty.wait(60).consensus == []
+        assert shallow.wait(60).status == [_lib.RH_WINDOW_UNPOLISHED] * 2
+        assert shallow.consensus == [w.backbone for w in shallow.windows]
+        assert (small.wait(60).consensus, small.status) == batch_alone(big[:1])
+
+
+# ---- cfg5-sized requests: contigs of 20 kbp to 2 Mbp at 20-100x ----
+
+CFG5_REQUESTS = [  # (seed, contig bp, depth): 40, ~400 and ~4,000 windows of 500 bp
+    (1051, 20000, 20),
+    (1052, 200000, 60),
+    (1053, 2000000, 100),
+]
+
+
+def cfg5_request_windows(workdir, seed, contig, depth):
+    """Windows of one polishing job: simgen contig + reads, then the product
+    CLI's input step and GPU overlap alignment (--dump-windows)."""
+    from racon_amd.dumps import read_windows
+
+    prefix = os.path.join(workdir, f"req{seed}")
+    bindir = os.path.join(_lib.PKG_DIR, "bin")
+    subprocess.run([os.path.join(bindir, "simgen"), "contig", str(seed), prefix, str(contig),
+                    str(depth), "8000", "4000", "0.12", "0.4:0.2:0.4"], check=True,
+                   stdout=subprocess.DEVNULL)
+    dump = prefix + ".windows.bin"
+    subprocess.run([os.path.join(bindir, "racon"), "-t", "16", "-w", "500", "-m", "3", "-x", "-5",
+                    "-g", "-4", "-q", "10", "-e", "0.3", "--dump-windows", dump,
+                    prefix + ".reads.fastq", prefix + ".paf", prefix + ".contigs.fasta"],
+                   check=True, stdout=subprocess.DEVNULL)
+    return read_windows(dump)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_gpu_cfg5_sized_requests_alone_and_mixed(tmp_path):
+    sys.path.insert(0, _lib.PKG_DIR + "/../oracle")
+    import oracle_py
+
+    reqs = []
+    for seed, contig, depth in CFG5_REQUESTS:
+        ws = cfg5_request_windows(str(tmp_path), seed, contig, depth)
+        print(f"request seed {seed}: {contig} bp at {depth}x -> {len(ws)} windows", flush=True)
+        reqs.append(ws)
+    assert [len(ws) for ws in reqs][0] >= 40 and len(reqs[2]) >= 3900
+    alone = [batch_alone(ws) for ws in reqs]
+    print("alone done", flush=True)
+    with PolishServer(8 << 30, executors=4, launch_windows=2048) as srv:
+        # the big job first; the small ones (and a second copy of the middle
+        # one) arrive together while it runs
+        order = [2, 0, 1, 1]
+        hs = [srv.submit(reqs[2])] + srv.submit_many([reqs[0], reqs[1], reqs[1]])
+        for h in hs:
+            h.wait(300)
+        st = srv.stats()
+    print(f"mixed stats {st}", flush=True)
+    for k, h in zip(order, hs):
+        print(f"request {k}: latency {h.latency:.3f} s, {h.launches} launches "
+              f"({h.shared_launches} shared)", flush=True)
+        assert (h.consensus, h.status) == alone[k], f"request {k}"
+    assert st["shared_launches"] >= 1
+    # the oracle on a sample of every request's windows
+    rng = np.random.default_rng(5)
+    for k, ws in enumerate(reqs):
+        sample = rng.choice(len(ws), size=min(12, len(ws)), replace=False)
+        for i in sample:
+            ocons, ost, _, _ = oracle_py.window_consensus(*oracle_args(ws[i]), True, 3, -5, -4)
+            assert (alone[k][0][i], alone[k][1][i]) == (ocons, ost), f"request {k} window {i}"
