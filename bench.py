@@ -291,6 +291,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--skip-banded", action="store_true",
+                    help="skip the banded (-b) comparison run")
     ap.add_argument("--kernel-only", action="store_true",
                     help="profiling runs: time only rh_poa_run (inputs resident), skip the "
                          "comparators and the CPU baseline")
@@ -447,6 +449,30 @@ def main() -> int:
             extra["batched_over_static_kernel"] = round(static_ms / kernel_avg, 3)
             log(f"static grid: kernel {static_ms:.1f} ms ({sst.launches} launches) vs batched "
                 f"{kernel_avg:.1f} ms")
+
+        # ---- racon -b: the banded approximation (not bit-exact; reported beside) ----
+        if not args.skip_banded:
+            bb = racon_amd.PoaBatch(m, x, g, trim=True)
+            bb.set_band(256)
+            bb.upload_packed(arr, n_windows, keep)
+            bb.run(racon_amd.RH_POA_BATCHED)  # sizes the slots
+            bms = []
+            for _ in range(2):
+                bb.run(racon_amd.RH_POA_BATCHED)
+                bms.append(bb.stats().kernel_ms)
+            bst = bb.stats()
+            bout, _ = bb.download()
+            bb.close()
+            band_ms = min(bms)
+            same = sum(1 for a, c in zip(bout, gpu_out) if a == c)
+            extra["banded"] = {
+                "band": 256, "kernel_ms": round(band_ms, 3),
+                "windows_per_s_kernel": round(n_windows / (band_ms / 1e3), 2),
+                "speedup_over_exact_kernel": round(kernel_avg / band_ms, 3),
+                "cells": int(bst.cells), "band_reruns": int(bst.band_reruns),
+                "windows_equal_exact": same, "windows": n_windows}
+            log(f"banded (-b): kernel {band_ms:.1f} ms vs exact {kernel_avg:.1f} ms, "
+                f"{same}/{n_windows} windows equal, {bst.band_reruns} rerun exactly")
 
         # ---- plain: one launch per window, same kernel body ----
         subset = windows[:args.plain_windows]

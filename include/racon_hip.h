@@ -108,6 +108,7 @@ typedef struct {
   uint32_t slots;       /* resident waves (slots) of the last launch    */
   uint32_t launches;    /* kernel launches of the last run              */
   uint64_t rejected;    /* windows left to the caller (RH_WINDOW_REJECTED) */
+  uint64_t band_reruns; /* banded windows rerun unbanded (path left the band) */
 } rh_poa_stats;
 
 /* Creates a batch on `device` with racon's scores (match > 0, mismatch and
@@ -168,6 +169,15 @@ int32_t rh_poa_get_stats(const rh_poa_batch* batch, rh_poa_stats* stats);
  * (GenomeWorks fixes its batch shape in BatchConfig, cudabatch.cpp:56-59). */
 int32_t rh_poa_set_tuning(rh_poa_batch* batch, uint32_t slots, uint32_t rows,
                           uint32_t ring_rows);
+
+/* racon -b (--cuda-banded-alignment, src/main.cpp:127-128; GenomeWorks'
+ * static band of 256, src/cuda/cudabatch.cpp:56-59): band = 256 computes
+ * each DP row of a layer longer than 256 only in 256 columns that follow
+ * the diagonal of the (subgraph rows x layer length) matrix - an
+ * approximation whose output may differ from racon's CPU path. A window
+ * whose alignment path would leave the band is rerun exactly (counted in
+ * rh_poa_stats.band_reruns). band = 0 (the default) is the exact DP. */
+int32_t rh_poa_set_band(rh_poa_batch* batch, uint32_t band);
 
 /* Autotuner: times the uploaded batch's kernel per layer-length bin (main
  * pass, long-layer pass) for each candidate depth of the DP row ring in LDS

@@ -63,6 +63,7 @@ class RhPoaStats(ctypes.Structure):
         ("slots", ctypes.c_uint32),
         ("launches", ctypes.c_uint32),
         ("rejected", ctypes.c_uint64),
+        ("band_reruns", ctypes.c_uint64),
     ]
 
 
@@ -168,6 +169,7 @@ _SIGNATURES = {
     "rh_poa_set_tuning": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.c_uint32]),
     "rh_poa_autotune": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhPoaTuning)]),
+    "rh_poa_set_band": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_uint32]),
     "rh_poa_phase_ticks": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     "rh_poa_window_costs": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64),
@@ -386,6 +388,10 @@ class PoaBatch:
     def set_tuning(self, slots: int = 0, rows: int = 0, ring_rows: int = 0) -> None:
         _check(self.lib.rh_poa_set_tuning(self.handle, slots, rows, ring_rows),
                "rh_poa_set_tuning")
+
+    def set_band(self, band: int) -> None:
+        """rh_poa_set_band: 256 for racon -b's banded approximation, 0 exact."""
+        _check(self.lib.rh_poa_set_band(self.handle, band), "rh_poa_set_band")
 
     def autotune(self) -> dict:
         """rh_poa_autotune on the uploaded batch: the fastest LDS ring depth

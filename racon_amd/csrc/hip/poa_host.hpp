@@ -49,6 +49,7 @@ struct PoaConfig {
   uint32_t tune_slots = 0, tune_rows = 0, tune_ring = 0;
   uint32_t slot_share = 1;
   uint32_t bin_ring[2] = {0, 0};  // per layer-length bin ring rows (autotuned; 0: tune_ring)
+  uint32_t band = 0;              // racon -b: band rows (256 columns) for long layers
 };
 
 // Layer-length bin of a window: 0 main pass, 1 long-layer pass.

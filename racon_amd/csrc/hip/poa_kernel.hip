@@ -1208,6 +1208,260 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
   return best_row;
 }
 
+// ---- banded rows: the opt-in approximation of racon -b ----
+//
+// Each DP row r (1..V) computes only 256 columns, 4 per lane: columns
+// lo(r) + 1 .. lo(r) + 256, where lo(r) follows the diagonal of the V x L
+// matrix (centred on column r L / V, a multiple of 4, inside [0, L + 3]).
+// lo never decreases with r, so a predecessor p < r holds row r's lane k in
+// its lane k + (lo(r) - lo(p)) / 4. Values outside a row's band read as
+// kBandNeg; a window whose traceback would leave the band is rerun unbanded
+// (kPoaBandMiss). Used for layers longer than 256 only, and only in the
+// packed 16-bit form (kBandNeg sits below every value that form holds).
+constexpr int kBandW = 4;
+constexpr int kBand = kWave * kBandW;  // 256 columns
+constexpr int32_t kBandNeg = -32768 + 64;
+
+__device__ __forceinline__ int band_lo(int r, int V, int L) {
+  // (single-precision r * (L / V): monotone in r, the same in the DP and the
+  // traceback)
+  const int c = static_cast<int>(static_cast<float>(r) * (static_cast<float>(L) /
+                                                           static_cast<float>(V))) - kBand / 2;
+  const int top = (L - kBand + 3) & ~3;
+  return min(max(c, 0) & ~3, top);
+}
+
+// Whether cell (i, j) lies in row i's band (row 0, the virtual start row,
+// holds every column).
+__device__ __forceinline__ bool in_band(int i, int j, int V, int L) {
+  if (i <= 0) return true;
+  const int lo = band_lo(i, V, L);
+  return (j > lo && j <= lo + kBand) || (j == 0 && lo == 0);
+}
+
+// Predecessor row p (entry e) at the lanes of a band row whose band starts
+// after column lo: pp[q] = (G[p][j0 + 2q], G[p][j0 + 2q + 1]) with
+// j0 = lo + 1 + 4 lane, kBandNeg outside p's band; returns G[p][j0 - 1] in
+// the low half.
+__device__ __forceinline__ uint32_t load_band(const int16_t* ring, int pitch, const Slot& s, int e,
+                                              int lo, int V, int L, int lane, uint32_t* pp) {
+  const int p = e & kPredRowMask;
+  const int sl = static_cast<unsigned>(e) >> kPredRowBits;
+  const uint32_t neg = pk_splat(kBandNeg);
+  if (p == 0) {  // the virtual start row: G = 0 everywhere
+    pp[0] = pp[1] = 0u;
+    return 0u;
+  }
+  const int lo_p = band_lo(p, V, L);
+  const int shift = max(lo - lo_p, 0) >> 2;  // p < r: lo_p <= lo
+  const int src = lane + shift;
+  if (sl < kSlotGlobal) {
+    const int16_t* row = ring + sl * pitch;
+    const bool in = src < kWave;
+    const uint2 v = *reinterpret_cast<const uint2*>(row + 4 * (in ? src : 0));
+    pp[0] = in ? v.x : neg;
+    pp[1] = in ? v.y : neg;
+    // column lo of p: the last one of p's lane shift - 1, or p's column 0
+    uint32_t c0 = neg;
+    if (shift > 0 && shift <= kWave) {
+      c0 = static_cast<uint16_t>(row[4 * (shift - 1) + 3]);
+    } else if (shift == 0 && lo == 0) {
+      c0 = static_cast<uint16_t>(row[pitch - 1]);
+    }
+    const uint32_t prev = from_prev_lane(pp[1]) >> 16;
+    return lane == 0 ? c0 : prev;
+  }
+  // a row kept in the slot's global matrix (int32, absolute columns)
+  const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+  int32_t hv[kBandW + 1];
+#pragma unroll
+  for (int u = 0; u <= kBandW; ++u) {
+    const int j = lo + 4 * lane + u;
+    const bool valid = (j > lo_p && j <= lo_p + kBand && j <= L) || (j == 0 && lo_p == 0);
+    hv[u] = valid ? row[j] : kBandNeg;
+  }
+  wait_stores();
+  pp[0] = (static_cast<uint32_t>(hv[1]) & 0xffffu) | (static_cast<uint32_t>(hv[2]) << 16);
+  pp[1] = (static_cast<uint32_t>(hv[3]) & 0xffffu) | (static_cast<uint32_t>(hv[4]) << 16);
+  return static_cast<uint32_t>(hv[0]) & 0xffffu;
+}
+
+// One band row: packed_compute with the lane-0 chain starting from column 0
+// only when the band starts there, values kept at or above kBandNeg, then
+// the stores of packed_store at the band's columns. Returns H[row][L], or
+// kNegInf when column L is outside the band.
+template <bool kMulti>
+__device__ __forceinline__ int32_t band_row(const PackedStrip<kBandW>& ps, const uint32_t* pp,
+                                            uint32_t hp0, const uint32_t* kk, uint32_t k0,
+                                            uint32_t b, uint32_t pay0, int lane, int lo, int L,
+                                            int32_t gap, int own, int pitch, int16_t* ring,
+                                            int32_t* Hg, uint16_t* Cr) {
+  constexpr int H2 = kBandW / 2;
+  const uint32_t bb = b * 0x10001u;
+  const uint32_t neg = pk_splat(kBandNeg);
+  const int32_t h0 = pk_half(hp0 + static_cast<uint32_t>(gap), 0);  // column 0 when lo == 0
+  uint32_t carry = lane == 0 ? (lo == 0 ? pk_splat(h0) : neg) : 0x80008000u;
+  uint32_t bd[H2], bv[H2], pre[H2], h[H2], cd[H2];
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    const uint32_t d = q == 0 ? ((pp[0] << 16) | (hp0 & 0xffffu))
+                              : __builtin_amdgcn_alignbit(pp[q], pp[q - 1], 16);
+    const uint32_t mis = pk_min_u(ps.base[q] ^ bb, ps.one);
+    bd[q] = pk_mad(mis, ps.diff, pk_add(d, ps.match_gap));
+    bv[q] = pk_add(pp[q], ps.gap);
+    carry = pk_max_hi(pk_scan(pk_max(bd[q], bv[q])), carry);
+    pre[q] = carry;
+  }
+  const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
+  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
+  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
+#pragma unroll
+  for (int q = 0; q < H2; ++q) {
+    h[q] = pk_max(pk_max_lo(pre[q], static_cast<uint32_t>(excl)), neg);
+    const uint32_t fd = pk_min_u(pk_sub(h[q], bd[q]), ps.one);
+    const uint32_t fv = pk_min_u(pk_sub(h[q], bv[q]), ps.one);
+    if (kMulti) {
+      const uint32_t kd = q == 0 ? ((kk[0] << 16) | (k0 & 0xffffu))
+                                 : __builtin_amdgcn_alignbit(kk[q], kk[q - 1], 16);
+      const uint32_t dc = pk_add(kd, 0x20002000u);     // diagonal | kCodeViaK | k
+      const uint32_t vc = pk_add(kk[q], 0x60006000u);  // vertical | kCodeViaK | k
+      const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
+      cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
+    } else {
+      cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+    }
+  }
+  const int j0 = lo + 1 + kBandW * lane;
+  *reinterpret_cast<uint2*>(Cr + (j0 - 1)) = make_uint2(cd[0], cd[1]);
+  if (own < kSlotGlobal) {
+    *reinterpret_cast<uint2*>(ring + own * pitch + 4 * lane) = make_uint2(h[0], h[1]);
+  } else if (own == kSlotGlobal) {
+#pragma unroll
+    for (int u = 0; u < kBandW; ++u) {
+      if (j0 + u <= L) Hg[j0 + u] = pk_half(h[u / 2], u & 1);
+    }
+  }
+  if (lane == 0) {
+    const int16_t c0 = static_cast<int16_t>(lo == 0 ? h0 : kBandNeg);
+    if (own < kSlotGlobal) ring[own * pitch + pitch - 1] = c0;
+    if (own == kSlotGlobal) Hg[0] = c0;
+    if (lo == 0) {
+      Cr[pitch - 1] = static_cast<uint16_t>(kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu))
+                                                   : (kCodeVert | pay0));
+    }
+  }
+  const int e = L - 1 - lo;  // column L in the band: lane e / 4, column e % 4
+  if (e < 0 || e >= kBand) return kNegInf;
+  const int eu = e % kBandW;
+  uint32_t v = eu < 2 ? h[0] : h[1];
+  return __builtin_amdgcn_readlane(pk_half(v, eu & 1), e / kBandW) + L * gap;
+}
+
+// DP rows of one layer in band rows (layers longer than 256, packed form).
+// Same codes, sink scores and end cell as dp_layer_strip inside the bands.
+__device__ __forceinline__ int dp_layer_band(CParams& P, const Slot& s, uint8_t* lds, int V, int L,
+                                             int* best_score, bool* tied) {
+  const int lane = lane_id();
+  const int pitch = static_cast<int>(P.caps.cols);
+  const int32_t gap = uniform(P.gap);
+  const int32_t match = uniform(P.match);
+  const int32_t mismatch = uniform(P.mismatch);
+  int16_t* ring = reinterpret_cast<int16_t*>(lds);
+  const uint8_t* seq = lds + align_up(static_cast<uint64_t>(P.ring_rows) * pitch * 2, 16);
+  PackedStrip<kBandW> ps;
+  {
+    uint32_t sw0 = 0;
+    packed_strip_init<kBandW>(ps, &sw0, 1, match, mismatch, gap);
+  }
+  int32_t best = kNegInf;
+  int best_row = 0;
+  int ties = 0;
+  const int total_preds = uniform(s.pred_off()[V]);
+  for (int r0 = 1; r0 <= V; r0 += kWave) {
+    const int rk = r0 + lane;
+    int off_v = 0, end_v = 0, meta_v = 0;
+    if (rk <= V) {
+      off_v = s.pred_off()[rk - 1];
+      end_v = s.pred_off()[rk];
+      meta_v = s.row_meta()[rk - 1];
+    }
+    const int base = __builtin_amdgcn_readlane(off_v, 0);
+    const int pr0 = base + lane < total_preds ? s.pred_rows()[base + lane] : 0;
+    const int pr1 = base + kWave + lane < total_preds ? s.pred_rows()[base + kWave + lane] : 0;
+    wait_stores();
+    const int chunk = min(kWave, V - r0 + 1);
+    for (int rr = 0; rr < chunk; ++rr) {
+      const int r = r0 + rr;
+      const int off = __builtin_amdgcn_readlane(off_v, rr);
+      const int np = __builtin_amdgcn_readlane(end_v, rr) - off;
+      const int meta = __builtin_amdgcn_readlane(meta_v, rr);
+      auto pred_at = [&](int k) -> int {
+        const int idx = off + k - base;
+        if (idx < kWave) return __builtin_amdgcn_readlane(pr0, idx);
+        if (idx < 2 * kWave) return __builtin_amdgcn_readlane(pr1, idx - kWave);
+        const int e = uniform(s.pred_rows()[off + k]);
+        wait_stores();
+        return e;
+      };
+      const int lo = uniform(band_lo(r, V, L));
+      // the row's layer bytes: columns lo + 1 + 4 lane .. + 3 (lo + 4 lane
+      // + 3 <= L + 2 < pitch: inside the layer's LDS bytes and padding)
+      const uint32_t sw = *reinterpret_cast<const uint32_t*>(seq + lo + kBandW * lane);
+      ps.base[0] = __builtin_amdgcn_perm(sw, sw, 0x0c010c00u);
+      ps.base[1] = __builtin_amdgcn_perm(sw, sw, 0x0c030c02u);
+      const uint32_t b = static_cast<uint32_t>(meta & 0xff);
+      const int own = (meta >> kRowMetaSlotShift) & 31;
+      int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+      uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
+      int32_t h_last;
+      if (np == 1) {
+        const int e = pred_at(0);
+        const uint32_t delta = static_cast<uint32_t>(r - (e & kPredRowMask));
+        const uint32_t pay0 = delta <= kCodePayload ? delta : kCodeViaK;
+        uint32_t pp[2];
+        const uint32_t hp0 = load_band(ring, pitch, s, e, lo, V, L, lane, pp);
+        h_last = band_row<false>(ps, pp, hp0, pp, 0u, b, pay0, lane, lo, L, gap, own, pitch, ring,
+                                 Hg, Cr);
+      } else {
+        uint32_t mx[2], kx[2] = {0u, 0u};
+        uint32_t m0 = load_band(ring, pitch, s, pred_at(0), lo, V, L, lane, mx);
+        uint32_t k0 = 0;
+        for (int k = 1; k < np; ++k) {
+          uint32_t pp[2];
+          const uint32_t hk = load_band(ring, pitch, s, pred_at(k), lo, V, L, lane, pp);
+          const uint32_t ks = static_cast<uint32_t>(k) * 0x10001u;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const uint32_t m = pk_max(mx[q], pp[q]);
+            kx[q] = pk_mad(pk_min_u(pk_sub(m, mx[q]), ps.one), pk_sub(ks, kx[q]), kx[q]);
+            mx[q] = m;
+          }
+          const uint32_t m = pk_max(m0, hk);
+          k0 = pk_mad(pk_min_u(pk_sub(m, m0), ps.one), pk_sub(ks, k0), k0);
+          m0 = m;
+        }
+        h_last = band_row<true>(ps, mx, m0, kx, k0, b, 0u, lane, lo, L, gap, own, pitch, ring,
+                                Hg, Cr);
+      }
+      if ((meta & kRowMetaSink) != 0) {
+        if (lane == 0) s.sink_score()[r] = h_last;
+        if (h_last > best) {
+          best = h_last;
+          best_row = r;
+          ties = 0;
+        } else if (h_last == best) {
+          ++ties;
+        }
+      }
+      if (own == kSlotGlobal) wait_stores();
+      wave_fence();
+    }
+  }
+  *best_score = best;
+  *tied = ties > 0 && best > kNegInf;
+  return best == kNegInf ? 0 : best_row;
+}
+
 // Uniform copy of a pointer (both halves read from the first active lane).
 template <typename T>
 __device__ __forceinline__ T* uniform_ptr(T* p) {
@@ -1226,7 +1480,8 @@ enum DpKind : int {
   kDpPacked12,
   kDpPacked16,
   kDpGeneric16,
-  kDpGeneric32
+  kDpGeneric32,
+  kDpBand4
 };
 
 // The DP of one layer as an out-of-line call: its loops get a register
@@ -1262,6 +1517,8 @@ __device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLa
     row = dp_layer_strip<12, true>(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpPacked16) {
     row = dp_layer_strip<16, true>(P, s, lds, V, L, &best_score, &tied);
+  } else if (kKind == kDpBand4) {
+    row = dp_layer_band(P, s, lds, V, L, &best_score, &tied);
   } else if (kKind == kDpGeneric16) {
     row = dp_layer<int16_t>(P, s, lds, V, L, &best_score, &tied);
   } else {
@@ -2387,7 +2644,11 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     asm volatile("" : "+v"(lds_addr));
     // packed pairs keep H - j * gap in 16 bits
     const bool packed = strip_ok && static_cast<int64_t>(max_abs) * (V + 2 * L + 2) < 32000;
-    if (packed && L <= 4 * kWave) {
+    // racon -b: band rows for layers longer than the band (approximation)
+    const bool banded = packed && P.band != 0 && L > kBand;
+    if (banded) {
+      dp = dp_layer_call<kDpBand4>(pp, s.p, ly, lds_addr, V, L);
+    } else if (packed && L <= 4 * kWave) {
       dp = dp_layer_call<kDpPacked4>(pp, s.p, ly, lds_addr, V, L);
     } else if (packed && L <= 8 * kWave) {
       dp = dp_layer_call<kDpPacked8>(pp, s.p, ly, lds_addr, V, L);
@@ -2410,6 +2671,10 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     best_score = uniform(static_cast<int>(dp >> 32));
     tied = uniform(static_cast<int>(static_cast<uint32_t>(dp) >> 31)) != 0;
     wave_sync();
+    if (banded && best_score == kNegInf) {
+      R.status = kPoaBandMiss;  // no sink reaches column L inside its band
+      return;
+    }
     if (tied) {
       best_row = uniform(sink_tie_by_sets(s, V, full, best_score, static_cast<int>(caps.stack)));
     }
@@ -2439,7 +2704,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       }
       best_row = uniform(found);
     }
-    cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(L);
+    cells += static_cast<uint64_t>(V) * static_cast<uint64_t>(banded ? kBand : L);
     clk.lap(R, kPhaseDp);
 
     // 4. traceback, from the end cell to (0, 0). The alignment of layer
@@ -2468,6 +2733,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       int steps = 0;
       const int step_cap = uniform(V + L + 1);  // every move lowers i or j
       bool bad = false;
+      bool band_miss = false;
       while (i > 0 && !bad) {
         i = uniform(i);
         j = uniform(j);
@@ -2507,6 +2773,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
             if (kind == kCodeDiag) ndj = dj + 1;
           }
           if (ndi >= 16 || ndj >= 8 || i0 - ndi <= 0) return kNextExit;
+          // band rows: a move out of the band is left to the decoder, which
+          // stops the walk there
+          if (banded && !in_band(i0 - ndi, j0 - ndj, V, L)) return kNextExit;
           return static_cast<uint32_t>(ndi * 8 + ndj) | (kind == kCodeDiag ? kDiagBit : 0u);
         };
         const int dja = 2 * (lane & 3);
@@ -2561,13 +2830,14 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
             i -= delta;
             bad = i < 0;
           }
+          if (!bad && banded && !in_band(i, j, V, L)) band_miss = bad = true;
         }
         if (lane < j0 - j) {
           pos_node[j0 - 1 - lane] = out;
         }
       }
       if (bad) {
-        R.status = kPoaBadInput;
+        R.status = band_miss ? kPoaBandMiss : kPoaBadInput;
         return;
       }
       // from the virtual start row only horizontal moves remain

@@ -68,6 +68,7 @@ enum : int32_t {
   kPoaLoopLimit = -4,          // heaviest-bundle completion did not end
   kPoaBadInput = -5,           // layer end outside the graph
   kPoaInDegree = -6,           // a node with more than 8192 in-edges
+  kPoaBandMiss = -7,           // banded DP: the path leaves the band (rerun unbanded)
   kPoaNotRun = -100,
   kPoaRejectedOnHost = -101,   // outside the device limits, never launched
 };
@@ -169,7 +170,7 @@ struct PoaParams {
   uint32_t ring_rows;  // DP rows of 16-bit scores kept in LDS (int32 mode: half)
   uint32_t lds_bytes;  // dynamic LDS per wave: ring + layer bytes
   uint32_t n_priority; // queue positions below this run at raised wave priority
-  uint32_t pad;
+  uint32_t band;       // 0: exact DP; else band rows for long layers (racon -b)
 };
 
 // Dynamic LDS of one wave: the DP row ring (ring_rows x cols x 2 bytes)

@@ -82,7 +82,8 @@ uint64_t window_cost(const PoaWindow& w) {
 }
 
 bool is_retry_status(int32_t s) {
-  return s == kPoaNodeOverflow || s == kPoaRowOverflow || s == kPoaOutOverflow;
+  return s == kPoaNodeOverflow || s == kPoaRowOverflow || s == kPoaOutOverflow ||
+         s == kPoaBandMiss;
 }
 
 int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out) {
@@ -360,6 +361,7 @@ int32_t launch_passes(PoaLane& lane, const PoaTables& t, const PoaConfig& c,
     p.ring_rows = pl.plan.ring;
     p.lds_bytes = pl.plan.lds;
     p.n_priority = priority_windows(t.host, order);
+    p.band = c.band;
     PoaParams* dp = lane.d_params.as<PoaParams>() + k;
     RH_HIP_CHECK(hipMemcpyAsync(dp, &p, sizeof(PoaParams), hipMemcpyHostToDevice, lane.stream),
                  RH_ERR_DEVICE);
@@ -590,6 +592,7 @@ int32_t rh_poa_submit(rh_poa_batch* b, int32_t mode) {
   b->stats.launches = 0;
   b->stats.retried = 0;
   b->stats.rejected = 0;
+  b->stats.band_reruns = 0;
   b->ran = false;
   b->pass_orders.clear();
   b->submitted = true;
@@ -653,6 +656,7 @@ int32_t rh_poa_wait(rh_poa_batch* b) {
   std::vector<uint32_t> retry;
   for (uint32_t w : b->packed.gpu_order) {
     if (rh::is_retry_status(b->results[w].status)) retry.push_back(w);
+    if (b->results[w].status == rh::kPoaBandMiss) b->stats.band_reruns++;
   }
   if (!retry.empty()) {
     b->stats.retried = retry.size();
@@ -680,13 +684,15 @@ int32_t rh_poa_wait(rh_poa_batch* b) {
     std::vector<std::vector<uint32_t>> retry_orders;
     std::vector<PassLaunch> passes;
     std::vector<uint32_t> refused;  // stay in their overflow status: rejected below
-    int32_t rc = rh::plan_bins(b->lane, windows.data(), retry, 0, RH_POA_BATCHED, b->config,
+    rh::PoaConfig exact = b->config;
+    exact.band = 0;  // a window whose path left the band reruns unbanded
+    int32_t rc = rh::plan_bins(b->lane, windows.data(), retry, 0, RH_POA_BATCHED, exact,
                                &retry_orders, &passes, &refused);
     if (rc != RH_OK) return rc;
     if (!passes.empty()) {
       std::vector<const std::vector<uint32_t>*> orders;
       for (const auto& o : retry_orders) orders.push_back(&o);
-      rc = rh::launch_passes(b->lane, b->tables(), b->config, orders, passes, RH_POA_BATCHED,
+      rc = rh::launch_passes(b->lane, b->tables(), exact, orders, passes, RH_POA_BATCHED,
                              &b->stats.launches);
       if (rc != RH_OK) return rc;
       rc = rh::finish_passes(b->lane, &kernel_ms);
@@ -718,6 +724,16 @@ int32_t rh_poa_run(rh_poa_batch* b, int32_t mode) {
     return rc;
   }
   return rh_poa_wait(b);
+}
+
+int32_t rh_poa_set_band(rh_poa_batch* b, uint32_t band) {
+  rh::clear_error();
+  if (b == nullptr || (band != 0 && band != 256)) {
+    rh::set_error("rh_poa_set_band: null batch or a band other than 0 and 256");
+    return RH_ERR_ARGUMENT;
+  }
+  b->config.band = band;
+  return RH_OK;
 }
 
 int32_t rh_poa_autotune(rh_poa_batch* b, rh_poa_tuning* out) {

@@ -384,6 +384,7 @@ int32_t retry(rh_server* s, Executor* ex, const std::vector<uint32_t>& ws,
   std::vector<uint32_t> refused;
   rh::PoaConfig c = ex->config;
   c.tune_slots = 0;  // exact slots are large: as many as fit
+  c.band = 0;
   int32_t rc = rh::plan_bins(lane, s->host_windows.data(), again, 0, RH_POA_BATCHED, c,
                              &orders, &passes, &refused);
   if (rc != RH_OK) return rc;

@@ -173,9 +173,12 @@ class HipEngine : public PolishEngine {
     Slot slots[2];
     const uint32_t n_batches = chunks.size() > 1 ? 2 : 1;
     for (uint32_t k = 0; k < n_batches; ++k) {
-      const int32_t rc = rh_poa_create(0, config_.match, config_.mismatch, config_.gap,
-                                       trim ? 1 : 0, &slots[k].batch);
+      int32_t rc = rh_poa_create(0, config_.match, config_.mismatch, config_.gap, trim ? 1 : 0,
+                                 &slots[k].batch);
       if (rc != RH_OK) die("rh_poa_create", rc);
+      // racon -b: the banded approximation (GenomeWorks' band of 256)
+      rc = rh_poa_set_band(slots[k].batch, config_.banded ? 256 : 0);
+      if (rc != RH_OK) die("rh_poa_set_band", rc);
     }
     auto upload = [&](uint64_t c) {
       Slot& sl = slots[c % n_batches];

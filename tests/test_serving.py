@@ -110,7 +110,7 @@ def test_gpu_budget_admits_in_arrival_order():
 
 @pytest.mark.gpu
 def test_gpu_request_over_budget_is_refused_and_empty_requests_complete():
-    big = make_windows(620, 40, length=500, depth=30)
+    big = make_windows(620, 90, length=500, depth=30)  # ~2 MB
     with PolishServer(1 << 20, executors=1) as srv:
         with pytest.raises(RuntimeError, match="exceeds the server's budget"):
             srv.submit(big)
