@@ -131,8 +131,9 @@ int32_t rh_poa_upload(rh_poa_batch* batch, uint32_t n, const rh_window* windows)
  * counts them). The limits: a graph bound (backbone + all layer bytes) of
  * 2^24 - 1 nodes; a layer whose bytes alone exceed the LDS of one workgroup
  * (160 KiB on gfx950); a graph node with more than 8,192 in-edges; a
- * column of more than 9 distinct bytes (aligned sets hold 8 entries per
- * node). */
+ * column of more than 17 distinct bytes (a window whose column exceeds the
+ * default 9 - aligned sets of 8 entries per node - reruns in a kernel with
+ * 16, which holds every IUPAC code). */
 int32_t rh_poa_run(rh_poa_batch* batch, int32_t mode);
 
 /* rh_poa_run in two halves, for callers that keep several batches in flight

@@ -15,7 +15,9 @@ from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libracon_hip.so")
+# RACON_HIP_LIBRARY names another build of the same library (e.g. one
+# compiled with other kernel occupancy settings, for comparisons)
+LIB_PATH = os.environ.get("RACON_HIP_LIBRARY") or os.path.join(PKG_DIR, "lib", "libracon_hip.so")
 RACON_BIN = os.path.join(PKG_DIR, "bin", "racon")
 
 RH_OK = 0

@@ -86,6 +86,7 @@ struct PassPlan {
   PoaSlotCaps caps;
   uint32_t ring;  // LDS ring rows (0: every DP row in the slot's global matrix)
   uint32_t lds;   // dynamic LDS bytes per wave
+  bool wide;      // the wide-aligned-set kernel (kMaxAlignedWide)
 };
 
 // A pass ready to launch: its plan, resident slots and slot size.
@@ -98,10 +99,11 @@ struct PassLaunch {
 
 // The plan for windows `ws`: graph arrays for the largest, the row pitch of
 // the longest layer, DP rows capped at row_cap (0: exact, node bound + 1),
-// and the deepest ring that fits lds_limit. False when even a ring of 0 rows
-// does not fit (the layer bytes alone exceed the limit).
+// and the deepest ring that fits lds_limit, for the default or the wide
+// kernel. False when even a ring of 0 rows does not fit (the layer bytes
+// alone exceed the limit).
 bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t row_cap,
-               uint32_t lds_limit, uint32_t tune_ring, PassPlan* plan);
+               uint32_t lds_limit, uint32_t tune_ring, bool wide, PassPlan* plan);
 
 // Resident slots of a pass: the kernel's occupancy times the CUs divided by
 // the lane's share (all windows for RH_POA_STATIC, one for RH_POA_PLAIN),
@@ -120,11 +122,20 @@ int32_t launch_passes(PoaLane& lane, const PoaTables& t, const PoaConfig& c,
 int32_t finish_passes(PoaLane& lane, double* kernel_ms);
 
 // Splits windows into the main and the long-layer bin, plans and sizes a pass
-// per non-empty bin (row_cap 0: exact rows). Windows of a bin whose plan
-// does not fit the LDS limit are appended to `refused`.
+// per non-empty bin (row_cap 0: exact rows; wide: the wide kernel). Windows
+// of a bin whose plan does not fit the LDS limit are appended to `refused`.
 int32_t plan_bins(PoaLane& lane, const PoaWindow* host, const std::vector<uint32_t>& ws,
                   uint32_t row_cap, int32_t mode, const PoaConfig& c,
                   std::vector<std::vector<uint32_t>>* orders, std::vector<PassLaunch>* passes,
-                  std::vector<uint32_t>* refused);
+                  std::vector<uint32_t>* refused, bool wide = false);
+
+// Plans the rerun of `ws` (windows that ended in the retry statuses
+// `status`) in exact slots: windows whose aligned sets were full in the wide
+// kernel, the others in the default one. `c` is the rerun's configuration
+// (the exact DP: band 0). Appends the orders and passes.
+int32_t plan_retries(PoaLane& lane, const PoaWindow* host, const std::vector<uint32_t>& ws,
+                     const std::vector<int32_t>& status, const PoaConfig& c,
+                     std::vector<std::vector<uint32_t>>* orders,
+                     std::vector<PassLaunch>* passes, std::vector<uint32_t>* refused);
 
 }  // namespace rh

@@ -58,6 +58,20 @@ typedef const __attribute__((address_space(4))) PoaLayer CLayer;
 #define RH_POA_WAVES_PER_EU 4
 #endif
 
+// Aligned-set entries kept per node. The default build keeps kMaxAligned;
+// poa_kernel_wide.hip compiles this file again with kMaxAlignedWide (and
+// "_wide" kernel and launcher names) for the windows whose columns hold more
+// distinct bytes (IUPAC codes), which the host reruns there.
+#ifndef RH_POA_ALIGNED
+#define RH_POA_ALIGNED kMaxAligned
+#endif
+#ifdef RH_POA_WIDE
+#define RH_POA_SYM(name) name##_wide
+#else
+#define RH_POA_SYM(name) name
+#endif
+constexpr uint32_t kAligned = RH_POA_ALIGNED;
+
 constexpr int kStrip = 4;                  // DP columns per lane per tile
 constexpr int kTile = kWave * kStrip;      // 256 columns per tile
 constexpr int32_t kNegInf = -(1 << 29);
@@ -259,7 +273,7 @@ __device__ __forceinline__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* 
         if (s.lead()[v] != 0) {
           const int na = s.n_aligned()[v];
           for (int k = 0; k < na; ++k) {
-            const int a = uniform(s.aligned()[v * kMaxAligned + k]);
+            const int a = uniform(s.aligned()[v * kAligned + k]);
             if (keep != nullptr && keep[a] == 0) {
               continue;
             }
@@ -284,7 +298,7 @@ __device__ __forceinline__ int topo_sort(Slot& s, const uint8_t* keep, int32_t* 
             out[count++] = v;
             const int na = s.n_aligned()[v];
             for (int k = 0; k < na; ++k) {
-              const int a = uniform(s.aligned()[v * kMaxAligned + k]);
+              const int a = uniform(s.aligned()[v * kAligned + k]);
               if (keep == nullptr || keep[a] != 0) {
                 out[count++] = a;
               }
@@ -1536,6 +1550,7 @@ __device__ __noinline__ uint64_t dp_layer_call(CParams* params, GByte* base, CLa
 __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool* order_ok_out,
                                               int* n_path_out) {
   bool order_ok = true;
+  bool aligned_full = false;
   int n_path = 0;
   const int32_t* pos_node = s.path();
   const int before = s.n_nodes;
@@ -1554,7 +1569,7 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
       int same = -1;
       const int cnt = s.n_aligned()[node];
       for (int k = 0; k < cnt; ++k) {
-        const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+        const int a = uniform(s.aligned()[node * kAligned + k]);
         if (s.base()[a] == letter) {
           same = a;
           break;
@@ -1562,20 +1577,21 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
       }
       if (same == -1) {
         nv = add_node(s, caps, letter);
-        if (s.overflow || cnt + 1 > static_cast<int>(kMaxAligned)) {
+        if (!s.overflow && cnt + 1 > static_cast<int>(kAligned)) {
+          aligned_full = true;
           s.overflow = true;
-          break;
         }
+        if (s.overflow) break;
         for (int k = 0; k < cnt; ++k) {
-          const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+          const int a = uniform(s.aligned()[node * kAligned + k]);
           const int ca = s.n_aligned()[a];
-          s.aligned()[nv * kMaxAligned + k] = a;
-          s.aligned()[a * kMaxAligned + ca] = nv;
+          s.aligned()[nv * kAligned + k] = a;
+          s.aligned()[a * kAligned + ca] = nv;
           s.n_aligned()[a] = static_cast<uint8_t>(ca + 1);
         }
-        s.aligned()[nv * kMaxAligned + cnt] = node;
+        s.aligned()[nv * kAligned + cnt] = node;
         s.n_aligned()[nv] = static_cast<uint8_t>(cnt + 1);
-        s.aligned()[node * kMaxAligned + cnt] = nv;
+        s.aligned()[node * kAligned + cnt] = nv;
         s.n_aligned()[node] = static_cast<uint8_t>(cnt + 1);
       } else {
         nv = same;
@@ -1589,7 +1605,7 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
         anchor = s.node_rank()[node];
         const int cnt = s.n_aligned()[node];
         for (int k = 0; k < cnt; ++k) {
-          const int a = uniform(s.aligned()[node * kMaxAligned + k]);
+          const int a = uniform(s.aligned()[node * kAligned + k]);
           if (a < before) anchor = max(anchor, s.node_rank()[a]);
         }
       } else if (head == -1) {
@@ -1600,7 +1616,7 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
         anchor = s.node_rank()[head];
         const int cnt = s.n_aligned()[head];
         for (int k = 0; k < cnt; ++k) {
-          const int a = uniform(s.aligned()[head * kMaxAligned + k]);
+          const int a = uniform(s.aligned()[head * kAligned + k]);
           if (a < before) anchor = max(anchor, s.node_rank()[a]);
         }
       }
@@ -1620,7 +1636,7 @@ __device__ __noinline__ Slot insert_sequential(Slot s, CCaps& caps, int L, bool*
   }
   s.n_seqs = s.n_seqs + 1;
   *order_ok_out = order_ok;
-  *n_path_out = n_path;
+  *n_path_out = aligned_full ? -1 : n_path;  // -1: an aligned set was full
   return s;
 }
 
@@ -1651,7 +1667,7 @@ __device__ __noinline__ InsertResult insert_parallel(GByte* slot_base, CLayout* 
                                                      int cap_edges) {
   constexpr int kU = 4;   // chunks per round in the edge and label passes
   constexpr int kUA = 2;  // in the passes that hold aligned sets
-  constexpr int kA = static_cast<int>(kMaxAligned);
+  constexpr int kA = static_cast<int>(kAligned);
   const Slot s = carve(uniform_ptr(slot_base), *uniform_ptr(layout));
   const uint8_t* lseq = (const uint8_t*)(LByte*)(size_t)uniform(static_cast<int>(lseq_addr));
   L = uniform(L);
@@ -1720,7 +1736,11 @@ __device__ __noinline__ InsertResult insert_parallel(GByte* slot_base, CLayout* 
     res.status = 1;
     return res;
   }
-  if (__ballot(full) != 0 || n_nodes + n_new > cap_nodes) {
+  if (__ballot(full) != 0) {
+    res.status = kPoaAlignedFull;  // a column with more distinct bytes than kAligned + 1
+    return res;
+  }
+  if (n_nodes + n_new > cap_nodes) {
     res.status = kPoaNodeOverflow;
     return res;
   }
@@ -2214,17 +2234,17 @@ __device__ __noinline__ int include_walk(GByte* slot_base, CLayout* layout, uint
     if (claims > nn) return kPoaNodeOverflow;
     int e = -1;
     int na = 0;
-    int al[kMaxAligned];
+    int al[kAligned];
     if (mine) {
       e = in_head[v];
       na = n_aligned_of[v];
 #pragma unroll
-      for (int k = 0; k < static_cast<int>(kMaxAligned); ++k) al[k] = aligned[v * kMaxAligned + k];
+      for (int k = 0; k < static_cast<int>(kAligned); ++k) al[k] = aligned[v * kAligned + k];
     }
     wave_fence();  // the claims before the visit-bit reads below
     const int max_na = uniform(__builtin_amdgcn_readlane(wave_prefix_max(na), kWave - 1));
 #pragma unroll
-    for (int k = 0; k < static_cast<int>(kMaxAligned); ++k) {
+    for (int k = 0; k < static_cast<int>(kAligned); ++k) {
       if (k >= max_na) break;
       const bool c = k < na && unseen(al[k]);
       if (!push(c, al[k])) return kPoaNodeOverflow;
@@ -2311,7 +2331,7 @@ __device__ __forceinline__ int sink_tie_by_sets(const Slot& s, int V, bool full,
         }
         const int na = n_aligned[x];
         for (int k = 0; k < na; ++k) {
-          const int a = uniform(aligned[x * kMaxAligned + k]);
+          const int a = uniform(aligned[x * kAligned + k]);
           if (kept(a) && uniform(mark[a]) != tag) {
             if (top >= kReachCap) return -1;
             mark[a] = tag;
@@ -2324,11 +2344,11 @@ __device__ __forceinline__ int sink_tie_by_sets(const Slot& s, int V, bool full,
       if (r != v) {
         bool member = false;
         const int nv = n_aligned[v];
-        for (int k = 0; k < nv; ++k) member |= aligned[v * kMaxAligned + k] == r;
+        for (int k = 0; k < nv; ++k) member |= aligned[v * kAligned + k] == r;
         if (!member) return -1;
         const int nr = n_aligned[r];
         for (int k = 0; k < nr; ++k) {
-          const int a = uniform(aligned[r * kMaxAligned + k]);
+          const int a = uniform(aligned[r * kAligned + k]);
           if (!kept(a)) continue;
           ++pos;
           if (a == v) break;
@@ -2561,7 +2581,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
               s.stack()[top++] = s.e_tail()[e];
             }
             for (int k = 0; k < na; ++k) {
-              s.stack()[top++] = s.aligned()[v * kMaxAligned + k];
+              s.stack()[top++] = s.aligned()[v * kAligned + k];
             }
             s.in_sub()[v] = 1;
           }
@@ -2875,7 +2895,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       }
     }
     if (s.overflow) {
-      R.status = kPoaNodeOverflow;
+      R.status = n_path < 0 ? kPoaAlignedFull : kPoaNodeOverflow;
       return;
     }
     wave_sync();
@@ -3116,7 +3136,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       int c = s.cov()[v];
       const int na2 = s.n_aligned()[v];
       for (int k = 0; k < na2; ++k) {
-        c += s.cov()[s.aligned()[v * kMaxAligned + k]];
+        c += s.cov()[s.aligned()[v * kAligned + k]];
       }
       return c;
     };
@@ -3155,7 +3175,7 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
 // are read through a restrict pointer: uniform scalar loads, and no private
 // copy of the struct (a by-value kernel argument whose address is taken is
 // copied to scratch, with its fields held in vector registers).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_batch_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void RH_POA_SYM(poa_batch_kernel)(
     const PoaParams* __restrict__ params) {
   CParams& P = *(CParams*)params;
   extern __shared__ uint8_t lds[];
@@ -3199,7 +3219,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES
 }
 
 // Plain kernel: one launch per window (single wave, window index given).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_single_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void RH_POA_SYM(poa_single_kernel)(
     const PoaParams* __restrict__ params, uint32_t q) {
   CParams& P = *(CParams*)params;
   extern __shared__ uint8_t lds[];
@@ -3222,7 +3242,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES
 // Block b takes window order[first + b] and scratch slot b; the hardware
 // dispatcher starts blocks in index order as CUs free up. Kept as the
 // comparison for the persistent kernel's queue.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void poa_static_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES_PER_EU))) void RH_POA_SYM(poa_static_kernel)(
     const PoaParams* __restrict__ params, uint32_t first) {
   CParams& P = *(CParams*)params;
   extern __shared__ uint8_t lds[];
@@ -3246,26 +3266,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RH_POA_WAVES
 namespace rh {
 
 // Host launchers (kept in this translation unit with the kernels).
-hipError_t launch_poa_batch(const PoaParams* d_params, uint32_t lds_bytes, uint32_t grid,
-                            hipStream_t stream) {
-  hipLaunchKernelGGL(poa_batch_kernel, dim3(grid), dim3(64), lds_bytes, stream, d_params);
+hipError_t RH_POA_SYM(launch_poa_batch)(const PoaParams* d_params, uint32_t lds_bytes,
+                                        uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL(RH_POA_SYM(poa_batch_kernel), dim3(grid), dim3(64), lds_bytes, stream,
+                     d_params);
   return hipGetLastError();
 }
 
-hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
-                             hipStream_t stream) {
-  hipLaunchKernelGGL(poa_single_kernel, dim3(1), dim3(64), lds_bytes, stream, d_params, q);
+hipError_t RH_POA_SYM(launch_poa_single)(const PoaParams* d_params, uint32_t lds_bytes,
+                                         uint32_t q, hipStream_t stream) {
+  hipLaunchKernelGGL(RH_POA_SYM(poa_single_kernel), dim3(1), dim3(64), lds_bytes, stream,
+                     d_params, q);
   return hipGetLastError();
 }
 
-hipError_t launch_poa_static(const PoaParams* d_params, uint32_t lds_bytes, uint32_t first,
-                             uint32_t count, hipStream_t stream) {
-  hipLaunchKernelGGL(poa_static_kernel, dim3(count), dim3(64), lds_bytes, stream, d_params, first);
+hipError_t RH_POA_SYM(launch_poa_static)(const PoaParams* d_params, uint32_t lds_bytes,
+                                         uint32_t first, uint32_t count, hipStream_t stream) {
+  hipLaunchKernelGGL(RH_POA_SYM(poa_static_kernel), dim3(count), dim3(64), lds_bytes, stream,
+                     d_params, first);
   return hipGetLastError();
 }
 
-hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, poa_batch_kernel, 64,
+hipError_t RH_POA_SYM(poa_occupancy)(int* blocks_per_cu, uint32_t lds_bytes) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                      RH_POA_SYM(poa_batch_kernel), 64,
                                                       lds_bytes);
 }
 

@@ -69,6 +69,7 @@ enum : int32_t {
   kPoaBadInput = -5,           // layer end outside the graph
   kPoaInDegree = -6,           // a node with more than 8192 in-edges
   kPoaBandMiss = -7,           // banded DP: the path leaves the band (rerun unbanded)
+  kPoaAlignedFull = -8,        // an aligned set is full: rerun in the wide kernel
   kPoaNotRun = -100,
   kPoaRejectedOnHost = -101,   // outside the device limits, never launched
 };
@@ -80,10 +81,15 @@ struct PoaSlotCaps {
   uint32_t rows;    // DP rows (subgraph nodes + 1)
   uint32_t cols;    // DP row pitch in elements (>= max layer length + 1, padded)
   uint32_t stack;   // DFS stack entries
-  uint32_t pad;
+  uint32_t aligned; // aligned-set entries per node (the kernel variant's)
 };
 
-constexpr uint32_t kMaxAligned = 8;  // aligned-set entries kept per node
+// Aligned-set entries kept per node: a column of the graph holds at most
+// this many + 1 distinct bytes. The default kernel keeps kMaxAligned; the
+// wide variant (poa_kernel_wide.hip) reruns windows that need more (A, C,
+// G, T and the 11 IUPAC ambiguity codes, N included, are 15 bytes).
+constexpr uint32_t kMaxAligned = 8;
+constexpr uint32_t kMaxAlignedWide = 16;
 
 // DP row pitch >= layer length + kPoaColPad: strips that end past column L
 // (up to 12 columns, stored from column + 3) stay inside their row.
@@ -115,7 +121,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.in_tail = take(4 * n);
   l.out_tail = take(4 * n);
   l.n_aligned = take(n);
-  l.aligned = take(4 * n * kMaxAligned);
+  l.aligned = take(4 * n * (c.aligned != 0 ? c.aligned : kMaxAligned));
   l.cov = take(4 * n);
   l.last_label = take(4 * n);
   l.e_tail = take(4 * e);

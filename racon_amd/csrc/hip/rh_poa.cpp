@@ -26,6 +26,14 @@ hipError_t launch_poa_single(const PoaParams* d_params, uint32_t lds_bytes, uint
 hipError_t launch_poa_static(const PoaParams* d_params, uint32_t lds_bytes, uint32_t first,
                              uint32_t count, hipStream_t stream);
 hipError_t poa_occupancy(int* blocks_per_cu, uint32_t lds_bytes);
+// the same kernels with wide aligned sets (poa_kernel_wide.hip)
+hipError_t launch_poa_batch_wide(const PoaParams* d_params, uint32_t lds_bytes, uint32_t grid,
+                                 hipStream_t stream);
+hipError_t launch_poa_single_wide(const PoaParams* d_params, uint32_t lds_bytes, uint32_t q,
+                                  hipStream_t stream);
+hipError_t launch_poa_static_wide(const PoaParams* d_params, uint32_t lds_bytes, uint32_t first,
+                                  uint32_t count, hipStream_t stream);
+hipError_t poa_occupancy_wide(int* blocks_per_cu, uint32_t lds_bytes);
 
 namespace {
 thread_local std::string g_error;
@@ -83,7 +91,7 @@ uint64_t window_cost(const PoaWindow& w) {
 
 bool is_retry_status(int32_t s) {
   return s == kPoaNodeOverflow || s == kPoaRowOverflow || s == kPoaOutOverflow ||
-         s == kPoaBandMiss;
+         s == kPoaBandMiss || s == kPoaAlignedFull;
 }
 
 int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out) {
@@ -230,7 +238,7 @@ void PoaLane::release() {
 }
 
 bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t row_cap,
-               uint32_t lds_limit, uint32_t tune_ring, PassPlan* plan) {
+               uint32_t lds_limit, uint32_t tune_ring, bool wide, PassPlan* plan) {
   PoaSlotCaps caps{};
   uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
   for (uint32_t w : ws) {
@@ -242,7 +250,8 @@ bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t 
   caps.edges = max_edges;
   caps.cols = static_cast<uint32_t>(align_up(max_cols - 1 + kPoaColPad, 16));
   caps.rows = row_cap == 0 ? max_nodes + 1 : std::min<uint32_t>(max_nodes + 1, row_cap);
-  caps.stack = 2 * (max_edges + (kMaxAligned + 1) * max_nodes) + 64;
+  caps.aligned = wide ? kMaxAlignedWide : kMaxAligned;
+  caps.stack = 2 * (max_edges + (caps.aligned + 1) * max_nodes) + 64;
   uint32_t ring = ring_rows_for(caps.cols, tune_ring);
   while (ring > 0 && poa_lds_bytes(caps.cols, ring) > lds_limit) --ring;
   if (ring == 1) ring = 0;  // one 16-bit row is no slot in 32-bit mode; keep none
@@ -252,13 +261,16 @@ bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t 
   plan->caps = caps;
   plan->ring = ring;
   plan->lds = poa_lds_bytes(caps.cols, ring);
+  plan->wide = wide;
   return true;
 }
 
 int32_t size_pass(PoaLane& lane, uint64_t n_windows, const PassPlan& plan, int32_t mode,
                   const PoaConfig& c, PassLaunch* out) {
   int per_cu = 0, cus = 0;
-  RH_HIP_CHECK(poa_occupancy(&per_cu, plan.lds), RH_ERR_DEVICE);
+  RH_HIP_CHECK(plan.wide ? poa_occupancy_wide(&per_cu, plan.lds)
+                         : poa_occupancy(&per_cu, plan.lds),
+               RH_ERR_DEVICE);
   RH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lane.device),
                RH_ERR_DEVICE);
   uint64_t max_slots = static_cast<uint64_t>(std::max(1, per_cu)) * std::max(1, cus);
@@ -288,7 +300,7 @@ int32_t size_pass(PoaLane& lane, uint64_t n_windows, const PassPlan& plan, int32
 int32_t plan_bins(PoaLane& lane, const PoaWindow* host, const std::vector<uint32_t>& ws,
                   uint32_t row_cap, int32_t mode, const PoaConfig& c,
                   std::vector<std::vector<uint32_t>>* orders, std::vector<PassLaunch>* passes,
-                  std::vector<uint32_t>* refused) {
+                  std::vector<uint32_t>* refused, bool wide) {
   std::vector<uint32_t> bins[2];
   for (uint32_t w : ws) {
     bins[layer_bin(host[w])].push_back(w);
@@ -299,7 +311,7 @@ int32_t plan_bins(PoaLane& lane, const PoaWindow* host, const std::vector<uint32
     if (mode == RH_POA_STATIC) std::sort(bin.begin(), bin.end());
     PassPlan plan{};
     const uint32_t ring = c.bin_ring[k] != 0 ? c.bin_ring[k] : c.tune_ring;
-    if (!plan_pass(host, bin, row_cap, lane.lds_limit, ring, &plan)) {
+    if (!plan_pass(host, bin, row_cap, lane.lds_limit, ring, wide, &plan)) {
       refused->insert(refused->end(), bin.begin(), bin.end());
       continue;
     }
@@ -367,7 +379,9 @@ int32_t launch_passes(PoaLane& lane, const PoaTables& t, const PoaConfig& c,
                  RH_ERR_DEVICE);
     if (mode == RH_POA_PLAIN) {
       for (uint32_t q = 0; q < order.size(); ++q) {
-        RH_HIP_CHECK(launch_poa_single(dp, p.lds_bytes, q, lane.stream), RH_ERR_KERNEL);
+        RH_HIP_CHECK(pl.plan.wide ? launch_poa_single_wide(dp, p.lds_bytes, q, lane.stream)
+                                  : launch_poa_single(dp, p.lds_bytes, q, lane.stream),
+                     RH_ERR_KERNEL);
         ++*launches;
       }
     } else if (mode == RH_POA_STATIC) {
@@ -375,20 +389,35 @@ int32_t launch_passes(PoaLane& lane, const PoaTables& t, const PoaConfig& c,
       // the slots of all windows do not fit in memory at once
       for (uint64_t first = 0; first < order.size(); first += pl.slots) {
         const uint64_t count = std::min<uint64_t>(pl.slots, order.size() - first);
-        RH_HIP_CHECK(launch_poa_static(dp, p.lds_bytes, static_cast<uint32_t>(first),
-                                       static_cast<uint32_t>(count), lane.stream),
+        const uint32_t f = static_cast<uint32_t>(first), n = static_cast<uint32_t>(count);
+        RH_HIP_CHECK(pl.plan.wide ? launch_poa_static_wide(dp, p.lds_bytes, f, n, lane.stream)
+                                  : launch_poa_static(dp, p.lds_bytes, f, n, lane.stream),
                      RH_ERR_KERNEL);
         ++*launches;
       }
     } else {
-      RH_HIP_CHECK(launch_poa_batch(dp, p.lds_bytes, static_cast<uint32_t>(pl.slots),
-                                    lane.stream),
+      const uint32_t grid = static_cast<uint32_t>(pl.slots);
+      RH_HIP_CHECK(pl.plan.wide ? launch_poa_batch_wide(dp, p.lds_bytes, grid, lane.stream)
+                                : launch_poa_batch(dp, p.lds_bytes, grid, lane.stream),
                    RH_ERR_KERNEL);
       ++*launches;
     }
   }
   RH_HIP_CHECK(hipEventRecord(lane.ev1, lane.stream), RH_ERR_DEVICE);
   return RH_OK;
+}
+
+int32_t plan_retries(PoaLane& lane, const PoaWindow* host, const std::vector<uint32_t>& ws,
+                     const std::vector<int32_t>& status, const PoaConfig& c,
+                     std::vector<std::vector<uint32_t>>* orders,
+                     std::vector<PassLaunch>* passes, std::vector<uint32_t>* refused) {
+  std::vector<uint32_t> narrow, wide;
+  for (size_t k = 0; k < ws.size(); ++k) {
+    (status[k] == kPoaAlignedFull ? wide : narrow).push_back(ws[k]);
+  }
+  int32_t rc = plan_bins(lane, host, narrow, 0, RH_POA_BATCHED, c, orders, passes, refused);
+  if (rc != RH_OK) return rc;
+  return plan_bins(lane, host, wide, 0, RH_POA_BATCHED, c, orders, passes, refused, true);
 }
 
 int32_t finish_passes(PoaLane& lane, double* kernel_ms) {
@@ -686,8 +715,10 @@ int32_t rh_poa_wait(rh_poa_batch* b) {
     std::vector<uint32_t> refused;  // stay in their overflow status: rejected below
     rh::PoaConfig exact = b->config;
     exact.band = 0;  // a window whose path left the band reruns unbanded
-    int32_t rc = rh::plan_bins(b->lane, windows.data(), retry, 0, RH_POA_BATCHED, exact,
-                               &retry_orders, &passes, &refused);
+    std::vector<int32_t> why;
+    for (uint32_t w : retry) why.push_back(b->results[w].status);
+    int32_t rc = rh::plan_retries(b->lane, windows.data(), retry, why, exact, &retry_orders,
+                                  &passes, &refused);
     if (rc != RH_OK) return rc;
     if (!passes.empty()) {
       std::vector<const std::vector<uint32_t>*> orders;

@@ -356,9 +356,11 @@ int32_t read_results(rh_server* s, rh::PoaLane& lane, const std::vector<Part>& p
 // Reruns the windows whose graph outgrew the default slot, in slots sized
 // from their exact bounds and with consensus space of their node bound.
 int32_t retry(rh_server* s, Executor* ex, const std::vector<uint32_t>& ws,
-              const std::vector<Request*>& owner, double* kernel_ms, uint64_t* retried) {
+              const std::vector<Request*>& owner, const std::vector<int32_t>& why,
+              double* kernel_ms, uint64_t* retried) {
   rh::PoaLane& lane = ex->lane;
   std::vector<uint32_t> again;
+  std::vector<int32_t> again_why;
   {
     std::lock_guard<std::mutex> lock(s->mu);
     for (size_t k = 0; k < ws.size(); ++k) {
@@ -370,6 +372,7 @@ int32_t retry(rh_server* s, Executor* ex, const std::vector<uint32_t>& ws,
       w.out_off = at;
       w.out_cap = w.node_bound;
       again.push_back(ws[k]);
+      again_why.push_back(why[k]);
     }
   }
   *retried += again.size();
@@ -385,8 +388,8 @@ int32_t retry(rh_server* s, Executor* ex, const std::vector<uint32_t>& ws,
   rh::PoaConfig c = ex->config;
   c.tune_slots = 0;  // exact slots are large: as many as fit
   c.band = 0;
-  int32_t rc = rh::plan_bins(lane, s->host_windows.data(), again, 0, RH_POA_BATCHED, c,
-                             &orders, &passes, &refused);
+  int32_t rc = rh::plan_retries(lane, s->host_windows.data(), again, again_why, c, &orders,
+                                &passes, &refused);
   if (rc != RH_OK) return rc;
   if (!passes.empty()) {
     std::vector<const std::vector<uint32_t>*> ptrs;
@@ -463,17 +466,19 @@ void executor_loop(rh_server* s, Executor* ex) {
     if (rc == RH_OK) {
       std::vector<uint32_t> again;
       std::vector<Request*> owner;
+      std::vector<int32_t> why;
       size_t k = 0;
       for (const Part& p : parts) {
         for (size_t i = 0; i < p.count; ++i, ++k) {
           if (rh::is_retry_status(res[k].status)) {
             again.push_back(ws[k]);
+            why.push_back(res[k].status);
             owner.push_back(p.r);
           }
         }
       }
       if (!again.empty()) {
-        rc = retry(s, ex, again, owner, &kernel_ms, &retried);
+        rc = retry(s, ex, again, owner, why, &kernel_ms, &retried);
         if (rc == RH_OK) rc = read_results(s, ex->lane, parts, ws, &res);
       }
     }

@@ -3,9 +3,12 @@
 - layers longer than the main pass allows (-w 14000 and -w 20000 windows):
   they run in a pass of their own with wide DP rows and a shallow LDS ring;
 - more than 8,190 layers in one window (deep amplicon-like coverage);
-- a column with more distinct bytes than an aligned set holds: the window
-  is returned as RH_WINDOW_REJECTED (backbone, for the caller's CPU path)
-  while every other window of the batch is polished as usual.
+- a column with IUPAC codes (15 distinct bytes): the default kernel's
+  aligned sets (9 bytes per column) overflow and the window reruns in the
+  wide kernel (17 bytes per column);
+- a column with more distinct bytes than even the wide kernel holds: the
+  window is returned as RH_WINDOW_REJECTED (backbone, for the caller's CPU
+  path) while every other window of the batch is polished as usual.
 """
 import sys
 
@@ -74,22 +77,39 @@ def test_more_than_8190_layers():
     check_against_oracle(wins, out, status)
 
 
-def test_wide_column_window_is_rejected_not_fatal():
+def column_window(rng, length, codes):
+    """A window whose layers are the backbone with one different byte each
+    at column 60: the column's aligned set needs one member per byte."""
+    backbone = random_seq(rng, length)
+    layers = []
+    for c in codes:
+        seq = bytearray(backbone)
+        seq[60] = c if c != backbone[60] else ord("X")
+        layers.append(Layer(bytes(seq), None, 0, length - 1))
+    return Window(backbone, b"!" * length, layers, True)
+
+
+def test_iupac_column_window_reruns_in_the_wide_kernel():
+    rng = np.random.default_rng(67)
+    # A, C, G, T with N and the ten other IUPAC codes: 15 bytes in one column
+    wide = column_window(rng, 120, b"ACGTNRYKMSWBDHV")
+    wins = make_windows(68, 4, length=300, depth=10)
+    wins.insert(2, wide)
+    out, status, st = run(wins)
+    print(f"retried {st.retried}, rejected {st.rejected}", flush=True)
+    assert st.rejected == 0 and st.retried >= 1
+    check_against_oracle(wins, out, status)
+
+
+def test_wider_column_window_is_rejected_not_fatal():
     rng = np.random.default_rng(65)
     length = 120
-    backbone = random_seq(rng, length)
-    # eleven layers, each the backbone with a different byte at column 60:
-    # the column's aligned set would need 11 members
-    layers = []
-    for c in b"CGTNRYKMSWB":
-        seq = bytearray(backbone)
-        seq[60] = c if c != backbone[60] else ord("D")
-        layers.append(Layer(bytes(seq), None, 0, length - 1))
-    wide = Window(backbone, b"!" * length, layers, True)
+    # twenty distinct bytes in one column: more than the wide kernel holds
+    wide = column_window(rng, length, b"ACGTNRYKMSWBDHVEFIJL")
     wins = make_windows(66, 4, length=300, depth=10)
     wins.insert(2, wide)
     out, status, st = run(wins)
     assert status[2] == _lib.RH_WINDOW_REJECTED
-    assert out[2] == backbone
+    assert out[2] == wide.backbone
     assert st.rejected == 1
     check_against_oracle(wins, out, status, skip={2})
