@@ -1,5 +1,9 @@
 #include "io.hpp"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <zlib.h>
 
 #include <cstdio>
@@ -48,27 +52,9 @@ bool overlap_format_of(const std::string& path, OverlapFormat* format) {
   return true;
 }
 
-std::string read_file(const std::string& path) {
-  // plain files: one read into a buffer of the file's size
-  if (FILE* p = fopen(path.c_str(), "rb")) {
-    unsigned char magic[2] = {0, 0};
-    const size_t got = fread(magic, 1, 2, p);
-    const bool gz = got == 2 && magic[0] == 0x1f && magic[1] == 0x8b;
-    if (!gz && fseek(p, 0, SEEK_END) == 0) {
-      const long size = ftell(p);
-      if (size >= 0 && fseek(p, 0, SEEK_SET) == 0) {
-        std::string out(static_cast<size_t>(size), '\0');
-        const size_t n = size > 0 ? fread(&out[0], 1, out.size(), p) : 0;
-        fclose(p);
-        if (n != out.size()) {
-          fprintf(stderr, "[racon::io] error: unable to read file %s!\n", path.c_str());
-          exit(1);
-        }
-        return out;
-      }
-    }
-    fclose(p);
-  }
+namespace {
+
+std::string inflate_file(const std::string& path) {
   gzFile f = gzopen(path.c_str(), "rb");
   if (f == nullptr) {
     fprintf(stderr, "[racon::io] error: unable to open file %s!\n", path.c_str());
@@ -92,30 +78,64 @@ std::string read_file(const std::string& path) {
   return out;
 }
 
+}  // namespace
+
+FileText::FileText(const std::string& path) {
+  const int fd = open(path.c_str(), O_RDONLY);
+  if (fd < 0) {
+    fprintf(stderr, "[racon::io] error: unable to open file %s!\n", path.c_str());
+    exit(1);
+  }
+  struct stat st {};
+  unsigned char magic[2] = {0, 0};
+  const bool sized = fstat(fd, &st) == 0 && S_ISREG(st.st_mode);
+  const bool gz = pread(fd, magic, 2, 0) == 2 && magic[0] == 0x1f && magic[1] == 0x8b;
+  if (sized && !gz && st.st_size > 0) {
+    void* m = mmap(nullptr, static_cast<size_t>(st.st_size), PROT_READ,
+                   MAP_PRIVATE | MAP_POPULATE, fd, 0);
+    if (m != MAP_FAILED) {
+      map_ = m;
+      map_size_ = static_cast<size_t>(st.st_size);
+      data_ = static_cast<const char*>(m);
+      size_ = map_size_;
+    }
+  }
+  close(fd);
+  if (map_ == nullptr && !(sized && !gz && st.st_size == 0)) {
+    inflated_ = inflate_file(path);  // gzip (or a stream that cannot be mapped)
+    data_ = inflated_.data();
+    size_ = inflated_.size();
+  }
+}
+
+FileText::~FileText() {
+  if (map_ != nullptr) munmap(map_, map_size_);
+}
+
 namespace {
 
 // Line cursor over an in-memory file.
 struct Lines {
-  const std::string& text;
+  const char* text;
+  size_t size;
   size_t pos = 0;
 
-  explicit Lines(const std::string& t) : text(t) {}
+  explicit Lines(const FileText& t) : text(t.data()), size(t.size()) {}
 
   // Next line without the newline and trailing whitespace; false at end.
   bool next(const char** begin, size_t* length) {
-    if (pos >= text.size()) {
+    if (pos >= size) {
       return false;
     }
-    size_t end = text.find('\n', pos);
-    if (end == std::string::npos) {
-      end = text.size();
-    }
+    const void* nl = memchr(text + pos, '\n', size - pos);
+    const size_t end = nl != nullptr ? static_cast<size_t>(static_cast<const char*>(nl) - text)
+                                     : size;
     size_t stop = end;
     while (stop > pos && (text[stop - 1] == '\r' || text[stop - 1] == ' ' ||
                           text[stop - 1] == '\t')) {
       --stop;
     }
-    *begin = text.data() + pos;
+    *begin = text + pos;
     *length = stop - pos;
     pos = end + 1;
     return true;
@@ -206,7 +226,7 @@ Span joined(const std::vector<Span>& spans, uint32_t a, uint32_t b, std::string&
 // copies, the quality check) are then built in parallel on `pool`.
 void parse_sequences(const std::string& path, SequenceFormat format,
                      std::vector<std::unique_ptr<Sequence>>& dst, ThreadPool* pool) {
-  std::string text = read_file(path);
+  FileText text(path);
   Lines lines(text);
   const char* line;
   size_t length;
@@ -295,7 +315,7 @@ void parse_sequences(const std::string& path, SequenceFormat format,
 
 void parse_overlaps(const std::string& path, OverlapFormat format,
                     std::vector<std::unique_ptr<Overlap>>& dst) {
-  std::string text = read_file(path);
+  FileText text(path);
   Lines lines(text);
   const char* line;
   size_t length;
