@@ -293,6 +293,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--skip-banded", action="store_true",
                     help="skip the banded (-b) comparison run")
+    ap.add_argument("--skip-serving-probe", action="store_true",
+                    help="skip the 4-second serving-loop overload run")
     ap.add_argument("--kernel-only", action="store_true",
                     help="profiling runs: time only rh_poa_run (inputs resident), skip the "
                          "comparators and the CPU baseline")
@@ -490,6 +492,19 @@ def main() -> int:
         log(f"plain: {len(subset)} windows in {plain_s:.2f} s -> {plain_wps:.1f} windows/s")
         extra["plain_windows_per_s"] = round(plain_wps, 3)
         extra["batched_over_plain"] = round(windows_per_s / plain_wps, 2)
+
+        # ---- serving loop under overload (Poisson arrivals, 32 requests/s) ----
+        if not args.skip_serving_probe:
+            probe = argparse.Namespace(**vars(args))
+            probe.serving = "32"
+            probe.serving_seconds = 4.0
+            rep = serving_bench(windows, probe, (m, x, g))
+            row = rep["rates"][0]
+            extra["serving_overload"] = {
+                k: row[k] for k in ("rate", "requests", "windows", "windows_per_s", "p50_s",
+                                    "p99_s", "launches", "mixed_launches")}
+            extra["serving_overload"]["executors"] = args.serving_executors
+            extra["serving_overload"]["launch_windows"] = args.serving_launch
 
         # ---- overlap alignment kernel ----
         ab = racon_amd.AlignBatch()

@@ -65,13 +65,14 @@ struct Seg {
   __device__ uint8_t tc(int j) const { return rev ? t[tl - 1 - j] : t[j]; }
 };
 
-// Match words of one block as bit planes: rows holding A, C, G or T (acgt)
-// with their 2-bit codes (lo, hi), rows holding N, and rows past the query
-// end (pad), which match everything. A target byte then finds its rows with
-// a few word operations and no per-byte table.
+// Match words of one block: for each of A, C, G, T the rows holding it (or
+// past the query end, pad, which match everything), the rows holding N, and
+// pad. A target A/C/G/T byte picks its word with two selects on its 2-bit
+// code; other bytes take the N word or an exact byte compare (rare_eq).
 struct Peq {
-  Word lo, hi, acgt, n;
-  Word pad;  // rows beyond the query
+  Word a, c, g, t;  // match words of A, C, G, T (pad included)
+  Word n;           // rows holding N
+  Word pad;         // rows beyond the query
 };
 
 // 2-bit code of A, C, G, T (0..3) from the byte's bits 1 and 2; any byte
@@ -82,7 +83,8 @@ __device__ __forceinline__ bool is_acgt(int c, int code) {
 }
 
 __device__ Peq build_peq(const Seg& g, int block) {
-  Peq p{0, 0, 0, 0, 0};
+  Peq p{0, 0, 0, 0, 0, 0};
+  Word lo = 0, hi = 0, acgt = 0;
   const int r0 = block * 64;
   for (int i = 0; i < 64; ++i) {
     const int r = r0 + i;
@@ -94,14 +96,25 @@ __device__ Peq build_peq(const Seg& g, int block) {
     const int c = g.qc(r);
     const int code = code2(c);
     if (is_acgt(c, code)) {
-      p.acgt |= bit;
-      if (code & 1) p.lo |= bit;
-      if (code & 2) p.hi |= bit;
+      acgt |= bit;
+      if (code & 1) lo |= bit;
+      if (code & 2) hi |= bit;
     } else if (c == 'N') {
       p.n |= bit;
     }
   }
+  p.a = (acgt & ~lo & ~hi) | p.pad;
+  p.c = (acgt & lo & ~hi) | p.pad;
+  p.g = (acgt & ~lo & hi) | p.pad;
+  p.t = (acgt & lo & hi) | p.pad;
   return p;
+}
+
+// Match word of an A/C/G/T byte from its 2-bit code.
+__device__ __forceinline__ Word acgt_eq(const Peq& p, int code) {
+  const Word e01 = (code & 1) ? p.c : p.a;
+  const Word e23 = (code & 1) ? p.t : p.g;
+  return (code & 2) ? e23 : e01;
 }
 
 // Match word for a target byte outside A/C/G/T/N (exact byte compare).
@@ -208,7 +221,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
       wave_sync();
       continue;
     }
-    Peq peq{0, 0, 0, 0, 0};
+    Peq peq{0, 0, 0, 0, 0, 0};
     if (active_lane) {
       peq = build_peq(g, b);
     }
@@ -273,10 +286,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
        const int hin = lane == 0 ? h0 : (up >> 8) - 1;
        // (lanes past the strip compute unused values; they store nothing and
        // feed only lanes past the strip)
-       const int code = code2(tc);
-       const Word mlo = (code & 1) ? ~Word(0) : Word(0);
-       const Word mhi = (code & 2) ? ~Word(0) : Word(0);
-       const Word eq = (peq.acgt & ~((peq.lo ^ mlo) | (peq.hi ^ mhi))) | peq.pad;
+       const Word eq = acgt_eq(peq, code2(tc));
        const int hout = block_step(P, M, eq, hin);
        score += hout;
        if (store && active_lane) {
@@ -306,9 +316,7 @@ __device__ int forward(const Seg& g, int stop, int32_t* col, bool store, const S
         // rows equal to the target byte (exact byte equality, as edlib's
         // alphabet without equalities)
         const int code = code2(tc);
-        const Word mlo = (code & 1) ? ~Word(0) : Word(0);
-        const Word mhi = (code & 2) ? ~Word(0) : Word(0);
-        Word eq = (peq.acgt & ~((peq.lo ^ mlo) | (peq.hi ^ mhi))) | peq.pad;
+        Word eq = acgt_eq(peq, code);
         if (!is_acgt(tc, code)) {
           eq = tc == 'N' ? (peq.n | peq.pad) : rare_eq(g, b, tc, peq.pad);
         }
