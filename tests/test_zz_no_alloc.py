@@ -49,9 +49,12 @@ class Hip:
 
     def to_device(self, data: bytes):
         p = self.alloc(len(data))
+        self.copy_in(p, data)
+        return p
+
+    def copy_in(self, p, data: bytes):
         buf = ctypes.create_string_buffer(data, len(data))
         assert self.rt.hipMemcpy(p, buf, len(data), 1) == 0  # host to device
-        return p
 
     def to_host(self, p, nbytes) -> bytes:
         buf = ctypes.create_string_buffer(max(1, nbytes))
@@ -64,7 +67,11 @@ class Hip:
         self.held = []
 
 
-def noalloc_align(pairs, hip):
+def device_batch(hip, pairs):
+    """Caller-owned device copies of the pairs (arena, rh_dev_pair records)
+    and the output buffers. Returns the rh_align_run_noalloc arguments after
+    (device, stream), a reader of the results (ops, distances, statuses) and
+    a clearer that refills the outputs with junk."""
     n = len(pairs)
     blob = bytearray()
     recs = np.zeros(n, dtype=np.dtype(_lib.DEV_PAIR_FIELDS, align=True))
@@ -83,16 +90,31 @@ def noalloc_align(pairs, hip):
     d_pairs = hip.to_device(recs.tobytes())
     ops = hip.to_device(b"\xff" * max(1, out))
     results = hip.to_device(b"\0" * (n * res_dt.itemsize))
+    args = (n, d_pairs, arena, max(len(q) for q, _ in pairs), max(len(t) for _, t in pairs), ops,
+            results)
+
+    def read():
+        res = np.frombuffer(hip.to_host(results, n * res_dt.itemsize), dtype=res_dt)
+        ops_h = hip.to_host(ops, out)
+        got_ops = [ops_h[int(recs[i]["ops_offset"]):int(recs[i]["ops_offset"]) +
+                         int(res[i]["n_ops"])] for i in range(n)]
+        return (got_ops, [int(d) for d in res["edit_distance"]],
+                [int(s) for s in res["status"]])
+
+    def clear():
+        hip.copy_in(ops, b"\xff" * max(1, out))
+        hip.copy_in(results, b"\x5a" * (n * res_dt.itemsize))
+
+    return args, read, clear
+
+
+def noalloc_align(pairs, hip):
+    args, read, _ = device_batch(hip, pairs)
     lib = _lib.load_library()
-    rc = lib.rh_align_run_noalloc(0, None, n, d_pairs, arena, max(len(q) for q, _ in pairs),
-                                  max(len(t) for _, t in pairs), ops, results)
+    rc = lib.rh_align_run_noalloc(0, None, *args)
     assert rc == 0, lib.rh_last_error().decode()
     assert hip.rt.hipDeviceSynchronize() == 0
-    res = np.frombuffer(hip.to_host(results, n * res_dt.itemsize), dtype=res_dt)
-    ops_h = hip.to_host(ops, out)
-    got_ops = [ops_h[int(recs[i]["ops_offset"]):int(recs[i]["ops_offset"]) + int(res[i]["n_ops"])]
-               for i in range(n)]
-    return got_ops, [int(d) for d in res["edit_distance"]], [int(s) for s in res["status"]]
+    return read()
 
 
 def test_noalloc_matches_batch_entry_twice():
