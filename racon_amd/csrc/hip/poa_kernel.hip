@@ -85,6 +85,11 @@ constexpr uint16_t kCodeVert = 0x4000;
 constexpr uint16_t kCodeHorz = 0x8000;
 constexpr uint16_t kCodeViaK = 0x2000;
 constexpr uint32_t kCodePayload = 0x1fff;
+// Rows with one predecessor in the packed DP store only the move of each
+// cell, as a 2-bit kind (0 diagonal, 1 vertical, 2 horizontal; column j at
+// bits 2 (j - 1) % 8 of byte (j - 1) / 4 of the row's code memory): the
+// predecessor is implied, and column 0 is always vertical.
+constexpr uint32_t kKindHorz = 2;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
@@ -353,6 +358,8 @@ __device__ __forceinline__ void wave_fence() {
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 constexpr int kRowMetaSink = 0x100;
+// The row has one predecessor (or only the virtual start row).
+constexpr int kRowMetaSingle = 0x200;
 // Bits 10..14 of a row's meta word: where the row's scores live while later
 // rows need them. Slots 0..kMaxLdsSlots-1 are LDS rows; kSlotGlobal rows go
 // through the slot's global H matrix; kSlotNone rows are read by no later row.
@@ -892,8 +899,6 @@ __device__ __forceinline__ void packed_compute(const PackedStrip<W>& ps, const u
     pre[q] = carry;
   }
   const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
-  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
-  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
 #pragma unroll
   for (int q = 0; q < H2; ++q) {
     o.h[q] = pk_max_lo(pre[q], static_cast<uint32_t>(excl));
@@ -910,8 +915,8 @@ __device__ __forceinline__ void packed_compute(const PackedStrip<W>& ps, const u
       const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
       o.cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
     } else {
-      // pay + fd * (0x4000 + fv * (0x4000 - pay))
-      o.cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+      // the move kind: fd * (1 + fv)
+      o.cd[q] = pk_mad(fd, fv, fd);
     }
     // computed here, not sunk into the store branch, so that a second row's
     // scan can interleave with this row's codes
@@ -929,9 +934,35 @@ __device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t 
                                                 int end_u) {
   constexpr int H2 = W / 2;
   if (live) {
+    if (kMulti) {
 #pragma unroll
-    for (int q = 0; q < W / 4; ++q) {
-      *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(o.cd[2 * q], o.cd[2 * q + 1]);
+      for (int q = 0; q < W / 4; ++q) {
+        *reinterpret_cast<uint2*>(Cr + (j0 - 1) + 4 * q) = make_uint2(o.cd[2 * q], o.cd[2 * q + 1]);
+      }
+    } else {
+      // move kinds, 2 bits per column: pair q's halves (columns 2q, 2q + 1
+      // of the strip) go to bits 4q and 4q + 2
+      auto pack8 = [&](int g) -> uint32_t {  // pairs 4g..4g+3: 16 bits
+        uint32_t x = o.cd[min(4 * g, H2 - 1)];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) x |= o.cd[min(4 * g + q, H2 - 1)] << (4 * q);
+        return (x & 0xffffu) | (x >> 14);
+      };
+      uint8_t* K = reinterpret_cast<uint8_t*>(Cr) + (j0 - 1) / 4;
+      if constexpr (W == 4) {
+        const uint32_t x = o.cd[0] | (o.cd[1] << 4);
+        *K = static_cast<uint8_t>(x | (x >> 14));
+      } else if constexpr (W == 8) {
+        *reinterpret_cast<uint16_t*>(K) = static_cast<uint16_t>(pack8(0));
+      } else if constexpr (W == 12) {
+        const uint32_t v = pack8(0);
+        const uint32_t x = o.cd[4] | (o.cd[5] << 4);
+        K[0] = static_cast<uint8_t>(v);
+        K[1] = static_cast<uint8_t>(v >> 8);
+        K[2] = static_cast<uint8_t>(x | (x >> 14));
+      } else {
+        *reinterpret_cast<uint32_t*>(K) = pack8(0) | (pack8(1) << 16);
+      }
     }
     if (own < kSlotGlobal) {
       int16_t* row = ring + own * pitch;
@@ -950,8 +981,7 @@ __device__ __forceinline__ int32_t packed_store(const PackedOut<W>& o, uint32_t 
   if (lane == 0) {
     if (own < kSlotGlobal) ring[own * pitch + pitch - 1] = static_cast<int16_t>(o.h0);
     if (own == kSlotGlobal) Hg[0] = o.h0;
-    Cr[pitch - 1] = static_cast<uint16_t>(
-        kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu)) : (kCodeVert | pay0));
+    if (kMulti) Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | kCodeViaK | (k0 & 0xffffu));
   }
   // (each pair opaque, so that the selection stays register selects rather
   // than an indexed load from a stack copy of the row)
@@ -2078,7 +2108,7 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
     }
 #pragma unroll
     for (int i = 0; i < kU; ++i) {
-      if (v[i] >= 0) meta[i] |= oc[i] == 0 ? kRowMetaSink : 0;
+      if (v[i] >= 0) meta[i] |= (oc[i] == 0 ? kRowMetaSink : 0) | (np[i] <= 1 ? kRowMetaSingle : 0);
       max_np = max(max_np, v[i] >= 0 ? max(np[i], 1) : 0);  // row 0 stands in for none
     }
 #pragma unroll
@@ -2746,6 +2776,9 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       const int32_t* pred_off = s.pred_off();
       const int32_t* pred_rows = s.pred_rows();
       const uint64_t* row_pred4 = s.row_pred4();
+      const int32_t* row_meta = s.row_meta();
+      // the packed DP (not banded) keeps move kinds for one-predecessor rows
+      const bool kinds = packed && !banded && L <= 16 * kWave;
       int32_t* pos_node = s.path();
       constexpr uint32_t kNextExit = 0xff;  // successor: the decoder takes the move
       constexpr uint32_t kDiagBit = 0x100;
@@ -2766,8 +2799,22 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
         const int rc = max(ri, 1);
         const int cj = j0 - 2 * (lane & 3);
         const uint16_t* crow = code + static_cast<int64_t>(rc) * pitch;
-        const uint32_t c0 = crow[code_index(max(cj, 0), pitch)];
-        const uint32_t c1 = crow[code_index(max(cj - 1, 0), pitch)];
+        uint32_t c0, c1;
+        if (kinds && (row_meta[rc - 1] & kRowMetaSingle) != 0) {
+          // a one-predecessor row of the packed DP: its 2-bit move kinds,
+          // turned into the codes of moves to in-edge 0
+          const uint8_t* krow = reinterpret_cast<const uint8_t*>(crow);
+          auto kind_code = [&](int j) -> uint32_t {
+            if (j <= 0) return kCodeVert | kCodeViaK;
+            const uint32_t k = (static_cast<uint32_t>(krow[(j - 1) >> 2]) >> (2 * ((j - 1) & 3))) & 3u;
+            return k == kKindHorz ? kCodeHorz : ((k << 14) | kCodeViaK);
+          };
+          c0 = kind_code(cj);
+          c1 = kind_code(cj - 1);
+        } else {
+          c0 = crow[code_index(max(cj, 0), pitch)];
+          c1 = crow[code_index(max(cj - 1, 0), pitch)];
+        }
         const uint64_t d4 = row_pred4[rc - 1];  // first four predecessor row deltas of the row
         const int nrow = sub_order[rc - 1];     // graph node of the row
         const uint32_t cw = ri >= 1 ? ((cj >= 0 ? c0 : 0u) | ((cj >= 1 ? c1 : 0u) << 16)) : 0u;
