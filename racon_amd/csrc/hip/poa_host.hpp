@@ -16,7 +16,13 @@ namespace rh {
 // layer_first relative to the layer records, out_off relative to an output
 // region of out_bytes) and, per window, whether it needs the kernel.
 struct PackedWindows {
-  HostArena arena;
+  HostArena arena;                      // the input bytes, unless staged (below)
+  // With a pinned staging buffer the bytes go there instead: the copies are
+  // left to the upload (copy_and_upload), which overlaps them with the copy
+  // to the device. host_arena points at the bytes either way.
+  std::vector<ArenaCopy> staged_copies;
+  const uint8_t* host_arena = nullptr;
+  uint64_t arena_bytes = 0;
   std::vector<PoaWindow> windows;
   std::vector<PoaLayer> layers;
   std::vector<uint8_t> shallow;         // 1: fewer than 3 sequences (backbone kept)
@@ -28,7 +34,8 @@ struct PackedWindows {
 // Packs n windows (racon's std::sort of layers by begin, window.cpp:78-85;
 // layer checks of Window::add_layer, window.cpp:45-63). Fails with
 // RH_ERR_ARGUMENT on a malformed window or layer.
-int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out);
+int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out,
+                     PinnedBuffer* staging = nullptr);
 
 // Estimated cost of a window: graph bound x (longest layer + 1).
 uint64_t window_cost(const PoaWindow& w);

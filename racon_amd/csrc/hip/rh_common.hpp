@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -84,6 +85,89 @@ inline void run_copies(uint8_t* base, const std::vector<ArenaCopy>& copies, uint
     pool.emplace_back(run, c, std::min(nc, c + chunk));
   }
   for (auto& t : pool) t.join();
+}
+
+// Pinned host memory that grows on demand (never shrinks): staging for
+// uploads that a batch object repeats, so that the copy engine reads it
+// directly instead of through the runtime's bounce buffers.
+struct PinnedBuffer {
+  uint8_t* ptr = nullptr;
+  uint64_t bytes = 0;
+  hipError_t reserve(uint64_t n) {
+    if (n <= bytes) return hipSuccess;
+    release();
+    const uint64_t want = n + n / 4;  // headroom for the next, slightly larger batch
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) return e;
+    ptr = static_cast<uint8_t*>(p);
+    bytes = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (ptr != nullptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+// Runs the copies (laid out back to back in arena order, covering
+// [0, bytes)) into the pinned `base` on up to 16 host threads, and sends each
+// finished chunk of about 8 MiB to `dev` on `stream` while later chunks are
+// still being copied. The copies must all be finished (and the stream
+// synchronised) before `base` is reused.
+inline hipError_t copy_and_upload(uint8_t* base, const std::vector<ArenaCopy>& copies,
+                                  uint64_t bytes, uint8_t* dev, hipStream_t stream) {
+  const size_t nc = copies.size();
+  if (nc == 0 || bytes == 0) return hipSuccess;
+  constexpr uint64_t kChunk = 8ull << 20;
+  constexpr size_t kClaim = 256;  // copies per claim of a worker
+  // chunk k: copies [cb[k], cb[k + 1]), arena bytes from copies[cb[k]].dst
+  std::vector<size_t> cb{0};
+  uint64_t start = 0;
+  for (size_t c = 0; c < nc; ++c) {
+    const uint64_t end = copies[c].dst + copies[c].len;
+    if (end - start >= kChunk) {
+      cb.push_back(c + 1);
+      start = end;
+    }
+  }
+  if (cb.back() != nc) cb.push_back(nc);
+  const size_t nk = cb.size() - 1;
+  std::unique_ptr<std::atomic<size_t>[]> left(new std::atomic<size_t>[nk]);
+  for (size_t k = 0; k < nk; ++k) left[k].store(cb[k + 1] - cb[k], std::memory_order_relaxed);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const size_t c0 = next.fetch_add(kClaim, std::memory_order_relaxed);
+      if (c0 >= nc) return;
+      const size_t c1 = std::min(nc, c0 + kClaim);
+      for (size_t c = c0; c < c1; ++c) {
+        if (copies[c].len != 0) std::memcpy(base + copies[c].dst, copies[c].src, copies[c].len);
+      }
+      // credit every chunk the claimed range touches
+      size_t k = static_cast<size_t>(std::upper_bound(cb.begin(), cb.end(), c0) - cb.begin()) - 1;
+      for (size_t c = c0; c < c1; ++k) {
+        const size_t e = std::min(c1, cb[k + 1]);
+        left[k].fetch_sub(e - c, std::memory_order_release);
+        c = e;
+      }
+    }
+  };
+  const unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < workers; ++t) pool.emplace_back(work);
+  hipError_t err = hipSuccess;
+  for (size_t k = 0; k < nk; ++k) {
+    while (left[k].load(std::memory_order_acquire) != 0) std::this_thread::yield();
+    const uint64_t b0 = copies[cb[k]].dst;
+    const uint64_t b1 = k + 1 < nk ? copies[cb[k + 1]].dst : bytes;
+    if (err == hipSuccess && b1 > b0) {
+      err = hipMemcpyAsync(dev + b0, base + b0, b1 - b0, hipMemcpyHostToDevice, stream);
+    }
+  }
+  for (auto& t : pool) t.join();
+  return err;
 }
 
 // A copy on `stream`, waited for. Unlike hipMemcpy (the null stream, which

@@ -94,9 +94,13 @@ bool is_retry_status(int32_t s) {
          s == kPoaBandMiss || s == kPoaAlignedFull;
 }
 
-int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out) {
+int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out,
+                     PinnedBuffer* staging) {
   PackedWindows& P = *out;
   P.arena.clear();
+  P.staged_copies.clear();
+  P.host_arena = nullptr;
+  P.arena_bytes = 0;
   P.windows.assign(n, PoaWindow{});
   P.layers.clear();
   P.gpu_order.clear();
@@ -200,8 +204,20 @@ int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWind
     P.gpu_order.push_back(i);
   }
   P.out_bytes = out_off;
-  P.arena.resize(cur);
-  run_copies(P.arena.data(), copies, cur);
+  P.arena_bytes = cur;
+  bool staged = false;
+  if (staging != nullptr) {
+    staged = staging->reserve(std::max<uint64_t>(cur, 1)) == hipSuccess;
+    if (!staged) (void)hipGetLastError();  // no pinned memory: the pageable path
+  }
+  if (staged) {
+    P.staged_copies = std::move(copies);
+    P.host_arena = staging->ptr;
+  } else {
+    P.arena.resize(cur);
+    run_copies(P.arena.data(), copies, cur);
+    P.host_arena = P.arena.data();
+  }
   // largest estimated cost first (longest-processing-time order)
   std::stable_sort(P.gpu_order.begin(), P.gpu_order.end(), [&](uint32_t x, uint32_t y) {
     return window_cost(P.windows[x]) > window_cost(P.windows[y]);
@@ -439,6 +455,10 @@ struct rh_poa_batch {
   std::vector<rh::PoaResult> results;
 
   rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results;
+  // pinned staging of the input bytes, from the batch's second upload on
+  // (a batch object that is reused pays the pinning once)
+  rh::PinnedBuffer staging;
+  uint64_t uploads = 0;
   bool uploaded = false;
   bool ran = false;
   bool submitted = false;  // between rh_poa_submit and rh_poa_wait
@@ -525,6 +545,7 @@ void rh_poa_destroy(rh_poa_batch* b) {
   b->d_layers.release();
   b->d_out.release();
   b->d_results.release();
+  b->staging.release();
   delete b;
 }
 
@@ -553,11 +574,13 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   auto t0 = std::chrono::steady_clock::now();
   b->stats = rh_poa_stats{};
   b->uploaded = false;
-  const int32_t rc = rh::pack_windows(n, windows, b->trim, &b->packed);
+  const int32_t rc =
+      rh::pack_windows(n, windows, b->trim, &b->packed, b->uploads > 0 ? &b->staging : nullptr);
   if (rc != RH_OK) {
     rh::set_error(std::string("rh_poa_upload: ") + rh::last_error());
     return rc;
   }
+  ++b->uploads;
   const rh::PackedWindows& P = b->packed;
   b->results.assign(n, make_result());
   for (uint32_t i = 0; i < n; ++i) {
@@ -566,11 +589,11 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   b->stats.pack_ms = rh::ms_since(t0);
   b->stats.windows = n;
   b->stats.gpu_windows = P.gpu_order.size();
-  b->stats.input_bytes = P.arena.size();
+  b->stats.input_bytes = P.arena_bytes;
 
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t st = b->lane.stream;
-  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(P.arena.size(), 1)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(P.arena_bytes, 1)), RH_ERR_MEMORY);
   RH_HIP_CHECK(b->d_windows.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaWindow)),
                RH_ERR_MEMORY);
   RH_HIP_CHECK(b->d_layers.reserve(std::max<size_t>(P.layers.size(), 1) * sizeof(rh::PoaLayer)),
@@ -578,8 +601,12 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaResult)),
                RH_ERR_MEMORY);
   RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(P.out_bytes, 1)), RH_ERR_MEMORY);
-  if (!P.arena.empty()) {
-    RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, P.arena.data(), P.arena.size(),
+  if (!P.staged_copies.empty()) {
+    RH_HIP_CHECK(rh::copy_and_upload(b->staging.ptr, P.staged_copies, P.arena_bytes,
+                                     b->d_arena.as<uint8_t>(), st),
+                 RH_ERR_DEVICE);
+  } else if (P.arena_bytes > 0) {
+    RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, P.host_arena, P.arena_bytes,
                                 hipMemcpyHostToDevice, st),
                  RH_ERR_DEVICE);
   }
@@ -887,12 +914,12 @@ int32_t rh_poa_download(rh_poa_batch* b, char* out, uint64_t capacity, uint64_t*
     uint8_t st;
     const int32_t rs = b->results[i].status;
     if (P.shallow[i]) {
-      src = P.arena.data() + P.windows[i].backbone_off;
+      src = P.host_arena + P.windows[i].backbone_off;
       len = P.windows[i].backbone_len;
       st = RH_WINDOW_UNPOLISHED;
     } else if (rs != rh::kPoaPolished && rs != rh::kPoaPolishedUntrimmed) {
       // outside the device limits: the backbone, for the caller's CPU path
-      src = P.arena.data() + P.windows[i].backbone_off;
+      src = P.host_arena + P.windows[i].backbone_off;
       len = P.windows[i].backbone_len;
       st = RH_WINDOW_REJECTED;
     } else {

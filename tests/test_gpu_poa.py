@@ -168,3 +168,29 @@ def test_submit_wait_two_batches_in_flight():
         for i, w in enumerate(wins):
             cons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
             assert (out[i], status[i]) == (cons, ost), f"window {i}"
+
+
+def test_reused_batch_stages_uploads_in_pinned_memory():
+    # a batch object's second and later uploads pack into pinned staging and
+    # copy the arena to the device chunk by chunk while packing; the results
+    # must not depend on it (shallow windows return their backbone from the
+    # staged bytes)
+    big = make_windows(80, 400, length=500, depth=30)  # ~12 MB arena: several chunks
+    small = make_windows(81, 5, length=300, depth=8) + make_windows(82, 3, length=200, depth=1)
+    b = _lib.PoaBatch(3, -5, -4, trim=True)
+    try:
+        b.upload(big)  # first upload: pageable arena
+        b.run()
+        first = b.download()
+        b.upload(small)
+        b.run()
+        out, status = b.download()
+        for i, w in enumerate(small):
+            cons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
+            assert (out[i], status[i]) == (cons, ost), f"window {i}"
+        b.upload(big)
+        b.run()
+        again = b.download()
+    finally:
+        b.close()
+    assert again == first
