@@ -7,6 +7,7 @@
 // first) or one launch per window, and reruns the few windows whose graph
 // outgrew a default slot in slots sized from their exact bounds.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -69,9 +70,6 @@ uint32_t ring_rows_for(uint32_t cols, uint32_t tuned) {
 // least 1.5 times the mean: the ones that can outlast the rest of the batch
 // on a single wave (2x and 1.25x measured slower on a 10k-window batch).
 uint32_t priority_windows(const PoaWindow* host, const std::vector<uint32_t>& order) {
-#ifdef RH_POA_NO_PRIORITY
-  return 0;
-#endif
   if (order.empty()) {
     return 0;
   }
@@ -94,6 +92,51 @@ bool is_retry_status(int32_t s) {
          s == kPoaBandMiss || s == kPoaAlignedFull;
 }
 
+namespace {
+
+// Runs body(begin, end) over [0, n) on up to 16 host threads (serially for a
+// small n).
+template <typename F>
+void parallel_ranges(uint32_t n, uint32_t serial_below, F&& body) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (hw <= 1 || n < serial_below) {
+    body(0u, n);
+    return;
+  }
+  const uint32_t chunk = (n + 4 * hw - 1) / (4 * hw);
+  std::atomic<uint32_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const uint32_t b = next.fetch_add(chunk);
+      if (b >= n) return;
+      body(b, std::min(n, b + chunk));
+    }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < hw; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+}
+
+// Per-window sizes of a packing pass (first pass, in parallel).
+struct WindowPlan {
+  uint64_t bytes = 0;      // arena bytes
+  uint32_t copies = 0;     // arena copies
+  uint32_t layers = 0;     // layer records
+  uint32_t bad_layer = 0;  // 1 + index of the first invalid layer, else 0
+  uint64_t node_bound = 0;
+  uint32_t max_len = 0;
+  bool malformed = false;
+  bool shallow = false;
+};
+
+}  // namespace
+
+// Two passes over the windows, each on up to 16 host threads: sizes and
+// checks, then (after prefix sums) the window and layer records and the
+// arena copies at their final offsets. The layout is the one of appending
+// every window in order: backbone, its quality, then each layer's bytes and
+// quality in racon's order.
 int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWindows* out,
                      PinnedBuffer* staging) {
   PackedWindows& P = *out;
@@ -108,102 +151,131 @@ int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWind
   P.host_rejected.assign(n, 0);
   P.out_bytes = 0;
 
-  uint64_t total = 0, n_layers = 0;
+  std::vector<WindowPlan> plan(n);
+  parallel_ranges(n, 512, [&](uint32_t b, uint32_t e) {
+    for (uint32_t i = b; i < e; ++i) {
+      const rh_window& w = windows[i];
+      WindowPlan& q = plan[i];
+      if (w.backbone == nullptr || w.backbone_quality == nullptr || w.backbone_length == 0 ||
+          (w.n_layers > 0 && w.layers == nullptr)) {
+        q.malformed = true;
+        continue;
+      }
+      for (uint32_t k = 0; k < w.n_layers; ++k) {
+        const rh_layer& l = w.layers[k];
+        if (l.sequence == nullptr || l.length == 0 || l.begin >= l.end ||
+            l.end > w.backbone_length) {
+          q.bad_layer = k + 1;
+          break;
+        }
+      }
+      q.bytes = 2ull * w.backbone_length;
+      q.copies = 2;
+      q.shallow = w.n_layers + 1 < 3;
+      if (q.shallow) continue;
+      q.layers = w.n_layers;
+      uint64_t sum_len = 0;
+      for (uint32_t k = 0; k < w.n_layers; ++k) {
+        const rh_layer& l = w.layers[k];
+        const uint32_t parts = l.quality != nullptr ? 2u : 1u;
+        q.bytes += static_cast<uint64_t>(parts) * l.length;
+        q.copies += parts;
+        sum_len += l.length;
+        q.max_len = std::max(q.max_len, l.length);
+      }
+      q.node_bound = w.backbone_length + sum_len;
+    }
+  });
   for (uint32_t i = 0; i < n; ++i) {
-    const rh_window& w = windows[i];
-    if (w.backbone == nullptr || w.backbone_quality == nullptr || w.backbone_length == 0 ||
-        (w.n_layers > 0 && w.layers == nullptr)) {
+    if (plan[i].malformed) {
       set_error("window " + std::to_string(i) + " is malformed");
       return RH_ERR_ARGUMENT;
     }
-    total += 2ull * w.backbone_length;
-    for (uint32_t k = 0; k < w.n_layers; ++k) {
-      total += 2ull * w.layers[k].length;
-    }
-    n_layers += w.n_layers;
   }
-  P.layers.reserve(n_layers);
-
-  // offsets first (same layout as appending in order), bytes copied after
-  std::vector<ArenaCopy> copies;
-  copies.reserve(total / 256 + 2ull * n);
-  uint64_t cur = 0;
-  auto place = [&](const char* src, uint32_t len) {
-    copies.push_back(ArenaCopy{src, cur, len});
-    const uint64_t at = cur;
-    cur += len;
-    return at;
-  };
-  uint64_t out_off = 0;
-  std::vector<uint32_t> rank;
   for (uint32_t i = 0; i < n; ++i) {
-    const rh_window& w = windows[i];
-    PoaWindow& d = P.windows[i];
-    d.backbone_off = place(w.backbone, w.backbone_length);
-    d.backbone_qual = place(w.backbone_quality, w.backbone_length);
-    d.backbone_len = w.backbone_length;
-    d.n_layers = w.n_layers;
-    d.layer_first = static_cast<uint32_t>(P.layers.size());
-    d.offset = static_cast<uint32_t>(0.01 * w.backbone_length);
-    d.flags = (w.tgs ? 1u : 0u) | (trim ? 2u : 0u);
+    if (plan[i].bad_layer != 0) {
+      set_error("window " + std::to_string(i) + " layer " +
+                std::to_string(plan[i].bad_layer - 1) + " is invalid");
+      return RH_ERR_ARGUMENT;
+    }
+  }
 
-    for (uint32_t k = 0; k < w.n_layers; ++k) {
-      const rh_layer& l = w.layers[k];
-      if (l.sequence == nullptr || l.length == 0 || l.begin >= l.end ||
-          l.end > w.backbone_length) {
-        set_error("window " + std::to_string(i) + " layer " + std::to_string(k) +
-                  " is invalid");
-        return RH_ERR_ARGUMENT;
-      }
-    }
-    if (w.n_layers + 1 < 3) {
-      P.shallow[i] = 1;
-      d.n_layers = 0;
-      continue;
-    }
-    // racon's layer order: std::sort of indices 1..n by begin (window.cpp:78-85)
-    rank.resize(w.n_layers + 1);
-    std::iota(rank.begin(), rank.end(), 0u);
-    std::sort(rank.begin() + 1, rank.end(), [&](uint32_t lhs, uint32_t rhs) {
-      return w.layers[lhs - 1].begin < w.layers[rhs - 1].begin;
-    });
-    uint64_t sum_len = 0;
-    uint32_t max_len = 0;
-    for (uint32_t k = 1; k <= w.n_layers; ++k) {
-      const rh_layer& l = w.layers[rank[k] - 1];
-      PoaLayer pl{};
-      pl.seq_off = place(l.sequence, l.length);
-      if (l.quality != nullptr) {
-        pl.qual_off = place(l.quality, l.length);
-      } else {
-        pl.qual_off = kNoQuality;
-      }
-      pl.len = l.length;
-      pl.begin = l.begin;
-      pl.end = l.end;
-      P.layers.push_back(pl);
-      sum_len += l.length;
-      max_len = std::max(max_len, l.length);
-    }
-    const uint64_t node_bound = w.backbone_length + sum_len;
-    const uint64_t edge_bound = w.backbone_length + sum_len + w.n_layers;
+  // prefix sums: arena offset, first copy, first layer record, output offset
+  std::vector<uint64_t> at(n + 1), out_at(n + 1);
+  std::vector<uint32_t> copy_at(n + 1), layer_at(n + 1);
+  at[0] = out_at[0] = 0;
+  copy_at[0] = layer_at[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const WindowPlan& q = plan[i];
     // DP rows are packed in 24 bits of a predecessor entry: a larger graph is
     // refused here (RH_WINDOW_REJECTED, the caller's CPU path takes it); the
     // in-edge index of a move code (13 bits) is checked on the device
-    if (node_bound >= (1ull << 24) - 1) {
-      P.host_rejected[i] = 1;
-      continue;
-    }
-    d.node_bound = static_cast<uint32_t>(node_bound);
-    d.edge_bound = static_cast<uint32_t>(edge_bound);
-    d.max_layer_len = max_len;
-    d.out_cap = static_cast<uint32_t>(
-        std::min<uint64_t>(node_bound, 2ull * w.backbone_length + 1024));
-    d.out_off = out_off;
-    out_off += align_up(d.out_cap, 16);
-    P.gpu_order.push_back(i);
+    const bool gpu = !q.shallow && q.node_bound < (1ull << 24) - 1;
+    const uint64_t out_cap =
+        gpu ? std::min<uint64_t>(q.node_bound, 2ull * windows[i].backbone_length + 1024) : 0;
+    at[i + 1] = at[i] + q.bytes;
+    copy_at[i + 1] = copy_at[i] + q.copies;
+    layer_at[i + 1] = layer_at[i] + q.layers;
+    out_at[i + 1] = out_at[i] + (gpu ? align_up(out_cap, 16) : 0);
+    if (q.shallow) P.shallow[i] = 1;
+    if (!q.shallow && !gpu) P.host_rejected[i] = 1;
   }
-  P.out_bytes = out_off;
+  const uint64_t cur = at[n];
+  std::vector<ArenaCopy> copies(copy_at[n]);
+  P.layers.resize(layer_at[n]);
+
+  parallel_ranges(n, 512, [&](uint32_t b, uint32_t e) {
+    std::vector<uint32_t> rank;
+    for (uint32_t i = b; i < e; ++i) {
+      const rh_window& w = windows[i];
+      const WindowPlan& q = plan[i];
+      PoaWindow& d = P.windows[i];
+      uint64_t pos = at[i];
+      ArenaCopy* cp = copies.data() + copy_at[i];
+      auto place = [&](const char* src, uint32_t len) {
+        *cp++ = ArenaCopy{src, pos, len};
+        const uint64_t off = pos;
+        pos += len;
+        return off;
+      };
+      d.backbone_off = place(w.backbone, w.backbone_length);
+      d.backbone_qual = place(w.backbone_quality, w.backbone_length);
+      d.backbone_len = w.backbone_length;
+      d.n_layers = q.shallow ? 0 : w.n_layers;
+      d.layer_first = layer_at[i];
+      d.offset = static_cast<uint32_t>(0.01 * w.backbone_length);
+      d.flags = (w.tgs ? 1u : 0u) | (trim ? 2u : 0u);
+      if (q.shallow) continue;
+      // racon's layer order: std::sort of indices 1..n by begin (window.cpp:78-85)
+      rank.resize(w.n_layers + 1);
+      std::iota(rank.begin(), rank.end(), 0u);
+      std::sort(rank.begin() + 1, rank.end(), [&](uint32_t lhs, uint32_t rhs) {
+        return w.layers[lhs - 1].begin < w.layers[rhs - 1].begin;
+      });
+      PoaLayer* pl = P.layers.data() + layer_at[i];
+      for (uint32_t k = 1; k <= w.n_layers; ++k) {
+        const rh_layer& l = w.layers[rank[k] - 1];
+        PoaLayer r{};
+        r.seq_off = place(l.sequence, l.length);
+        r.qual_off = l.quality != nullptr ? place(l.quality, l.length) : kNoQuality;
+        r.len = l.length;
+        r.begin = l.begin;
+        r.end = l.end;
+        *pl++ = r;
+      }
+      if (P.host_rejected[i]) continue;
+      d.node_bound = static_cast<uint32_t>(q.node_bound);
+      d.edge_bound = static_cast<uint32_t>(q.node_bound + w.n_layers);
+      d.max_layer_len = q.max_len;
+      d.out_cap = static_cast<uint32_t>(
+          std::min<uint64_t>(q.node_bound, 2ull * w.backbone_length + 1024));
+      d.out_off = out_at[i];
+    }
+  });
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!P.shallow[i] && !P.host_rejected[i]) P.gpu_order.push_back(i);
+  }
+  P.out_bytes = out_at[n];
   P.arena_bytes = cur;
   bool staged = false;
   if (staging != nullptr) {
