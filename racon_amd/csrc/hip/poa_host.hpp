@@ -40,6 +40,13 @@ int32_t pack_windows(uint32_t n, const rh_window* windows, bool trim, PackedWind
 // Estimated cost of a window: graph bound x (longest layer + 1).
 uint64_t window_cost(const PoaWindow& w);
 
+// Graph nodes a main pass sizes a window's slot for: the backbone plus the
+// larger of twice the longest layer and a quarter of all layer bytes (the
+// node bound counts every layer byte as a new node; racon's sample reads and
+// a 5 Mbp 50x ONT contig grow at most 0.14 and 0.08 nodes per layer byte).
+// A graph that outgrows it ends kPoaNodeOverflow and reruns in exact slots.
+uint64_t graph_estimate(const PoaWindow& w);
+
 // Statuses after which a window reruns in a slot sized from its exact bounds.
 bool is_retry_status(int32_t s);
 
@@ -104,8 +111,10 @@ struct PassLaunch {
   PoaSlotLayout layout{};
 };
 
-// The plan for windows `ws`: graph arrays for the largest, the row pitch of
-// the longest layer, DP rows capped at row_cap (0: exact, node bound + 1),
+// The plan for windows `ws`: graph arrays for the largest (by the graph
+// estimate, or by the exact node bound when row_cap is 0), the row pitch of
+// the longest layer, DP rows capped at row_cap (0: exact, node bound + 1)
+// with a pool of kPoaHPoolRows global score rows (0: one per DP row),
 // and the deepest ring that fits lds_limit, for the default or the wide
 // kernel. False when even a ring of 0 rows does not fit (the layer bytes
 // alone exceed the limit).

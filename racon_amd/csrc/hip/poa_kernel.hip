@@ -90,6 +90,11 @@ constexpr uint32_t kCodePayload = 0x1fff;
 // bits 2 (j - 1) % 8 of byte (j - 1) / 4 of the row's code memory): the
 // predecessor is implied, and column 0 is always vertical.
 constexpr uint32_t kKindHorz = 2;
+// Bits 16..22 of a row's meta word: the row's entry in the slot's pool of
+// global score rows (a row that found no LDS slot, in a pass with a pool).
+constexpr int kRowMetaGShift = 16;
+constexpr int kRowMetaGMask = 127;
+static_assert(kPoaHPoolRows <= kRowMetaGMask + 1, "pool entries fit the meta bits");
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
@@ -148,6 +153,16 @@ struct Slot {
   __device__ int32_t* pred_rows() const { return (int32_t*)(p + L->pred_rows); }
   __device__ int32_t* row_meta() const { return (int32_t*)(p + L->row_meta); }
   __device__ int32_t* H() const { return (int32_t*)(p + L->H); }
+  // global score row of DP row `row` (meta: its row meta word): its pool
+  // entry when the slot keeps a pool, else row `row` of the matrix
+  __device__ int32_t* H_row(int row, int meta, int pitch) const {
+    const int64_t at = L->hpool != 0 ? ((meta >> kRowMetaGShift) & kRowMetaGMask) : row;
+    return H() + at * pitch;
+  }
+  __device__ int32_t* H_row(int row, int pitch) const {
+    return L->hpool != 0 ? H_row(row, __builtin_amdgcn_readfirstlane(row_meta()[row - 1]), pitch)
+                         : H() + static_cast<int64_t>(row) * pitch;
+  }
   __device__ uint16_t* code() const { return (uint16_t*)(p + L->code); }
   __device__ int64_t* scores() const { return (int64_t*)(p + L->scores); }
   __device__ int32_t* preds() const { return (int32_t*)(p + L->preds); }
@@ -484,7 +499,7 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
     const uint32_t b = static_cast<uint32_t>(meta & 0xff);
     const int own = (meta >> kRowMetaSlotShift) & 31;
     T* Hr = ring + (own < kSlotGlobal ? own : 0) * pitch;
-    int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+    int32_t* Hg = s.H_row(r, meta, pitch);
     uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
 
     // column 0: max over predecessors of their column 0, plus gap
@@ -500,7 +515,7 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
       } else if (sl < kSlotGlobal) {
         h0 = ring[sl * pitch + kColShift];
       } else {
-        h0 = uniform(s.H()[static_cast<int64_t>(p) * pitch]);
+        h0 = uniform(s.H_row(p, pitch)[0]);
         wait_stores();
       }
       if (h0 + gap > c0) {
@@ -549,7 +564,7 @@ __device__ __forceinline__ int dp_layer(CParams& P, const Slot& s, uint8_t* lds,
           hp[0] = row[jr - 1];
           Quad<T>::load(row + jr, hp + 1);
         } else {
-          const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+          const int32_t* row = s.H_row(p, pitch);
 #pragma unroll
           for (int u = 0; u <= kStrip; ++u) {
             const int j = j0 - 1 + u;
@@ -673,7 +688,7 @@ __device__ __forceinline__ void load_strip(const int16_t* ring, int pitch, const
     const int32_t prev = static_cast<int32_t>(from_prev_lane(static_cast<uint32_t>(hp[W])));
     hp[0] = lane_id() == 0 ? c0 : prev;
   } else {
-    const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+    const int32_t* row = s.H_row(p, pitch);
 #pragma unroll
     for (int u = 0; u <= W; ++u) {
       const int j = j0 - 1 + u;
@@ -851,7 +866,7 @@ __device__ __forceinline__ uint32_t load_pairs(const int16_t* ring, int pitch, c
     const uint32_t prev = from_prev_lane(pp[W / 2 - 1]) >> 16;
     return lane_id() == 0 ? c0 : prev;
   } else {
-    const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+    const int32_t* row = s.H_row(p, pitch);
     int32_t hv[W + 1];
 #pragma unroll
     for (int u = 0; u <= W; ++u) {
@@ -1122,7 +1137,7 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
       auto pred_at = [&](int k) -> int { return pred_of(off, k); };
       const uint32_t b = static_cast<uint32_t>(meta & 0xff);
       const int own = (meta >> kRowMetaSlotShift) & 31;
-      int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+      int32_t* Hg = s.H_row(r, meta, pitch);
       uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
       // match score of strip column u against the row's base
       auto sc = [&](int u) -> int32_t {
@@ -1165,7 +1180,7 @@ __device__ __forceinline__ int dp_layer_strip(CParams& P, const Slot& s, uint8_t
                                                          own, pitch, ring, Hg, Cr, end_lane,
                                                          end_u);
               h_last = packed_store<W, false>(ob, 0u, pay2, lane, j0, jq, qs, live, L, gap, own2, pitch,
-                                              ring, s.H() + static_cast<int64_t>(r + 1) * pitch,
+                                              ring, s.H_row(r + 1, meta2, pitch),
                                               s.code() + static_cast<int64_t>(r + 1) * pitch,
                                               end_lane, end_u);
               account(h_a, r, meta);
@@ -1316,7 +1331,7 @@ __device__ __forceinline__ uint32_t load_band(const int16_t* ring, int pitch, co
     return lane == 0 ? c0 : prev;
   }
   // a row kept in the slot's global matrix (int32, absolute columns)
-  const int32_t* row = s.H() + static_cast<int64_t>(p) * pitch;
+  const int32_t* row = s.H_row(p, pitch);
   int32_t hv[kBandW + 1];
 #pragma unroll
   for (int u = 0; u <= kBandW; ++u) {
@@ -1455,7 +1470,7 @@ __device__ __forceinline__ int dp_layer_band(CParams& P, const Slot& s, uint8_t*
       ps.base[1] = __builtin_amdgcn_perm(sw, sw, 0x0c030c02u);
       const uint32_t b = static_cast<uint32_t>(meta & 0xff);
       const int own = (meta >> kRowMetaSlotShift) & 31;
-      int32_t* Hg = s.H() + static_cast<int64_t>(r) * pitch;
+      int32_t* Hg = s.H_row(r, meta, pitch);
       uint16_t* Cr = s.code() + static_cast<int64_t>(r) * pitch;
       int32_t h_last;
       if (np == 1) {
@@ -2001,7 +2016,8 @@ __device__ __forceinline__ int32_t wave_prefix_sum(int32_t x) {
 // and an LDS slot for every row that a later row reads: the lowest slot whose
 // previous row is no longer needed (lane k < n_slots holds the last row that
 // reads slot k). With kFull every node is in the layer's subgraph and the
-// membership checks are skipped. Returns the largest predecessor count.
+// membership checks are skipped. Returns the largest predecessor count, or
+// -1 when a row needs a global score row and the slot's pool has none free.
 // Each lane takes kU rows per round (64 kU rows) and walks their edge lists
 // in lockstep, one edge of every list per step, so that the loads of the
 // kU lists are in flight together; the walk keeps the first kP predecessor
@@ -2045,6 +2061,12 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   int run = 0;
   int max_np = 0;
   int busy = lane < n_slots ? 0 : 0x7fffffff;
+  // pooled global score rows: lane k holds the last reader of entry k
+  // (pool0) and of entry 64 + k (pool1); entries past the pool never free
+  const int hpool = uniform(static_cast<int>(s.L->hpool));
+  int pool0 = lane < hpool ? 0 : 0x7fffffff;
+  int pool1 = lane + kWave < hpool ? 0 : 0x7fffffff;
+  bool pool_full = false;
   for (int r0 = 0; r0 < V; r0 += kU * kWave) {
     int v[kU], ei[kU], eo[kU], np[kU], oc[kU], lu[kU], meta[kU], off[kU];
     int pr[kU][kP];
@@ -2116,6 +2138,7 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
       const int c0 = r0 + i * kWave;
       if (c0 >= V) break;
       int my_slot = kSlotNone;
+      int my_pool = 0;
       const int n_chunk = uniform(min(kWave, V - c0));
       for (int rr = 0; rr < n_chunk; ++rr) {
         const int l = __builtin_amdgcn_readlane(lu[i], rr);
@@ -2128,6 +2151,24 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
           const int sl = none ? kSlotGlobal : __ffsll(static_cast<long long>(free_mask)) - 1;
           busy = lane_write(none ? 0x7fffffff : l, none ? kWave - 1 : sl, busy);
           my_slot = lane_write(sl, rr, my_slot);
+          if (none && hpool != 0) {
+            // the lowest free pool entry (lane k of pool0 holds entry k, of
+            // pool1 entry 64 + k); none free: the window reruns exactly
+            const unsigned long long g0 = __ballot(pool0 < row);
+            const unsigned long long g1 = __ballot(pool1 < row);
+            int g = 0;
+            if (g0 != 0) {
+              g = __ffsll(static_cast<long long>(g0)) - 1;
+              pool0 = lane_write(l, g, pool0);
+            } else if (g1 != 0) {
+              const int t = __ffsll(static_cast<long long>(g1)) - 1;
+              pool1 = lane_write(l, t, pool1);
+              g = kWave + t;
+            } else {
+              pool_full = true;
+            }
+            my_pool = lane_write(g, rr, my_pool);
+          }
         }
       }
       const int r = c0 + lane;
@@ -2135,7 +2176,7 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
       const int x = wave_prefix_sum(cnt);
       off[i] = run + x - cnt;
       if (r < V) {
-        row_meta[r] = meta[i] | (my_slot << kRowMetaSlotShift);
+        row_meta[r] = meta[i] | (my_slot << kRowMetaSlotShift) | (my_pool << kRowMetaGShift);
         pred_off[r] = off[i];
         if (slots_in_lds) lslot[r] = static_cast<uint8_t>(my_slot);
       }
@@ -2180,6 +2221,7 @@ __device__ __noinline__ int build_rows(GByte* slot_base, CLayout* layout, int V,
   if (lane == 0) {
     pred_off[V] = run;
   }
+  if (pool_full) return -1;
   return __builtin_amdgcn_readlane(wave_prefix_max(max_np), kWave - 1);
 }
 
@@ -2544,7 +2586,10 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
   s.n_edges = n > 0 ? n - 1 : 0;
   s.n_seqs = 1;
   // visit marks of the tie resolution: (layer, lane) tags, none set yet
-  for (int v = lane; v < static_cast<int>(W.node_bound); v += kWave) {
+  // (node ids stay below the slot's node capacity, which may be below the
+  // window's node bound)
+  const int n_marks = min(static_cast<int>(W.node_bound), static_cast<int>(caps.nodes));
+  for (int v = lane; v < n_marks; v += kWave) {
     s.tie_mark()[v] = 0;
   }
   wave_sync();
@@ -2672,8 +2717,12 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
     const int max_np =
         uniform(full ? build_rows<true>(s.p, s.L, V, n_slots, rows_lds, ring_bytes)
                      : build_rows<false>(s.p, s.L, V, n_slots, rows_lds, ring_bytes));
-    strip_ok = narrow && L >= 1 && max_abs <= 4096 && max_np <= 255;
     wave_sync();
+    if (max_np < 0) {
+      R.status = kPoaRowOverflow;  // the pooled global rows ran out: rerun exactly
+      return;
+    }
+    strip_ok = narrow && L >= 1 && max_abs <= 4096 && max_np <= 255;
     if (max_np > static_cast<int>(kCodePayload) + 1) {
       // a move code holds the in-edge index in 13 bits
       R.status = kPoaInDegree;

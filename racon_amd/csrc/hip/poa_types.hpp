@@ -82,7 +82,13 @@ struct PoaSlotCaps {
   uint32_t cols;    // DP row pitch in elements (>= max layer length + 1, padded)
   uint32_t stack;   // DFS stack entries
   uint32_t aligned; // aligned-set entries per node (the kernel variant's)
+  uint32_t hrows;   // pooled global score rows (0: one per DP row)
 };
+
+// Pooled global score rows of a main pass: a DP row that finds no LDS slot
+// takes one of these (a window that needs more reruns in exact slots, which
+// keep one global row per DP row).
+constexpr uint32_t kPoaHPoolRows = 128;
 
 // Aligned-set entries kept per node: a column of the graph holds at most
 // this many + 1 distinct bytes. The default kernel keeps kMaxAligned; the
@@ -102,6 +108,7 @@ struct PoaSlotLayout {
       node_rank, mark, lead, stack, in_sub, row_of, sub_order, pred_off, pred_rows,
       row_meta, H, code, scores, preds, path, seq, qual, anchor, lpath,
       sink_score, tie_mark, row_pred4, total;
+  uint64_t hpool;  // caps.hrows
 };
 
 constexpr uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -140,7 +147,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.pred_off = take(4 * (n + 1));
   l.pred_rows = take(4 * (e + n));
   l.row_meta = take(4 * n);
-  l.H = take(4 * uint64_t(c.rows) * c.cols);
+  l.H = take(4 * uint64_t(c.hrows != 0 ? c.hrows : c.rows) * c.cols);
   l.code = take(2 * uint64_t(c.rows) * c.cols);
   l.scores = take(8 * n);
   l.preds = take(4 * n);
@@ -153,6 +160,7 @@ inline PoaSlotLayout poa_slot_layout(const PoaSlotCaps& c) {
   l.tie_mark = take(4 * n);
   l.row_pred4 = take(8 * uint64_t(c.rows));
   l.total = o;
+  l.hpool = c.hrows;
   return l;
 }
 

@@ -83,6 +83,12 @@ uint32_t priority_windows(const PoaWindow* host, const std::vector<uint32_t>& or
 
 }  // namespace
 
+uint64_t graph_estimate(const PoaWindow& w) {
+  const uint64_t layer_bytes = w.node_bound - w.backbone_len;
+  return std::min<uint64_t>(
+      w.node_bound, w.backbone_len + std::max<uint64_t>(2ull * w.max_layer_len, layer_bytes / 4));
+}
+
 uint64_t window_cost(const PoaWindow& w) {
   return static_cast<uint64_t>(w.node_bound) * (w.max_layer_len + 1);
 }
@@ -330,14 +336,22 @@ bool plan_pass(const PoaWindow* host, const std::vector<uint32_t>& ws, uint32_t 
   PoaSlotCaps caps{};
   uint32_t max_nodes = 1, max_edges = 1, max_cols = 1;
   for (uint32_t w : ws) {
-    max_nodes = std::max(max_nodes, host[w].node_bound);
-    max_edges = std::max(max_edges, host[w].edge_bound);
+    uint32_t nodes = host[w].node_bound, edges = host[w].edge_bound;
+    if (row_cap != 0) {
+      const uint64_t est = graph_estimate(host[w]);
+      nodes = static_cast<uint32_t>(std::min<uint64_t>(nodes, est));
+      edges = static_cast<uint32_t>(
+          std::min<uint64_t>(edges, 2 * est + host[w].edge_bound - host[w].node_bound));
+    }
+    max_nodes = std::max(max_nodes, nodes);
+    max_edges = std::max(max_edges, edges);
     max_cols = std::max(max_cols, host[w].max_layer_len + 1);
   }
   caps.nodes = max_nodes;
   caps.edges = max_edges;
   caps.cols = static_cast<uint32_t>(align_up(max_cols - 1 + kPoaColPad, 16));
   caps.rows = row_cap == 0 ? max_nodes + 1 : std::min<uint32_t>(max_nodes + 1, row_cap);
+  caps.hrows = row_cap == 0 ? 0 : std::min<uint32_t>(caps.rows, kPoaHPoolRows);
   caps.aligned = wide ? kMaxAlignedWide : kMaxAligned;
   caps.stack = 2 * (max_edges + (caps.aligned + 1) * max_nodes) + 64;
   uint32_t ring = ring_rows_for(caps.cols, tune_ring);

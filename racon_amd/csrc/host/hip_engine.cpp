@@ -3,6 +3,8 @@
 // CUDAPolisher batch objects (src/cuda/cudapolisher.cpp:74-214, 216-418),
 // except that every window and overlap runs on the device: there is no CPU
 // fallback, and a device failure ends the run with an error.
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -172,6 +174,7 @@ class HipEngine : public PolishEngine {
     };
     Slot slots[2];
     const uint32_t n_batches = chunks.size() > 1 ? 2 : 1;
+    const auto t_create = std::chrono::steady_clock::now();
     for (uint32_t k = 0; k < n_batches; ++k) {
       int32_t rc = rh_poa_create(0, config_.match, config_.mismatch, config_.gap, trim ? 1 : 0,
                                  &slots[k].batch);
@@ -180,6 +183,11 @@ class HipEngine : public PolishEngine {
       rc = rh_poa_set_band(slots[k].batch, config_.banded ? 256 : 0);
       if (rc != RH_OK) die("rh_poa_set_band", rc);
     }
+    const double create_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_create)
+            .count();
+    rh_poa_stats total{};  // summed over the chunks
+    uint64_t device_bytes = 0;
     auto upload = [&](uint64_t c) {
       Slot& sl = slots[c % n_batches];
       const uint64_t first = chunks[c].first, last = chunks[c].second;
@@ -230,6 +238,16 @@ class HipEngine : public PolishEngine {
       rc = rh_poa_download(batch, out.data(), out.size(), offsets.data(), lengths.data(),
                            codes.data());
       if (rc != RH_OK) die("rh_poa_download", rc);
+      device_bytes = std::max<uint64_t>(device_bytes, rh_poa_device_bytes(batch));
+      rh_poa_stats st{};
+      if (rh_poa_get_stats(batch, &st) == RH_OK) {
+        total.pack_ms += st.pack_ms;
+        total.h2d_ms += st.h2d_ms;
+        total.kernel_ms += st.kernel_ms;
+        total.d2h_ms += st.d2h_ms;
+        total.windows += st.windows;
+        total.retried += st.retried;
+      }
       for (uint64_t i = first; i < last; ++i) {
         const uint64_t k = i - first;
         take_result(windows, i, out.data() + offsets[k], lengths[k], codes[k], status);
@@ -238,6 +256,15 @@ class HipEngine : public PolishEngine {
     }
     if (ready && chunks.empty()) ready(0);
     for (uint32_t k = 0; k < n_batches; ++k) rh_poa_destroy(slots[k].batch);
+    // where the consensus phase went (the next chunk's upload overlaps the
+    // kernels of the current one, so the parts can sum to more than the phase)
+    fprintf(stderr,
+            "[racon_amd::consensus] %lu windows in %zu chunks: batch setup %.1f ms, pack %.1f ms, "
+            "H2D %.1f ms, kernels %.1f ms, D2H %.1f ms, %lu reruns, %.1f GB of device memory per "
+            "batch\n",
+            static_cast<unsigned long>(total.windows), chunks.size(), create_ms, total.pack_ms,
+            total.h2d_ms, total.kernel_ms, total.d2h_ms, static_cast<unsigned long>(total.retried),
+            device_bytes / 1e9);
   }
 
  private:

@@ -194,3 +194,40 @@ def test_reused_batch_stages_uploads_in_pinned_memory():
     finally:
         b.close()
     assert again == first
+
+
+def test_graph_beyond_the_size_estimate_reruns_exactly():
+    # main passes size graph arrays for backbone + max(2 x longest layer,
+    # layer bytes / 4) nodes; eight random layers six times as long as the
+    # backbone grow the graph to about 1.3 times that (the oracle: 800-850
+    # nodes against 650), so the window ends kPoaNodeOverflow and reruns in
+    # exact slots
+    from synth import random_seq
+    rng = np.random.default_rng(90)
+    wins = []
+    for _ in range(3):
+        backbone = random_seq(rng, 50)
+        layers = [_lib.Layer(random_seq(rng, 300), None, 0, 49) for _ in range(8)]
+        wins.append(_lib.Window(backbone, b"!" * 50, layers, True))
+    out, status, st = run_gpu(wins, (3, -5, -4))
+    assert st.retried >= 1
+    for i, w in enumerate(wins):
+        cons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
+        assert (out[i], status[i]) == (cons, ost), f"window {i}"
+
+
+def test_deep_windows_through_the_score_row_pool():
+    # with a 2-row LDS ring nearly every DP row that a later row reads takes
+    # an entry of the slot's pool of global score rows, and deep windows reuse
+    # each entry many times per layer (an exhausted pool ends
+    # kPoaRowOverflow and reruns exactly, like a row cap that is too small)
+    wins = make_windows(92, 6, length=500, depth=60)
+    b = _lib.PoaBatch(3, -5, -4, trim=True)
+    b.set_tuning(ring_rows=2)
+    b.upload(wins)
+    b.run()
+    out, status = b.download()
+    b.close()
+    for i, w in enumerate(wins):
+        cons, ost, _, _ = oracle_py.window_consensus(*oracle_args(w), True, 3, -5, -4)
+        assert (out[i], status[i]) == (cons, ost), f"window {i}"
