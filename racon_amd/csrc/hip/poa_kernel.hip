@@ -1372,8 +1372,6 @@ __device__ __forceinline__ int32_t band_row(const PackedStrip<kBandW>& ps, const
     pre[q] = carry;
   }
   const int32_t excl = wave_shift_right(wave_prefix_max(static_cast<int32_t>(carry) >> 16), -32768);
-  const uint32_t k1 = pk_splat(0x4000 - static_cast<int32_t>(pay0));
-  const uint32_t pay = pk_splat(static_cast<int32_t>(pay0));
 #pragma unroll
   for (int q = 0; q < H2; ++q) {
     h[q] = pk_max(pk_max_lo(pre[q], static_cast<uint32_t>(excl)), neg);
@@ -1387,11 +1385,17 @@ __device__ __forceinline__ int32_t band_row(const PackedStrip<kBandW>& ps, const
       const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
       cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
     } else {
-      cd[q] = pk_mad(fd, pk_mad(fv, k1, ps.c4000), pay);
+      cd[q] = pk_mad(fd, fv, fd);  // the move kind, as in packed_compute
     }
   }
   const int j0 = lo + 1 + kBandW * lane;
-  *reinterpret_cast<uint2*>(Cr + (j0 - 1)) = make_uint2(cd[0], cd[1]);
+  if (kMulti) {
+    *reinterpret_cast<uint2*>(Cr + (j0 - 1)) = make_uint2(cd[0], cd[1]);
+  } else {
+    // 2-bit kinds of the lane's 4 columns: byte (j0 - 1) / 4 of the row
+    const uint32_t x = cd[0] | (cd[1] << 4);
+    reinterpret_cast<uint8_t*>(Cr)[(j0 - 1) >> 2] = static_cast<uint8_t>(x | (x >> 14));
+  }
   if (own < kSlotGlobal) {
     *reinterpret_cast<uint2*>(ring + own * pitch + 4 * lane) = make_uint2(h[0], h[1]);
   } else if (own == kSlotGlobal) {
@@ -1404,9 +1408,8 @@ __device__ __forceinline__ int32_t band_row(const PackedStrip<kBandW>& ps, const
     const int16_t c0 = static_cast<int16_t>(lo == 0 ? h0 : kBandNeg);
     if (own < kSlotGlobal) ring[own * pitch + pitch - 1] = c0;
     if (own == kSlotGlobal) Hg[0] = c0;
-    if (lo == 0) {
-      Cr[pitch - 1] = static_cast<uint16_t>(kMulti ? (kCodeVert | kCodeViaK | (k0 & 0xffffu))
-                                                   : (kCodeVert | pay0));
+    if (kMulti && lo == 0) {
+      Cr[pitch - 1] = static_cast<uint16_t>(kCodeVert | kCodeViaK | (k0 & 0xffffu));
     }
   }
   const int e = L - 1 - lo;  // column L in the band: lane e / 4, column e % 4
@@ -2826,8 +2829,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
       const int32_t* pred_rows = s.pred_rows();
       const uint64_t* row_pred4 = s.row_pred4();
       const int32_t* row_meta = s.row_meta();
-      // the packed DP (not banded) keeps move kinds for one-predecessor rows
-      const bool kinds = packed && !banded && L <= 16 * kWave;
+      // the packed DP and the band rows keep move kinds for one-predecessor rows
+      const bool kinds = banded || (packed && L <= 16 * kWave);
       int32_t* pos_node = s.path();
       constexpr uint32_t kNextExit = 0xff;  // successor: the decoder takes the move
       constexpr uint32_t kDiagBit = 0x100;
