@@ -53,3 +53,16 @@ def test_cli_matches_oracle(name, reads, overlaps, targets, args, data_dir):
     cpu = run(ORACLE_CLI, args, data_dir, files)
     print(f"{name}: gpu {len(gpu)} bytes, oracle {len(cpu)} bytes", flush=True)
     assert gpu == cpu
+
+
+def test_fragment_fasta_records_the_known_one_bp_difference(data_dir):
+    # racon pins FragmentCorrectionWithoutQualitiesFull at 236 reads and
+    # 1,663,982 bp (test/racon_test.cpp:256-272); the oracle, and so the GPU
+    # path, produce 1,663,983 bp (an equal-weight tie inside spoa resolved
+    # the other way; tests/test_racon_pins.py holds the xfail). This records
+    # the current value, so any change to it shows up here.
+    files = ("sample_reads.fasta.gz", "sample_ava_overlaps.paf.gz", "sample_reads.fasta.gz")
+    out = run(_lib.RACON_BIN, ["-m", "1", "-x", "-1", "-g", "-1", "-w", "500", "-f", "-u"],
+              data_dir, files).decode().split("\n")
+    seqs = [out[i + 1] for i in range(0, len(out) - 1, 2)]
+    assert (len(seqs), sum(len(s) for s in seqs)) == (236, 1663983)
