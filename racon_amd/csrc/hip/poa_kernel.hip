@@ -86,10 +86,12 @@ constexpr uint16_t kCodeHorz = 0x8000;
 constexpr uint16_t kCodeViaK = 0x2000;
 constexpr uint32_t kCodePayload = 0x1fff;
 // Rows with one predecessor in the packed DP store only the move of each
-// cell, as a 2-bit kind (0 diagonal, 1 vertical, 2 horizontal; column j at
-// bits 2 (j - 1) % 8 of byte (j - 1) / 4 of the row's code memory): the
+// cell, as a 2-bit kind 2 fd + fv from the flags fd (not diagonal) and fv
+// (not vertical): 0 or 1 diagonal, 2 vertical, 3 horizontal (column j at
+// bits 2 (j - 1) % 8 of byte (j - 1) / 4 of the row's code memory). The
 // predecessor is implied, and column 0 is always vertical.
-constexpr uint32_t kKindHorz = 2;
+constexpr uint32_t kKindVert = 2;
+constexpr uint32_t kKindHorz = 3;
 // Bits 16..22 of a row's meta word: the row's entry in the slot's pool of
 // global score rows (a row that found no LDS slot, in a pass with a pool).
 constexpr int kRowMetaGShift = 16;
@@ -930,8 +932,8 @@ __device__ __forceinline__ void packed_compute(const PackedStrip<W>& ps, const u
       const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
       o.cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
     } else {
-      // the move kind: fd * (1 + fv)
-      o.cd[q] = pk_mad(fd, fv, fd);
+      // the move kind 2 fd + fv (both halves at once: no carries)
+      o.cd[q] = (fd << 1) | fv;
     }
     // computed here, not sunk into the store branch, so that a second row's
     // scan can interleave with this row's codes
@@ -1385,7 +1387,7 @@ __device__ __forceinline__ int32_t band_row(const PackedStrip<kBandW>& ps, const
       const uint32_t cv = pk_mad(fv, pk_sub(0x80008000u, vc), vc);
       cd[q] = pk_mad(fd, pk_sub(cv, dc), dc);
     } else {
-      cd[q] = pk_mad(fd, fv, fd);  // the move kind, as in packed_compute
+      cd[q] = (fd << 1) | fv;  // the move kind, as in packed_compute
     }
   }
   const int j0 = lo + 1 + kBandW * lane;
@@ -2859,7 +2861,8 @@ __device__ __forceinline__ void process_window(CParams& P, Slot& s, uint8_t* lds
           auto kind_code = [&](int j) -> uint32_t {
             if (j <= 0) return kCodeVert | kCodeViaK;
             const uint32_t k = (static_cast<uint32_t>(krow[(j - 1) >> 2]) >> (2 * ((j - 1) & 3))) & 3u;
-            return k == kKindHorz ? kCodeHorz : ((k << 14) | kCodeViaK);
+            return k == kKindHorz ? kCodeHorz
+                                  : ((k == kKindVert ? kCodeVert : kCodeDiag) | kCodeViaK);
           };
           c0 = kind_code(cj);
           c1 = kind_code(cj - 1);
