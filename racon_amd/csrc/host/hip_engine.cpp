@@ -92,54 +92,105 @@ class HipEngine : public PolishEngine {
   bool align_breaking_points(
       const std::vector<AlignTask>& tasks, uint32_t window_length,
       const std::function<void(uint64_t, const uint32_t*, uint64_t)>& done) override {
-    rh_align_batch* batch = nullptr;
-    int32_t rc = rh_align_create(0, &batch);
-    if (rc != RH_OK) die("rh_align_create", rc);
-    rc = rh_align_set_band(batch, config_.aligner_band_width);
-    if (rc != RH_OK) die("rh_align_set_band", rc);
+    // chunks of about 1 GiB of sequence on two batch objects: chunk c + 1 is
+    // packed and copied while chunk c's kernel runs
     const uint64_t kChunkBytes = 1ull << 30;
-    uint64_t first = 0;
-    std::vector<rh_pair> pairs;
-    std::vector<rh_break_span> spans;
+    std::vector<std::pair<uint64_t, uint64_t>> chunks;
+    for (uint64_t first = 0; first < tasks.size();) {
+      uint64_t last = first, bytes = 0;
+      while (last < tasks.size() && (last == first || bytes < kChunkBytes)) {
+        bytes += tasks[last].query_length + tasks[last].target_length;
+        ++last;
+      }
+      chunks.emplace_back(first, last);
+      first = last;
+    }
+    struct Slot {
+      rh_align_batch* batch = nullptr;
+      std::vector<rh_pair> pairs;
+      std::vector<rh_break_span> spans;
+    };
+    Slot slots[2];
+    const uint32_t n_batches = chunks.size() > 1 ? 2 : 1;
+    for (uint32_t k = 0; k < n_batches; ++k) {
+      int32_t rc = rh_align_create(0, &slots[k].batch);
+      if (rc != RH_OK) die("rh_align_create", rc);
+      rc = rh_align_set_band(slots[k].batch, config_.aligner_band_width);
+      if (rc != RH_OK) die("rh_align_set_band", rc);
+    }
+    auto upload = [&](uint64_t c) {
+      Slot& sl = slots[c % n_batches];
+      sl.pairs.clear();
+      sl.spans.clear();
+      for (uint64_t i = chunks[c].first; i < chunks[c].second; ++i) {
+        const AlignTask& t = tasks[i];
+        sl.pairs.push_back(rh_pair{t.query, t.query_length, t.target, t.target_length});
+        sl.spans.push_back(rh_break_span{t.t_begin, t.t_end, t.q_first});
+      }
+      return rh_align_upload(sl.batch, static_cast<uint32_t>(sl.pairs.size()), sl.pairs.data());
+    };
     std::vector<uint64_t> offsets;
     std::vector<uint32_t> counts;
     std::vector<uint32_t> points;
-    while (first < tasks.size()) {
-      uint64_t last = first, bytes = 0;
-      pairs.clear();
-      spans.clear();
-      while (last < tasks.size() && (last == first || bytes < kChunkBytes)) {
-        const AlignTask& t = tasks[last];
-        pairs.push_back(rh_pair{t.query, t.query_length, t.target, t.target_length});
-        spans.push_back(rh_break_span{t.t_begin, t.t_end, t.q_first});
-        bytes += t.query_length + t.target_length;
-        ++last;
+    rh_align_stats total{};  // summed over the chunks
+    double points_ms = 0;
+    int32_t rc = chunks.empty() ? RH_OK : upload(0);
+    if (rc != RH_OK) die("rh_align_upload", rc);
+    // the kernel of a chunk runs on a helper thread (rh_align_run blocks):
+    // chunk c + 1 is uploaded while chunk c's kernel runs, and its kernel
+    // starts before chunk c's breaking points and callbacks
+    int32_t run_rc = RH_OK;
+    std::thread runner;
+    auto start = [&](uint64_t c) {
+      rh_align_batch* b = slots[c % n_batches].batch;
+      runner = std::thread([&run_rc, b] { run_rc = rh_align_run(b); });
+    };
+    if (!chunks.empty()) start(0);
+    for (uint64_t c = 0; c < chunks.size(); ++c) {
+      Slot& sl = slots[c % n_batches];
+      int32_t up_rc = RH_OK;
+      if (c + 1 < chunks.size()) up_rc = upload(c + 1);
+      runner.join();
+      if (run_rc != RH_OK) die("rh_align_run", run_rc);
+      if (up_rc != RH_OK) die("rh_align_upload", up_rc);
+      if (c + 1 < chunks.size()) start(c + 1);
+      rh_align_stats st{};
+      if (rh_align_get_stats(sl.batch, &st) == RH_OK) {
+        total.pack_ms += st.pack_ms;
+        total.h2d_ms += st.h2d_ms;
+        total.kernel_ms += st.kernel_ms;
+        total.pairs += st.pairs;
       }
-      const uint32_t n = static_cast<uint32_t>(pairs.size());
-      rc = rh_align_upload(batch, n, pairs.data());
-      if (rc != RH_OK) die("rh_align_upload", rc);
-      rc = rh_align_run(batch);
-      if (rc != RH_OK) die("rh_align_run", rc);
-      points.resize(2 * rh_breaking_points_bound(window_length, spans.data(), n));
+      const auto t_points = std::chrono::steady_clock::now();
+      const uint32_t n = static_cast<uint32_t>(sl.pairs.size());
+      points.resize(2 * rh_breaking_points_bound(window_length, sl.spans.data(), n));
       offsets.resize(n);
       counts.resize(n);
-      rc = rh_align_breaking_points(batch, window_length, spans.data(), offsets.data(),
+      rc = rh_align_breaking_points(sl.batch, window_length, sl.spans.data(), offsets.data(),
                                     counts.data(), points.data(), points.size() / 2);
       if (rc != RH_OK) die("rh_align_breaking_points", rc);
+      points_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                             t_points)
+                       .count();
+      const uint64_t first = chunks[c].first;
       for (uint32_t i = 0; i < n; ++i) {
         if (counts[i] == RH_PAIR_REJECTED) {
           fprintf(stderr,
                   "[racon::HipEngine] error: overlap %lu (%u x %u) could not be aligned on the "
                   "device; no CPU path in this build\n",
-                  static_cast<unsigned long>(first + i), pairs[i].query_length,
-                  pairs[i].target_length);
+                  static_cast<unsigned long>(first + i), sl.pairs[i].query_length,
+                  sl.pairs[i].target_length);
           exit(1);
         }
         done(first + i, points.data() + 2 * offsets[i], counts[i]);
       }
-      first = last;
     }
-    rh_align_destroy(batch);
+    for (uint32_t k = 0; k < n_batches; ++k) rh_align_destroy(slots[k].batch);
+    fprintf(stderr,
+            "[racon_amd::align] %lu overlaps in %zu chunks: pack %.1f ms, H2D %.1f ms, kernels "
+            "%.1f ms, breaking points (kernel + copy) %.1f ms\n",
+            static_cast<unsigned long>(total.pairs), chunks.size(), total.pack_ms, total.h2d_ms,
+            total.kernel_ms, points_ms);
     return true;
   }
 
@@ -219,16 +270,25 @@ class HipEngine : public PolishEngine {
     std::vector<uint8_t> codes;
     int32_t rc = chunks.empty() ? RH_OK : upload(0);
     if (rc != RH_OK) die("rh_poa_upload", rc);
+    if (!chunks.empty()) {
+      rc = rh_poa_submit(slots[0].batch, RH_POA_BATCHED);
+      if (rc != RH_OK) die("rh_poa_submit", rc);
+    }
     for (uint64_t c = 0; c < chunks.size(); ++c) {
       rh_poa_batch* batch = slots[c % n_batches].batch;
-      // the kernel of chunk c on a helper thread, the next upload here
-      int32_t run_rc = RH_OK;
-      std::thread runner([&] { run_rc = rh_poa_run(batch, RH_POA_BATCHED); });
-      int32_t up_rc = RH_OK;
-      if (c + 1 < chunks.size()) up_rc = upload(c + 1);
-      runner.join();
-      if (run_rc != RH_OK) die("rh_poa_run", run_rc);
-      if (up_rc != RH_OK) die("rh_poa_upload", up_rc);
+      // chunk c's kernels run while the next chunk is packed and copied
+      // here; chunk c + 1's kernels are queued before chunk c's results are
+      // downloaded and handed out, so the device does not wait for the host
+      if (c + 1 < chunks.size()) {
+        rc = upload(c + 1);
+        if (rc != RH_OK) die("rh_poa_upload", rc);
+      }
+      rc = rh_poa_wait(batch);
+      if (rc != RH_OK) die("rh_poa_wait", rc);
+      if (c + 1 < chunks.size()) {
+        rc = rh_poa_submit(slots[(c + 1) % n_batches].batch, RH_POA_BATCHED);
+        if (rc != RH_OK) die("rh_poa_submit", rc);
+      }
       const uint64_t first = chunks[c].first, last = chunks[c].second;
       const uint64_t n = last - first;
       out.resize(rh_poa_output_bound(batch));
