@@ -10,9 +10,11 @@
 // plus the reverse score of the rest equals the subproblem score (then x = -1,
 // then x = |q|-1), and the two halves are solved in order.
 //
-// Every score here is exact (no band), which gives the same path as edlib's
-// banded computation: banding only drops cells whose score cannot be on an
-// optimal path, and every choice above reads cells on optimal paths.
+// Scores are computed in a diagonal band (Band below) that starts from a
+// guess of the distance and doubles until the banded distance fits it; every
+// score on an optimal path is then exact, which gives edlib's path: banding
+// only drops cells whose score cannot be on an optimal path, and every choice
+// above reads cells on optimal paths.
 //
 // Myers' bit-vector recurrence runs on 64-row blocks, one block per lane, in
 // an anti-diagonal pipeline: at step s lane b advances column s - b and hands
