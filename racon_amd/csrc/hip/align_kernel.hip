@@ -690,89 +690,192 @@ hipError_t launch_align_batch_cleared(const AlignParams& params, uint32_t grid,
   return launch_align_batch(params, grid, stream);
 }
 
+// Inclusive scans over the 64 lanes (lane shuffles; small kernels only).
+__device__ __forceinline__ int32_t lane_scan_add(int32_t x) {
+  const int lane = lane_id() & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ int32_t lane_scan_max(int32_t x) {
+  const int lane = lane_id() & (kWave - 1);
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t y = __shfl_up(x, d, kWave);
+    if (lane >= d) x = max(x, y);
+  }
+  return x;
+}
+
 // Breaking points of every aligned pair (racon overlap.cpp:226-292, the walk
 // over edlib's CIGAR), from the edit operations the batch kernel left in
-// device memory: one lane per pair. A match or mismatch advances target and
-// query, an insertion the query, a deletion the target; the first match of a
-// window and the position after its last match are emitted as a pair when
-// the target position reaches the window's last base (k * W - 1 for the
-// multiples of W inside (t_begin, t_end), then t_end - 1). counts[i] is the
-// number of points (2 per window with a match), or 0xffffffff when they
-// would outgrow the pair's room.
+// device memory. A match or mismatch advances target and query, an insertion
+// the query, a deletion the target; the first match of a window and the
+// position after its last match are emitted as a pair when the target
+// position reaches the window's last base (k * W - 1 for the multiples of W
+// inside (t_begin, t_end), then t_end - 1). counts[i] is the number of points
+// (2 per window with a match), or 0xffffffff when they would outgrow the
+// pair's room.
+//
+// One wave per pair walks the ops 256 at a time (4 per lane, adjacent bytes
+// across lanes): prefix sums of the target and query advances give every
+// op's positions, a prefix max the last match so far (matches advance the
+// target, so the last one has the largest position), and windows follow
+// from target positions: window w covers (end(w - 1), end(w)]. A match whose
+// window differs from the previous match's opens that window's entry (first
+// match); an op that reaches end(w) closes it when the last match so far lies
+// in w. Entry e of a pair is points 4e..4e+3 of its room: windows with a
+// match, in order.
 __global__ __launch_bounds__(256) void breaking_points_kernel(
     const AlignPair* __restrict__ pairs, const AlignResult* __restrict__ results,
     const uint8_t* __restrict__ ops, const BreakSpan* __restrict__ spans, uint32_t n,
     uint32_t window_length, uint32_t* __restrict__ points, uint32_t* __restrict__ counts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (i >= n) return;  // (uniform over the wave)
   const BreakSpan sp = spans[i];
   if (results[i].status != kAlignOk) {
-    counts[i] = 0xfffffffeu;  // not aligned on the device
+    if (lane == 0) counts[i] = 0xfffffffeu;  // not aligned on the device
     return;
   }
   const uint8_t* op = ops + pairs[i].out_off;
   const uint32_t n_ops = results[i].n_ops;
-  const uint32_t W = window_length;
-  // next window end: the first multiple of W above t_begin, less one, while
-  // below t_end; then t_end - 1; then none
-  uint64_t next_mult = (static_cast<uint64_t>(sp.t_begin) / W + 1) * W;
-  int64_t end = next_mult < sp.t_end ? static_cast<int64_t>(next_mult) - 1
-                                     : static_cast<int64_t>(sp.t_end) - 1;
-  bool last_end = next_mult >= sp.t_end;
-  int64_t t = static_cast<int64_t>(sp.t_begin) - 1;
-  int64_t q = static_cast<int64_t>(sp.q_first) - 1;
-  bool found = false;
-  uint32_t ft = 0, fq = 0, lt = 0, lq = 0;
-  uint32_t cnt = 0;
-  bool overflow = false;
+  const int64_t W = window_length;
+  // windows: ends m0 - 1, m0 + W - 1, ... for the nm multiples of W in
+  // (t_begin, t_end), then t_end - 1 (window nm)
+  const int64_t t_end = sp.t_end;
+  const int64_t m0_wide = (static_cast<int64_t>(sp.t_begin) / W + 1) * W;
+  const uint32_t nm = m0_wide < t_end ? static_cast<uint32_t>((t_end - 1 - m0_wide) / W + 1) : 0;
+  const uint32_t m0 = nm ? static_cast<uint32_t>(m0_wide) : 0;  // < t_end when used
+  const uint32_t Wn = window_length;
+  // (target positions here are in [0, t_end), so 32-bit arithmetic holds)
+  auto window_of = [&](int64_t t) -> uint32_t {
+    const uint32_t tt = static_cast<uint32_t>(t);
+    const uint32_t w = (nm == 0 || tt < m0) ? 0 : (tt - m0) / Wn + 1;
+    return w < nm ? w : nm;
+  };
+  auto window_end = [&](uint32_t w) -> int64_t {
+    return w < nm ? static_cast<int64_t>(m0) + static_cast<int64_t>(w) * W - 1 : t_end - 1;
+  };
   uint32_t* out = points + 2 * sp.out;
-  for (uint32_t k = 0; k < n_ops; ++k) {
-    const uint32_t o = op[k] & 3u;
-    if (o == 1u) {
-      ++q;
-      continue;
+  const int64_t cap_entries = sp.cap / 2;  // entries that fit the pair's room
+  // carried over chunks (uniform)
+  int64_t t_base = static_cast<int64_t>(sp.t_begin) - 1;
+  int64_t q_base = static_cast<int64_t>(sp.q_first) - 1;
+  int64_t last_t = -1, last_q = -1;  // last match so far (t, q after the op)
+  int64_t opened = 0;                // entries opened so far
+  int64_t closed = 0;                // entries closed so far
+  for (uint32_t c0 = 0; c0 < n_ops; c0 += 4 * kWave) {
+    uint32_t o[4];
+    int32_t dt = 0, dq = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t k = c0 + 4 * lane + u;
+      o[u] = k < n_ops ? (op[k] & 3u) : 4u;  // 4: past the end (no op)
+      dt += (o[u] != 1u && o[u] != 4u);
+      dq += (o[u] != 2u && o[u] != 4u);
     }
-    ++t;
-    if (o != 2u) {
-      ++q;
-      if (!found) {
-        found = true;
-        ft = static_cast<uint32_t>(t);
-        fq = static_cast<uint32_t>(q);
+    const int32_t st = lane_scan_add(dt), sq = lane_scan_add(dq);
+    int64_t t = t_base + (st - dt), q = q_base + (sq - dq);
+    // positions after each of the lane's ops, and the lane's last match
+    int64_t ta[4], qa[4];
+    int32_t lm_t = -1, lm_q = -1;  // relative to t_base / q_base + 1
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (o[u] != 1u && o[u] != 4u) ++t;
+      if (o[u] != 2u && o[u] != 4u) ++q;
+      ta[u] = t;
+      qa[u] = q;
+      if (o[u] == 0u || o[u] == 3u) {
+        lm_t = static_cast<int32_t>(t - t_base);
+        lm_q = static_cast<int32_t>(q - q_base);
       }
-      lt = static_cast<uint32_t>(t + 1);
-      lq = static_cast<uint32_t>(q + 1);
     }
-    if (t == end) {
-      if (found) {
-        if (cnt + 2 > sp.cap) {
-          overflow = true;
-        } else {
-          out[2 * cnt] = ft;
-          out[2 * cnt + 1] = fq;
-          out[2 * cnt + 2] = lt;
-          out[2 * cnt + 3] = lq;
+    // last match before the lane (exclusive max over lanes, then the carry)
+    const int32_t mt = lane_scan_max(lm_t), mq = lane_scan_max(lm_q);
+    const int32_t pt = __shfl_up(mt, 1, kWave), pq = __shfl_up(mq, 1, kWave);
+    int64_t prev_t = last_t, prev_q = last_q;
+    if (lane > 0 && pt >= 0) {
+      prev_t = t_base + pt;
+      prev_q = q_base + pq;
+    }
+    // windows of the lane's target advances (-1: none), and of the last
+    // match before each op
+    int64_t wa[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wa[u] = (o[u] != 1u && o[u] != 4u) ? window_of(ta[u]) : -1;
+    const int64_t pw0 = prev_t >= 0 ? window_of(prev_t) : -1;
+    // entries the lane opens, then their numbers
+    int32_t n_open = 0;
+    {
+      int64_t pw = pw0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (o[u] == 0u || o[u] == 3u) {
+          if (wa[u] != pw) ++n_open;
+          pw = wa[u];
         }
-        cnt += 2;
-      }
-      found = false;
-      if (last_end) {
-        end = -2;  // past the last window
-      } else {
-        next_mult += W;
-        last_end = next_mult >= sp.t_end;
-        end = last_end ? static_cast<int64_t>(sp.t_end) - 1 : static_cast<int64_t>(next_mult) - 1;
       }
     }
+    const int32_t so = lane_scan_add(n_open);
+    int64_t e_open = opened + (so - n_open);  // entries opened before this lane
+    int32_t n_close = 0;
+    int64_t pw = pw0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (o[u] == 0u || o[u] == 3u) {
+        if (wa[u] != pw) {
+          // first match of window w: entry e_open
+          if (e_open < cap_entries) {
+            out[4 * e_open] = static_cast<uint32_t>(ta[u]);
+            out[4 * e_open + 1] = static_cast<uint32_t>(qa[u]);
+          }
+          ++e_open;
+        }
+        prev_t = ta[u];
+        prev_q = qa[u];
+        pw = wa[u];
+      }
+      if (wa[u] >= 0) {
+        if (ta[u] == window_end(static_cast<uint32_t>(wa[u])) && pw == wa[u]) {
+          // the op reaches the window's last base with a match in the window:
+          // close its entry (the last one opened)
+          const int64_t e = e_open - 1;
+          if (e < cap_entries) {
+            out[4 * e + 2] = static_cast<uint32_t>(prev_t + 1);
+            out[4 * e + 3] = static_cast<uint32_t>(prev_q + 1);
+          }
+          ++n_close;
+        }
+      }
+    }
+    const int32_t sc = lane_scan_add(n_close);
+    // carry to the next chunk (from the last lane)
+    const int32_t all_t = __shfl(st, kWave - 1, kWave), all_q = __shfl(sq, kWave - 1, kWave);
+    const int32_t all_mt = __shfl(mt, kWave - 1, kWave), all_mq = __shfl(mq, kWave - 1, kWave);
+    if (all_mt >= 0) {
+      last_t = t_base + all_mt;
+      last_q = q_base + all_mq;
+    }
+    t_base += all_t;
+    q_base += all_q;
+    opened += __shfl(so, kWave - 1, kWave);
+    closed += __shfl(sc, kWave - 1, kWave);
   }
-  counts[i] = overflow ? 0xffffffffu : cnt;
+  if (lane == 0) {
+    counts[i] = 2 * closed > sp.cap ? 0xffffffffu : static_cast<uint32_t>(2 * closed);
+  }
 }
 
 hipError_t launch_breaking_points(const AlignPair* pairs, const AlignResult* results,
                                   const uint8_t* ops, const BreakSpan* spans, uint32_t n,
                                   uint32_t window_length, uint32_t* points, uint32_t* counts,
                                   hipStream_t stream) {
-  hipLaunchKernelGGL(breaking_points_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, pairs,
+  // one wave per pair, four pairs per workgroup
+  hipLaunchKernelGGL(breaking_points_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, pairs,
                      results, ops, spans, n, window_length, points, counts);
   return hipGetLastError();
 }

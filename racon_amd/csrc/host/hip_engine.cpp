@@ -137,8 +137,10 @@ class HipEngine : public PolishEngine {
     int32_t rc = chunks.empty() ? RH_OK : upload(0);
     if (rc != RH_OK) die("rh_align_upload", rc);
     // the kernel of a chunk runs on a helper thread (rh_align_run blocks):
-    // chunk c + 1 is uploaded while chunk c's kernel runs, and its kernel
-    // starts before chunk c's breaking points and callbacks
+    // chunk c + 1 is uploaded while chunk c's kernel runs; chunk c's
+    // breaking points (a few ms of device time) come before chunk c + 1's
+    // kernel, which would hold every compute unit until it ends, and chunk
+    // c's callbacks and chunk c + 2's upload run beside that kernel
     int32_t run_rc = RH_OK;
     std::thread runner;
     auto start = [&](uint64_t c) {
@@ -153,7 +155,6 @@ class HipEngine : public PolishEngine {
       runner.join();
       if (run_rc != RH_OK) die("rh_align_run", run_rc);
       if (up_rc != RH_OK) die("rh_align_upload", up_rc);
-      if (c + 1 < chunks.size()) start(c + 1);
       rh_align_stats st{};
       if (rh_align_get_stats(sl.batch, &st) == RH_OK) {
         total.pack_ms += st.pack_ms;
@@ -172,6 +173,7 @@ class HipEngine : public PolishEngine {
       points_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                              t_points)
                        .count();
+      if (c + 1 < chunks.size()) start(c + 1);
       const uint64_t first = chunks[c].first;
       for (uint32_t i = 0; i < n; ++i) {
         if (counts[i] == RH_PAIR_REJECTED) {
