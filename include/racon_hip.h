@@ -370,6 +370,67 @@ int32_t rh_align_breaking_points(rh_align_batch* batch, uint32_t window_length,
                                  const rh_break_span* spans, uint64_t* offsets, uint32_t* counts,
                                  uint32_t* points, uint64_t capacity);
 
+/* ---- layer extraction on the device ----
+ *
+ * Replaces racon's loop from breaking points to window layers
+ * (Polisher::initialize, src/polisher.cpp:403-457): every breaking-point pair
+ * of an overlap whose query span is at least 2% of the window length and
+ * whose mean base quality (over the span, Phred + 33) is at least the
+ * threshold becomes a layer of window first_window + t / window_length, the
+ * layers of a window in overlap order. The points stay on the device: a set
+ * collects the layers of every rh_align_breaking_points call, then groups
+ * them by window. */
+typedef struct rh_layer_set rh_layer_set;
+
+#define RH_NO_QUALITY 0xffffffffffffffffull
+
+/* Where the layers of pair i of a batch go. quality is the offset of the
+ * read's quality bytes in the set's quality bytes (RH_NO_QUALITY: the read
+ * has none, only the span filter applies); mirrored != 0 when those bytes run
+ * opposite to the pair's query, i.e. they are the forward quality of a '-'
+ * overlap or the reversed quality of a '+' one (racon sums
+ * sequence->reverse_quality() for '-', src/polisher.cpp:414-429). */
+typedef struct {
+  uint32_t overlap;       /* carried into the layer records                */
+  uint32_t query_length;  /* length of the read                            */
+  uint64_t first_window;  /* id of the first window of the overlap's target */
+  uint64_t quality;       /* offset in the quality bytes or RH_NO_QUALITY  */
+  uint32_t mirrored;
+  uint32_t reserved;
+} rh_layer_source;
+
+/* One layer: query bytes [q_begin, q_end) of the pair (in its orientation),
+ * covering backbone positions begin..end (inclusive) of window `window`
+ * (the arguments of Window::add_layer, src/polisher.cpp:431-453). */
+typedef struct {
+  uint32_t window;
+  uint32_t overlap;
+  uint32_t q_begin;
+  uint32_t q_end;
+  uint32_t begin;
+  uint32_t end;
+} rh_layer_record;
+
+/* A set for windows [0, n_windows) (n_windows < 2^32 - 1) with room for
+ * `capacity` breaking-point pairs: the sum of rh_breaking_points_bound / 2
+ * over the batches it will take. qualities (qualities_size bytes, may be
+ * NULL when 0) is copied to the device. */
+int32_t rh_layers_create(int32_t device, uint32_t window_length, double quality_threshold,
+                         uint64_t n_windows, uint64_t capacity, const char* qualities,
+                         uint64_t qualities_size, rh_layer_set** set);
+/* Applies the filters to the breaking points of the last
+ * rh_align_breaking_points call on `batch` (same device and window length),
+ * sources[i] for its pair i; n = the batch's pairs. Pairs with count
+ * RH_PAIR_REJECTED give no layers. Returns when the batch may be reused. */
+int32_t rh_layers_add(rh_layer_set* set, rh_align_batch* batch, const rh_layer_source* sources,
+                      uint32_t n);
+/* Groups the layers by window (stable: overlap order within a window) and
+ * gives their number. */
+int32_t rh_layers_finish(rh_layer_set* set, uint64_t* n_layers);
+/* The grouped layers, ordered by window, then by the order they were added. */
+int32_t rh_layers_download(rh_layer_set* set, rh_layer_record* records, uint64_t capacity);
+void rh_layers_destroy(rh_layer_set* set);
+
 /* ---- allocation-free overlap alignment ----
  *
  * No counterpart in racon. For callers that cannot allocate device memory

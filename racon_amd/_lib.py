@@ -86,6 +86,31 @@ class RhBreakSpan(ctypes.Structure):
     ]
 
 
+class RhLayerSource(ctypes.Structure):
+    _fields_ = [
+        ("overlap", ctypes.c_uint32),
+        ("query_length", ctypes.c_uint32),
+        ("first_window", ctypes.c_uint64),
+        ("quality", ctypes.c_uint64),
+        ("mirrored", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class RhLayerRecord(ctypes.Structure):
+    _fields_ = [
+        ("window", ctypes.c_uint32),
+        ("overlap", ctypes.c_uint32),
+        ("q_begin", ctypes.c_uint32),
+        ("q_end", ctypes.c_uint32),
+        ("begin", ctypes.c_uint32),
+        ("end", ctypes.c_uint32),
+    ]
+
+
+RH_NO_QUALITY = 0xFFFFFFFFFFFFFFFF
+
+
 class RhAlignStats(ctypes.Structure):
     _fields_ = [
         ("pack_ms", ctypes.c_double),
@@ -215,6 +240,15 @@ _SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_uint32),
                                                   ctypes.POINTER(ctypes.c_uint32),
                                                   ctypes.c_uint64]),
+    "rh_layers_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_uint32, ctypes.c_double,
+                                          ctypes.c_uint64, ctypes.c_uint64, ctypes.c_char_p,
+                                          ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "rh_layers_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(RhLayerSource), ctypes.c_uint32]),
+    "rh_layers_finish": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "rh_layers_download": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(RhLayerRecord),
+                                            ctypes.c_uint64]),
+    "rh_layers_destroy": (None, [ctypes.c_void_p]),
     # device pointers are passed as integers (c_void_p)
     "rh_align_run_noalloc": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint32,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
@@ -563,3 +597,50 @@ class AlignBatch:
         _check(self.lib.rh_align_get_stats(self.handle, ctypes.byref(st)),
                "rh_align_get_stats")
         return st
+class LayerSet:
+    """Window layers from device breaking points (rh_layers_*): racon's
+    span and mean-quality filters and the grouping by window of
+    src/polisher.cpp:403-457."""
+
+    def __init__(self, window_length: int, quality_threshold: float, n_windows: int,
+                 capacity: int, qualities: bytes = b"", device: int = 0):
+        self.lib = load_library()
+        self.handle = ctypes.c_void_p()
+        self._qualities = bytes(qualities)
+        _check(self.lib.rh_layers_create(device, window_length, quality_threshold, n_windows,
+                                         capacity, self._qualities or None,
+                                         len(self._qualities), ctypes.byref(self.handle)),
+               "rh_layers_create")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.rh_layers_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, batch: "AlignBatch", sources: Sequence[tuple]) -> None:
+        """sources[i] = (overlap, query_length, first_window, quality offset or
+        None, mirrored) for pair i of the batch's last breaking points."""
+        arr = (RhLayerSource * max(1, len(sources)))()
+        for i, (ov, qlen, fw, qoff, mirrored) in enumerate(sources):
+            arr[i].overlap, arr[i].query_length, arr[i].first_window = ov, qlen, fw
+            arr[i].quality = RH_NO_QUALITY if qoff is None else qoff
+            arr[i].mirrored = 1 if mirrored else 0
+        _check(self.lib.rh_layers_add(self.handle, batch.handle, arr, len(sources)),
+               "rh_layers_add")
+
+    def finish(self) -> list:
+        """Layers grouped by window: (window, overlap, q_begin, q_end, begin, end)."""
+        n = ctypes.c_uint64()
+        _check(self.lib.rh_layers_finish(self.handle, ctypes.byref(n)), "rh_layers_finish")
+        recs = (RhLayerRecord * max(1, n.value))()
+        _check(self.lib.rh_layers_download(self.handle, recs, n.value), "rh_layers_download")
+        return [(r.window, r.overlap, r.q_begin, r.q_end, r.begin, r.end)
+                for r in recs[:n.value]]
+
+

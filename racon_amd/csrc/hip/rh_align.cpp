@@ -45,9 +45,30 @@ struct rh_align_batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool uploaded = false, ran = false;
+  // the last rh_align_breaking_points call's points, still on the device
+  bool points_ready = false;
+  uint32_t points_window = 0;
+  uint64_t points_total = 0;
   uint32_t band0 = 0;  // first band guess of every pair (0: per pair, automatic)
   rh_align_stats stats{};
 };
+
+namespace rh {
+
+bool break_view(const rh_align_batch* b, BreakView* v) {
+  if (b == nullptr || v == nullptr || !b->points_ready) return false;
+  v->spans = b->d_spans.as<BreakSpan>();
+  v->counts = b->d_counts.as<uint32_t>();
+  v->points = b->d_points.as<uint32_t>();
+  v->n = static_cast<uint32_t>(b->pairs.size());
+  v->window_length = b->points_window;
+  v->points_total = b->points_total;
+  v->stream = b->stream;
+  v->device = b->device;
+  return true;
+}
+
+}  // namespace rh
 
 extern "C" {
 
@@ -179,6 +200,7 @@ int32_t rh_align_upload(rh_align_batch* b, uint32_t n, const rh_pair* pairs) {
   b->stats.h2d_ms = rh::ms_since(t1);
   b->uploaded = true;
   b->ran = false;
+  b->points_ready = false;
   return RH_OK;
 }
 
@@ -194,6 +216,7 @@ int32_t rh_align_run(rh_align_batch* b) {
   b->stats.launches = 0;
   if (n == 0) {
     b->ran = true;
+    b->points_ready = false;
     return RH_OK;
   }
   const rh::AlignSlotCaps caps = rh::align_caps_for(b->max_q, b->max_t);
@@ -253,6 +276,7 @@ int32_t rh_align_run(rh_align_batch* b) {
     b->stats.ops += b->results[i].n_ops;
   }
   b->ran = true;
+  b->points_ready = false;
   return RH_OK;
 }
 
@@ -374,6 +398,9 @@ int32_t rh_align_breaking_points(rh_align_batch* b, uint32_t window_length,
                  RH_ERR_DEVICE);
   }
   RH_HIP_CHECK(hipStreamSynchronize(b->stream), RH_ERR_DEVICE);
+  b->points_ready = true;
+  b->points_window = window_length;
+  b->points_total = total;
   for (uint32_t i = 0; i < n; ++i) {
     if (counts[i] == 0xffffffffu) {
       rh::set_error("rh_align_breaking_points: pair " + std::to_string(i) +

@@ -31,6 +31,24 @@ const char* last_error();
     }                                                                              \
   } while (0)
 
+// The breaking points an aligner batch holds on the device after
+// rh_align_breaking_points: spans[i] (room and offset of pair i), counts[i]
+// and the points, on the batch's stream.
+struct BreakSpan;
+struct BreakView {
+  const BreakSpan* spans;
+  const uint32_t* counts;
+  const uint32_t* points;
+  uint32_t n;
+  uint32_t window_length;
+  uint64_t points_total;
+  hipStream_t stream;
+  int32_t device;
+};
+// false when the batch holds no breaking points (not computed since its last
+// upload)
+bool break_view(const rh_align_batch* b, BreakView* v);
+
 inline double ms_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
       .count();

@@ -32,6 +32,33 @@ struct AlignTask {
   uint32_t q_first = 0;
 };
 
+// Where the layers of one aligned overlap go (polisher.cpp:403-457): the
+// overlap, the read's length, the first window of the overlap's target, and
+// the read's quality bytes in the job's quality buffer (kNoQuality: none),
+// mirrored when those bytes run opposite to the aligned query (the forward
+// quality of a '-' overlap or the reversed one of a '+' overlap).
+constexpr uint64_t kNoQuality = ~0ull;
+struct LayerSource {
+  uint32_t overlap = 0;
+  uint32_t query_length = 0;
+  uint64_t first_window = 0;
+  uint64_t quality = kNoQuality;
+  uint32_t mirrored = 0;
+  uint32_t reserved = 0;
+};
+
+// One window layer: query bytes [q_begin, q_end) of the overlap (in its
+// aligned orientation) over backbone positions begin..end (inclusive) of
+// window `window` (the arguments of Window::add_layer, polisher.cpp:431-453).
+struct LayerRecord {
+  uint32_t window;
+  uint32_t overlap;
+  uint32_t q_begin;
+  uint32_t q_end;
+  uint32_t begin;
+  uint32_t end;
+};
+
 // Scores and switches shared by every engine (racon CLI defaults:
 // main.cpp:51-66).
 struct EngineConfig {
@@ -83,6 +110,29 @@ class PolishEngine {
     (void)window_length;
     (void)done;
     return false;
+  }
+
+  // True when align_layers does the work (checked before any is done).
+  virtual bool device_layers() const { return false; }
+
+  // Aligns every task (sources[i] belongs to tasks[i]) and turns the
+  // breaking points into window layers with racon's filters (query span at
+  // least 2% of window_length, mean quality at least quality_threshold when
+  // the read has qualities): layers come back grouped by window, in task
+  // order within a window, windows [0, n_windows). Only called when
+  // device_layers() is true.
+  virtual void align_layers(const std::vector<AlignTask>& tasks,
+                            const std::vector<LayerSource>& sources, uint32_t window_length,
+                            double quality_threshold, uint64_t n_windows,
+                            const std::vector<char>& qualities,
+                            std::vector<LayerRecord>* layers) {
+    (void)tasks;
+    (void)sources;
+    (void)window_length;
+    (void)quality_threshold;
+    (void)n_windows;
+    (void)qualities;
+    (void)layers;
   }
 
   // Generates the consensus of every window (set through

@@ -5,6 +5,7 @@
 // fallback, and a device failure ends the run with an error.
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -92,6 +93,65 @@ class HipEngine : public PolishEngine {
   bool align_breaking_points(
       const std::vector<AlignTask>& tasks, uint32_t window_length,
       const std::function<void(uint64_t, const uint32_t*, uint64_t)>& done) override {
+    align_chunks(tasks, window_length, &done, nullptr, nullptr);
+    return true;
+  }
+
+  bool device_layers() const override { return true; }
+
+  // The breaking points stay on the device: each chunk's go through the
+  // layer filters into one set, which groups the layers by window at the end.
+  void align_layers(const std::vector<AlignTask>& tasks, const std::vector<LayerSource>& sources,
+                    uint32_t window_length, double quality_threshold, uint64_t n_windows,
+                    const std::vector<char>& qualities,
+                    std::vector<LayerRecord>* layers) override {
+    static_assert(sizeof(LayerSource) == sizeof(rh_layer_source) &&
+                      offsetof(LayerSource, first_window) == offsetof(rh_layer_source, first_window) &&
+                      offsetof(LayerSource, quality) == offsetof(rh_layer_source, quality) &&
+                      offsetof(LayerSource, mirrored) == offsetof(rh_layer_source, mirrored),
+                  "LayerSource mirrors rh_layer_source");
+    static_assert(sizeof(LayerRecord) == sizeof(rh_layer_record) &&
+                      offsetof(LayerRecord, end) == offsetof(rh_layer_record, end),
+                  "LayerRecord mirrors rh_layer_record");
+    std::vector<rh_break_span> spans(tasks.size());
+    for (uint64_t i = 0; i < tasks.size(); ++i) {
+      spans[i] = rh_break_span{tasks[i].t_begin, tasks[i].t_end, tasks[i].q_first};
+    }
+    uint64_t room = 0;  // breaking-point pairs the chunks can give
+    for (uint64_t i = 0; i < tasks.size(); i += 1u << 20) {
+      const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(tasks.size() - i, 1u << 20));
+      room += rh_breaking_points_bound(window_length, spans.data() + i, n) / 2;
+    }
+    rh_layer_set* set = nullptr;
+    int32_t rc = rh_layers_create(0, window_length, quality_threshold, n_windows, room,
+                                  qualities.data(), qualities.size(), &set);
+    if (rc != RH_OK) die("rh_layers_create", rc);
+    align_chunks(tasks, window_length, nullptr, set,
+                 reinterpret_cast<const rh_layer_source*>(sources.data()));
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t n_layers = 0;
+    rc = rh_layers_finish(set, &n_layers);
+    if (rc != RH_OK) die("rh_layers_finish", rc);
+    layers->resize(n_layers);
+    rc = rh_layers_download(set, reinterpret_cast<rh_layer_record*>(layers->data()), n_layers);
+    if (rc != RH_OK) die("rh_layers_download", rc);
+    rh_layers_destroy(set);
+    fprintf(stderr,
+            "[racon_amd::layers] %lu layers of %lu windows from %lu breaking-point pairs: "
+            "grouped and copied in %.1f ms\n",
+            static_cast<unsigned long>(n_layers), static_cast<unsigned long>(n_windows),
+            static_cast<unsigned long>(room),
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count());
+  }
+
+ private:
+  // Chunks of the tasks through two aligner batches. Each chunk's breaking
+  // points go to done() (per task) or, with a layer set, through
+  // rh_layers_add with sources[i] for task i.
+  void align_chunks(const std::vector<AlignTask>& tasks, uint32_t window_length,
+                    const std::function<void(uint64_t, const uint32_t*, uint64_t)>* done,
+                    rh_layer_set* layer_set, const rh_layer_source* sources) {
     // chunks of about 1 GiB of sequence on two batch objects: chunk c + 1 is
     // packed and copied while chunk c's kernel runs
     const uint64_t kChunkBytes = 1ull << 30;
@@ -170,10 +230,6 @@ class HipEngine : public PolishEngine {
       rc = rh_align_breaking_points(sl.batch, window_length, sl.spans.data(), offsets.data(),
                                     counts.data(), points.data(), points.size() / 2);
       if (rc != RH_OK) die("rh_align_breaking_points", rc);
-      points_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
-                                                             t_points)
-                       .count();
-      if (c + 1 < chunks.size()) start(c + 1);
       const uint64_t first = chunks[c].first;
       for (uint32_t i = 0; i < n; ++i) {
         if (counts[i] == RH_PAIR_REJECTED) {
@@ -184,17 +240,30 @@ class HipEngine : public PolishEngine {
                   sl.pairs[i].target_length);
           exit(1);
         }
-        done(first + i, points.data() + 2 * offsets[i], counts[i]);
+      }
+      if (layer_set != nullptr) {
+        rc = rh_layers_add(layer_set, sl.batch, sources + first, n);
+        if (rc != RH_OK) die("rh_layers_add", rc);
+      }
+      points_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                             t_points)
+                       .count();
+      if (c + 1 < chunks.size()) start(c + 1);
+      if (done != nullptr) {
+        for (uint32_t i = 0; i < n; ++i) {
+          (*done)(first + i, points.data() + 2 * offsets[i], counts[i]);
+        }
       }
     }
     for (uint32_t k = 0; k < n_batches; ++k) rh_align_destroy(slots[k].batch);
     fprintf(stderr,
             "[racon_amd::align] %lu overlaps in %zu chunks: pack %.1f ms, H2D %.1f ms, kernels "
-            "%.1f ms, breaking points (kernel + copy) %.1f ms\n",
+            "%.1f ms, breaking points%s (kernels + copy) %.1f ms\n",
             static_cast<unsigned long>(total.pairs), chunks.size(), total.pack_ms, total.h2d_ms,
-            total.kernel_ms, points_ms);
-    return true;
+            total.kernel_ms, layer_set != nullptr ? " and layer filters" : "", points_ms);
   }
+
+ public:
 
   // Windows go to the device in chunks of about 1 GiB of sequence, on two
   // batch objects: while one chunk's kernel runs, the next chunk is packed

@@ -292,11 +292,28 @@ void Polisher::initialize() {
   timings_.load = seconds_since(t_load);
 
   auto t_align = std::chrono::steady_clock::now();
-  find_overlap_breaking_points(overlaps);
+  std::vector<LayerRecord> device_layers;
+  const bool on_device = align_layers_on_device(overlaps, targets_size, &device_layers);
+  if (!on_device) {
+    find_overlap_breaking_points(overlaps);
+  }
   timings_.align = seconds_since(t_align);
 
   logger_->log();
   auto t_windows = std::chrono::steady_clock::now();
+
+  // device layers come grouped by window: window w's are records
+  // [layer_at[w], layer_at[w + 1])
+  std::vector<uint64_t> layer_at;
+  if (on_device) {
+    uint64_t n_windows = 0;
+    for (uint64_t i = 0; i < targets_size; ++i) {
+      n_windows += (sequences_[i]->data().size() + window_length_ - 1) / window_length_;
+    }
+    layer_at.assign(n_windows + 1, 0);
+    for (const LayerRecord& l : device_layers) ++layer_at[l.window + 1];
+    for (uint64_t w = 0; w < n_windows; ++w) layer_at[w + 1] += layer_at[w];
+  }
 
   // polisher.cpp:384-457
   std::vector<uint64_t> id_to_first_window_id(targets_size + 1, 0);
@@ -310,11 +327,73 @@ void Polisher::initialize() {
           sequences_[i]->quality().empty() ? &(dummy_quality_[0])
                                            : &(sequences_[i]->quality()[j]),
           length));
+      if (on_device) {
+        const uint64_t w = windows_.size() - 1;
+        windows_.back()->reserve_layers(static_cast<uint32_t>(layer_at[w + 1] - layer_at[w]));
+      }
     }
     id_to_first_window_id[i + 1] = id_to_first_window_id[i] + k;
   }
 
   targets_coverages_.resize(targets_size, 0);
+
+  if (on_device) {
+    // the engine applied the filters and grouped the layers by window, in
+    // overlap order: each window's layers are added by one worker
+    const double create_s = seconds_since(t_windows);
+    for (uint64_t i = 0; i < overlaps.size(); ++i) {
+      ++targets_coverages_[overlaps[i]->t_id()];
+    }
+    // each overlap's bytes in its aligned orientation
+    std::vector<const char*> data_at(overlaps.size()), quality_at(overlaps.size());
+    auto bases = [&](uint64_t begin, uint64_t end) {
+      for (uint64_t i = begin; i < end; ++i) {
+        const Overlap& o = *overlaps[i];
+        const auto& sequence = sequences_[o.q_id()];
+        data_at[i] = o.strand() ? sequence->reverse_complement().data() : sequence->data().data();
+        const std::string& q = o.strand() ? sequence->reverse_quality() : sequence->quality();
+        quality_at[i] = q.empty() ? nullptr : q.data();
+      }
+    };
+    auto attach = [&](uint64_t begin, uint64_t end) {  // windows [begin, end)
+      for (uint64_t r = layer_at[begin]; r < layer_at[end]; ++r) {
+        const LayerRecord& l = device_layers[r];
+        const uint32_t data_length = l.q_end - l.q_begin;
+        const char* quality = quality_at[l.overlap];
+        windows_[l.window]->add_layer(data_at[l.overlap] + l.q_begin, data_length,
+                                      quality == nullptr ? nullptr : quality + l.q_begin,
+                                      quality == nullptr ? 0 : data_length, l.begin, l.end);
+      }
+    };
+    const uint64_t n = device_layers.size();
+    const uint64_t workers = thread_pool_->size();
+    auto fan_out = [&](const std::function<void(uint64_t, uint64_t)>& f, uint64_t items) {
+      if (workers <= 1 || items < 4096) {
+        f(0, items);
+        return;
+      }
+      const uint64_t step = (items + 4 * workers - 1) / (4 * workers);
+      std::vector<std::future<void>> done;
+      for (uint64_t b = 0; b < items; b += step) {
+        done.emplace_back(thread_pool_->submit(f, b, std::min(items, b + step)));
+      }
+      for (auto& d : done) d.wait();
+    };
+    fan_out(bases, overlaps.size());
+    fan_out(attach, windows_.size());
+    const double attach_s = seconds_since(t_windows) - create_s;
+    for (auto& o : overlaps) o.reset();
+    timings_.windows = seconds_since(t_windows);
+    fprintf(stderr,
+            "[racon_amd::windows] %lu windows: created in %.1f ms, %lu device layers added in "
+            "%.1f ms, overlaps released in %.1f ms\n",
+            static_cast<unsigned long>(windows_.size()), 1e3 * create_s,
+            static_cast<unsigned long>(n), 1e3 * attach_s,
+            1e3 * (timings_.windows - create_s - attach_s));
+    timings_.num_windows = windows_.size();
+    logger_->log("[racon::Polisher::initialize] transformed data into windows");
+    return;
+  }
 
   // the two layer filters of each breaking-point pair (span length, mean
   // quality), decided on the pool; the layers are then added in overlap order
@@ -447,13 +526,95 @@ void Polisher::dump_windows() const {
   fclose(f);
 }
 
-void Polisher::find_overlap_breaking_points(
-    std::vector<std::unique_ptr<Overlap>>& overlaps) {
-  // polisher.cpp:462-484 with the per-overlap edlib calls batched through
-  // the engine; SAM overlaps carry their CIGAR and skip alignment
-  // (overlap.cpp:192).
+bool Polisher::align_layers_on_device(std::vector<std::unique_ptr<Overlap>>& overlaps,
+                                      uint64_t targets_size,
+                                      std::vector<LayerRecord>* layers) {
+  if (!engine_->device_layers() || overlaps.empty() || overlaps.size() >= (1ull << 32)) {
+    return false;
+  }
+  for (const auto& o : overlaps) {
+    if (!o->needs_alignment()) return false;
+  }
+  const auto t_inputs = std::chrono::steady_clock::now();
+  // window ids as the window loop below gives them (polisher.cpp:384-398)
+  std::vector<uint64_t> first_window(targets_size + 1, 0);
+  for (uint64_t i = 0; i < targets_size; ++i) {
+    const uint64_t length = sequences_[i]->data().size();
+    first_window[i + 1] = first_window[i] + (length + window_length_ - 1) / window_length_;
+  }
+  // the quality bytes of every read an overlap uses: the forward string, or
+  // the reversed one when transmute kept only that
+  std::vector<uint64_t> quality_at(sequences_.size(), kNoQuality);
+  std::vector<uint8_t> stored_reversed(sequences_.size(), 0);
+  uint64_t quality_bytes = 0;
+  for (const auto& o : overlaps) {
+    const uint64_t q = o->q_id();
+    if (quality_at[q] != kNoQuality) continue;
+    const auto& sequence = sequences_[q];
+    if (!sequence->quality().empty()) {
+      quality_at[q] = quality_bytes;
+      quality_bytes += sequence->quality().size();
+    } else if (!sequence->reverse_quality().empty()) {
+      quality_at[q] = quality_bytes;
+      stored_reversed[q] = 1;
+      quality_bytes += sequence->reverse_quality().size();
+    }
+  }
+  std::vector<char> qualities(quality_bytes);
+  auto copy_qualities = [&](uint64_t begin, uint64_t end) {
+    for (uint64_t q = begin; q < end; ++q) {
+      if (quality_at[q] == kNoQuality) continue;
+      const std::string& s =
+          stored_reversed[q] ? sequences_[q]->reverse_quality() : sequences_[q]->quality();
+      std::copy(s.begin(), s.end(), qualities.begin() + quality_at[q]);
+    }
+  };
+  {
+    const uint64_t n = sequences_.size();
+    const uint64_t workers = thread_pool_->size();
+    if (workers <= 1 || quality_bytes < (1u << 24)) {
+      copy_qualities(0, n);
+    } else {
+      const uint64_t step = (n + 4 * workers - 1) / (4 * workers);
+      std::vector<std::future<void>> copies;
+      for (uint64_t b = 0; b < n; b += step) {
+        copies.emplace_back(thread_pool_->submit(copy_qualities, b, std::min(n, b + step)));
+      }
+      for (auto& f : copies) f.wait();
+    }
+  }
+
   std::vector<AlignTask> tasks;
   std::vector<uint64_t> task_overlap;
+  collect_alignment_tasks(overlaps, &tasks, &task_overlap);
+  std::vector<LayerSource> sources(tasks.size());
+  for (uint64_t k = 0; k < tasks.size(); ++k) {
+    const Overlap& o = *overlaps[task_overlap[k]];
+    const auto& sequence = sequences_[o.q_id()];
+    LayerSource& src = sources[k];
+    src.overlap = static_cast<uint32_t>(task_overlap[k]);
+    src.query_length = static_cast<uint32_t>(
+        std::max(sequence->data().size(), sequence->reverse_complement().size()));
+    src.first_window = first_window[o.t_id()];
+    src.quality = quality_at[o.q_id()];
+    src.mirrored = (o.strand() != 0) != (stored_reversed[o.q_id()] != 0) ? 1 : 0;
+  }
+  fprintf(stderr,
+          "[racon_amd::layers] %.1f MB of read qualities and %lu layer sources gathered in "
+          "%.1f ms\n",
+          quality_bytes / 1e6, static_cast<unsigned long>(sources.size()),
+          1e3 * seconds_since(t_inputs));
+  engine_->align_layers(tasks, sources, window_length_, quality_threshold_,
+                        first_window[targets_size], qualities, layers);
+  logger_->log("[racon::Polisher::initialize] aligned overlaps");
+  return true;
+}
+
+void Polisher::collect_alignment_tasks(std::vector<std::unique_ptr<Overlap>>& overlaps,
+                                       std::vector<AlignTask>* tasks_out,
+                                       std::vector<uint64_t>* task_overlap_out) {
+  std::vector<AlignTask>& tasks = *tasks_out;
+  std::vector<uint64_t>& task_overlap = *task_overlap_out;
   for (uint64_t i = 0; i < overlaps.size(); ++i) {
     overlaps[i]->check_transmuted();
     if (overlaps[i]->needs_alignment()) {
@@ -485,6 +646,16 @@ void Polisher::find_overlap_breaking_points(
     }
     fclose(f);
   }
+}
+
+void Polisher::find_overlap_breaking_points(
+    std::vector<std::unique_ptr<Overlap>>& overlaps) {
+  // polisher.cpp:462-484 with the per-overlap edlib calls batched through
+  // the engine; SAM overlaps carry their CIGAR and skip alignment
+  // (overlap.cpp:192).
+  std::vector<AlignTask> tasks;
+  std::vector<uint64_t> task_overlap;
+  collect_alignment_tasks(overlaps, &tasks, &task_overlap);
 
   {
     std::vector<std::future<void>> futures;

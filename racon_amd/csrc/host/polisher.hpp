@@ -88,6 +88,15 @@ class Polisher {
  protected:
   virtual void find_overlap_breaking_points(
       std::vector<std::unique_ptr<Overlap>>& overlaps);
+  // The alignment inputs of the overlaps that need one (and the
+  // --dump-overlaps file).
+  void collect_alignment_tasks(std::vector<std::unique_ptr<Overlap>>& overlaps,
+                               std::vector<AlignTask>* tasks, std::vector<uint64_t>* task_overlap);
+  // Aligns the overlaps and extracts the window layers on the engine's
+  // device (polisher.cpp:403-457) when the engine can and every overlap needs
+  // an alignment; false (nothing done) otherwise.
+  bool align_layers_on_device(std::vector<std::unique_ptr<Overlap>>& overlaps,
+                              uint64_t targets_size, std::vector<LayerRecord>* layers);
 
   std::string sequences_path_;
   SequenceFormat sequences_format_;

@@ -42,6 +42,12 @@ class Window {
   void add_layer(const char* sequence, uint32_t sequence_length,
                  const char* quality, uint32_t quality_length, uint32_t begin,
                  uint32_t end);
+  // Room for n more layers (no reallocation while they are added).
+  void reserve_layers(uint32_t n) {
+    sequences_.reserve(sequences_.size() + n);
+    qualities_.reserve(qualities_.size() + n);
+    positions_.reserve(positions_.size() + n);
+  }
 
   // Number of sequences (backbone + layers).
   uint32_t size() const { return static_cast<uint32_t>(sequences_.size()); }
