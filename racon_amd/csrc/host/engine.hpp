@@ -119,19 +119,21 @@ class PolishEngine {
   // breaking points into window layers with racon's filters (query span at
   // least 2% of window_length, mean quality at least quality_threshold when
   // the read has qualities): layers come back grouped by window, in task
-  // order within a window, windows [0, n_windows). Only called when
+  // order within a window, windows [0, n_windows). gather_qualities fills
+  // the quality bytes the sources point into; the engine may call it on
+  // another thread while the first alignments run. Only called when
   // device_layers() is true.
   virtual void align_layers(const std::vector<AlignTask>& tasks,
                             const std::vector<LayerSource>& sources, uint32_t window_length,
                             double quality_threshold, uint64_t n_windows,
-                            const std::vector<char>& qualities,
+                            const std::function<void(std::vector<char>*)>& gather_qualities,
                             std::vector<LayerRecord>* layers) {
     (void)tasks;
     (void)sources;
     (void)window_length;
     (void)quality_threshold;
     (void)n_windows;
-    (void)qualities;
+    (void)gather_qualities;
     (void)layers;
   }
 

@@ -93,7 +93,7 @@ class HipEngine : public PolishEngine {
   bool align_breaking_points(
       const std::vector<AlignTask>& tasks, uint32_t window_length,
       const std::function<void(uint64_t, const uint32_t*, uint64_t)>& done) override {
-    align_chunks(tasks, window_length, &done, nullptr, nullptr);
+    align_chunks(tasks, window_length, &done, {}, nullptr);
     return true;
   }
 
@@ -103,7 +103,7 @@ class HipEngine : public PolishEngine {
   // layer filters into one set, which groups the layers by window at the end.
   void align_layers(const std::vector<AlignTask>& tasks, const std::vector<LayerSource>& sources,
                     uint32_t window_length, double quality_threshold, uint64_t n_windows,
-                    const std::vector<char>& qualities,
+                    const std::function<void(std::vector<char>*)>& gather_qualities,
                     std::vector<LayerRecord>* layers) override {
     static_assert(sizeof(LayerSource) == sizeof(rh_layer_source) &&
                       offsetof(LayerSource, first_window) == offsetof(rh_layer_source, first_window) &&
@@ -122,15 +122,27 @@ class HipEngine : public PolishEngine {
       const uint32_t n = static_cast<uint32_t>(std::min<uint64_t>(tasks.size() - i, 1u << 20));
       room += rh_breaking_points_bound(window_length, spans.data() + i, n) / 2;
     }
+    // the qualities are gathered and the set is made (its device buffers,
+    // the quality upload) beside the first chunk's alignment
     rh_layer_set* set = nullptr;
-    int32_t rc = rh_layers_create(0, window_length, quality_threshold, n_windows, room,
-                                  qualities.data(), qualities.size(), &set);
-    if (rc != RH_OK) die("rh_layers_create", rc);
-    align_chunks(tasks, window_length, nullptr, set,
+    int32_t set_rc = RH_OK;
+    std::thread maker([&] {
+      std::vector<char> qualities;
+      gather_qualities(&qualities);
+      set_rc = rh_layers_create(0, window_length, quality_threshold, n_windows, room,
+                                qualities.data(), qualities.size(), &set);
+    });
+    auto layer_set = [&]() -> rh_layer_set* {
+      if (maker.joinable()) maker.join();
+      if (set_rc != RH_OK) die("rh_layers_create", set_rc);
+      return set;
+    };
+    align_chunks(tasks, window_length, nullptr, layer_set,
                  reinterpret_cast<const rh_layer_source*>(sources.data()));
+    layer_set();  // (no chunks: made here)
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t n_layers = 0;
-    rc = rh_layers_finish(set, &n_layers);
+    int32_t rc = rh_layers_finish(set, &n_layers);
     if (rc != RH_OK) die("rh_layers_finish", rc);
     layers->resize(n_layers);
     rc = rh_layers_download(set, reinterpret_cast<rh_layer_record*>(layers->data()), n_layers);
@@ -151,7 +163,8 @@ class HipEngine : public PolishEngine {
   // rh_layers_add with sources[i] for task i.
   void align_chunks(const std::vector<AlignTask>& tasks, uint32_t window_length,
                     const std::function<void(uint64_t, const uint32_t*, uint64_t)>* done,
-                    rh_layer_set* layer_set, const rh_layer_source* sources) {
+                    const std::function<rh_layer_set*()>& layer_set,
+                    const rh_layer_source* sources) {
     // chunks of about 1 GiB of sequence on two batch objects: chunk c + 1 is
     // packed and copied while chunk c's kernel runs
     const uint64_t kChunkBytes = 1ull << 30;
@@ -241,8 +254,8 @@ class HipEngine : public PolishEngine {
           exit(1);
         }
       }
-      if (layer_set != nullptr) {
-        rc = rh_layers_add(layer_set, sl.batch, sources + first, n);
+      if (layer_set) {
+        rc = rh_layers_add(layer_set(), sl.batch, sources + first, n);
         if (rc != RH_OK) die("rh_layers_add", rc);
       }
       points_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
@@ -260,7 +273,7 @@ class HipEngine : public PolishEngine {
             "[racon_amd::align] %lu overlaps in %zu chunks: pack %.1f ms, H2D %.1f ms, kernels "
             "%.1f ms, breaking points%s (kernels + copy) %.1f ms\n",
             static_cast<unsigned long>(total.pairs), chunks.size(), total.pack_ms, total.h2d_ms,
-            total.kernel_ms, layer_set != nullptr ? " and layer filters" : "", points_ms);
+            total.kernel_ms, layer_set ? " and layer filters" : "", points_ms);
   }
 
  public:

@@ -560,16 +560,18 @@ bool Polisher::align_layers_on_device(std::vector<std::unique_ptr<Overlap>>& ove
       quality_bytes += sequence->reverse_quality().size();
     }
   }
-  std::vector<char> qualities(quality_bytes);
-  auto copy_qualities = [&](uint64_t begin, uint64_t end) {
-    for (uint64_t q = begin; q < end; ++q) {
-      if (quality_at[q] == kNoQuality) continue;
-      const std::string& s =
-          stored_reversed[q] ? sequences_[q]->reverse_quality() : sequences_[q]->quality();
-      std::copy(s.begin(), s.end(), qualities.begin() + quality_at[q]);
-    }
-  };
-  {
+  // (run by the engine, possibly beside the first alignments)
+  auto gather_qualities = [&](std::vector<char>* out) {
+    std::vector<char>& qualities = *out;
+    qualities.resize(quality_bytes);
+    auto copy_qualities = [&](uint64_t begin, uint64_t end) {
+      for (uint64_t q = begin; q < end; ++q) {
+        if (quality_at[q] == kNoQuality) continue;
+        const std::string& s =
+            stored_reversed[q] ? sequences_[q]->reverse_quality() : sequences_[q]->quality();
+        std::copy(s.begin(), s.end(), qualities.begin() + quality_at[q]);
+      }
+    };
     const uint64_t n = sequences_.size();
     const uint64_t workers = thread_pool_->size();
     if (workers <= 1 || quality_bytes < (1u << 24)) {
@@ -582,7 +584,7 @@ bool Polisher::align_layers_on_device(std::vector<std::unique_ptr<Overlap>>& ove
       }
       for (auto& f : copies) f.wait();
     }
-  }
+  };
 
   std::vector<AlignTask> tasks;
   std::vector<uint64_t> task_overlap;
@@ -600,12 +602,11 @@ bool Polisher::align_layers_on_device(std::vector<std::unique_ptr<Overlap>>& ove
     src.mirrored = (o.strand() != 0) != (stored_reversed[o.q_id()] != 0) ? 1 : 0;
   }
   fprintf(stderr,
-          "[racon_amd::layers] %.1f MB of read qualities and %lu layer sources gathered in "
-          "%.1f ms\n",
-          quality_bytes / 1e6, static_cast<unsigned long>(sources.size()),
+          "[racon_amd::layers] %lu layer sources (%.1f MB of read qualities) set up in %.1f ms\n",
+          static_cast<unsigned long>(sources.size()), quality_bytes / 1e6,
           1e3 * seconds_since(t_inputs));
   engine_->align_layers(tasks, sources, window_length_, quality_threshold_,
-                        first_window[targets_size], qualities, layers);
+                        first_window[targets_size], gather_qualities, layers);
   logger_->log("[racon::Polisher::initialize] aligned overlaps");
   return true;
 }
