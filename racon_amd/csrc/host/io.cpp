@@ -121,6 +121,7 @@ struct Lines {
   size_t pos = 0;
 
   explicit Lines(const FileText& t) : text(t.data()), size(t.size()) {}
+  Lines(const char* t, size_t n) : text(t), size(n) {}
 
   // Next line without the newline and trailing whitespace; false at end.
   bool next(const char** begin, size_t* length) {
@@ -314,46 +315,93 @@ void parse_sequences(const std::string& path, SequenceFormat format,
 }
 
 void parse_overlaps(const std::string& path, OverlapFormat format,
-                    std::vector<std::unique_ptr<Overlap>>& dst) {
+                    std::vector<std::unique_ptr<Overlap>>& dst, ThreadPool* pool) {
   FileText text(path);
-  Lines lines(text);
-  const char* line;
-  size_t length;
-  while (lines.next(&line, &length)) {
+  // one line's overlap (none for an empty or header line); false: malformed
+  auto parse_line = [format](const char* line, size_t length,
+                             std::unique_ptr<Overlap>* out) -> bool {
     if (length == 0) {
-      continue;
+      return true;
     }
     if (format == OverlapFormat::kMhap) {
       auto f = split(line, length, " \t");
       if (f.size() < 12) {
-        malformed(path);
+        return false;
       }
-      dst.emplace_back(new Overlap(
+      out->reset(new Overlap(
           to_u64(f[0]), to_u64(f[1]), to_double(f[2]), to_u64(f[3]),
           to_u64(f[4]), to_u64(f[5]), to_u64(f[6]), to_u64(f[7]), to_u64(f[8]),
           to_u64(f[9]), to_u64(f[10]), to_u64(f[11])));
     } else if (format == OverlapFormat::kPaf) {
       auto f = split(line, length, "\t");
       if (f.size() < 12) {
-        malformed(path);
+        return false;
       }
-      dst.emplace_back(new Overlap(
+      out->reset(new Overlap(
           f[0].first, f[0].second, to_u64(f[1]), to_u64(f[2]), to_u64(f[3]),
           f[4].second > 0 ? f[4].first[0] : '+', f[5].first, f[5].second,
           to_u64(f[6]), to_u64(f[7]), to_u64(f[8]), to_u64(f[9]),
           to_u64(f[10]), to_u64(f[11])));
     } else {
       if (line[0] == '@') {
-        continue;
+        return true;
       }
       auto f = split(line, length, "\t");
       if (f.size() < 11) {
-        malformed(path);
+        return false;
       }
-      dst.emplace_back(new Overlap(f[0].first, f[0].second, to_u64(f[1]),
-                                   f[2].first, f[2].second, to_u64(f[3]),
-                                   to_u64(f[4]), f[5].first, f[5].second));
+      out->reset(new Overlap(f[0].first, f[0].second, to_u64(f[1]),
+                             f[2].first, f[2].second, to_u64(f[3]),
+                             to_u64(f[4]), f[5].first, f[5].second));
     }
+    return true;
+  };
+  // the text cut into ranges at line starts, parsed on the pool; the
+  // overlaps are then appended range by range (file order)
+  const size_t workers = pool != nullptr ? pool->size() : 1;
+  const size_t ranges = workers <= 1 || text.size() < (1u << 20) ? 1 : 4 * workers;
+  std::vector<size_t> cuts{0};
+  for (size_t k = 1; k < ranges; ++k) {
+    const size_t at = std::max(cuts.back(), text.size() / ranges * k);
+    const void* nl = at < text.size() ? memchr(text.data() + at, '\n', text.size() - at) : nullptr;
+    const size_t cut = nl != nullptr
+                           ? static_cast<size_t>(static_cast<const char*>(nl) - text.data()) + 1
+                           : text.size();
+    if (cut > cuts.back() && cut < text.size()) cuts.push_back(cut);
+  }
+  cuts.push_back(text.size());
+  std::vector<std::vector<std::unique_ptr<Overlap>>> parts(cuts.size() - 1);
+  std::vector<uint8_t> bad(parts.size(), 0);
+  auto run = [&](size_t r) {
+    Lines lines(text.data() + cuts[r], cuts[r + 1] - cuts[r]);
+    const char* line;
+    size_t length;
+    while (lines.next(&line, &length)) {
+      std::unique_ptr<Overlap> o;
+      if (!parse_line(line, length, &o)) {
+        bad[r] = 1;
+        return;
+      }
+      if (o != nullptr) parts[r].emplace_back(std::move(o));
+    }
+  };
+  if (parts.size() == 1) {
+    run(0);
+  } else {
+    std::vector<std::future<void>> done;
+    for (size_t r = 0; r < parts.size(); ++r) done.emplace_back(pool->submit(run, r));
+    for (auto& f : done) f.wait();
+  }
+  size_t total = dst.size();
+  for (size_t r = 0; r < parts.size(); ++r) {
+    if (bad[r]) {
+      malformed(path);
+    }
+    total += parts[r].size();
+  }
+  dst.reserve(total);
+  for (auto& part : parts) {
+    for (auto& o : part) dst.emplace_back(std::move(o));
   }
 }
 

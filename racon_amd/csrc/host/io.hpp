@@ -47,7 +47,9 @@ class FileText {
 // built on its threads (same objects, same order).
 void parse_sequences(const std::string& path, SequenceFormat format,
                      std::vector<std::unique_ptr<Sequence>>& dst, ThreadPool* pool = nullptr);
+// Appends the file's overlaps to dst in file order; with a pool, the lines
+// are parsed on its threads in ranges.
 void parse_overlaps(const std::string& path, OverlapFormat format,
-                    std::vector<std::unique_ptr<Overlap>>& dst);
+                    std::vector<std::unique_ptr<Overlap>>& dst, ThreadPool* pool = nullptr);
 
 }  // namespace racon

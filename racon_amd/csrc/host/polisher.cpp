@@ -235,7 +235,7 @@ void Polisher::initialize() {
 
   {
     // polisher.cpp:310-355 with the whole file as one chunk
-    parse_overlaps(overlaps_path_, overlaps_format_, overlaps);
+    parse_overlaps(overlaps_path_, overlaps_format_, overlaps, thread_pool_.get());
     uint64_t c = 0;
     for (uint64_t i = 0; i < overlaps.size(); ++i) {
       overlaps[i]->transmute(sequences_, name_to_id, id_to_id);
