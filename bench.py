@@ -7,8 +7,11 @@ true positions, window 500, scores 3/-5/-4. The inputs are generated on the
 box (racon_amd/bin/simgen, seed 1002), and the product CLI aligns the
 overlaps on the GPU and writes the windows. A step is racon's polish phase
 over every window through the C ABI: rh_poa_upload (host pack + H2D),
-rh_poa_run (the persistent POA kernel) and rh_poa_download (D2H); the
-kernel-only time (HIP events on the batch's stream) is reported beside it.
+rh_poa_run (the persistent POA kernel) and rh_poa_download (D2H). Steps
+alternate between two batches (step k+1's transfers beside step k's
+kernels, as the CLI's chunks run), all inside the clock; --sequential times
+them one after the other on one batch. The kernel-only time (HIP events on
+the batch's stream) is reported beside it.
 Also reported: the static-grid comparator (one workgroup per window, one
 launch), the plain mode (one launch per window), the overlap aligner, and
 the CPU baseline (racon's CPU path restated, 1 and T threads). Printed: one
@@ -302,6 +305,9 @@ def main() -> int:
                     help="skip the banded (-b) comparison run")
     ap.add_argument("--skip-serving-probe", action="store_true",
                     help="skip the 4-second serving-loop overload run")
+    ap.add_argument("--sequential", action="store_true",
+                    help="polish-phase steps one after the other on one batch (default: two "
+                         "batches, each step's transfers beside the other step's kernels)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="profiling runs: time only rh_poa_run (inputs resident), skip the "
                          "comparators and the CPU baseline")
@@ -393,28 +399,80 @@ def main() -> int:
 
     # ---- the polish phase: pack + H2D + kernels + D2H per step ----
     bufs = batch.output_buffers(2 * batch.lib.rh_poa_output_bound(batch.handle))
-
-    def step():
-        batch.upload_packed(arr, n_windows, keep)
-        batch.run(racon_amd.RH_POA_BATCHED)
-        batch.download_into(bufs)
-
-    for i in range(args.warmup):
-        step()
-        log(f"warmup {i + 1}/{args.warmup} done")
+    # steps alternate between two batches, as the CLI's chunks do: step k + 1
+    # is packed and copied while step k's kernels run, and step k's results
+    # come back while step k + 1's run; the clock covers every step's pack,
+    # H2D, kernels and D2H, the first upload and the last download included
+    batches = [batch]
+    if not args.kernel_only and not args.sequential:
+        second = racon_amd.PoaBatch(m, x, g, trim=True)
+        second.upload_packed(arr, n_windows, keep)
+        if tuned is not None:
+            second.run(racon_amd.RH_POA_BATCHED)
+            log(f"autotune (second batch): {second.autotune()}")
+        batches.append(second)
+    all_bufs = [bufs] + [b.output_buffers(2 * b.lib.rh_poa_output_bound(b.handle))
+                         for b in batches[1:]]
     step_ms, parts = [], []
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        if args.kernel_only:
-            batch.run(racon_amd.RH_POA_BATCHED)
-        else:
-            step()
-        step_ms.append((time.perf_counter() - t0) * 1e3)
-        st = batch.stats()
+    last_bufs = bufs
+    seq_ms = []
+
+    def record(b, i, ms):
+        st = b.stats()
         parts.append((st.pack_ms, st.h2d_ms, st.kernel_ms, st.d2h_ms))
-        log(f"step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms (pack {st.pack_ms:.1f}, H2D "
+        log(f"step {i + 1}/{args.steps}: {ms:.1f} ms (pack {st.pack_ms:.1f}, H2D "
             f"{st.h2d_ms:.1f}, kernel {st.kernel_ms:.1f}, D2H {st.d2h_ms:.1f} ms; slots "
             f"{st.slots}, retried {st.retried})")
+
+    def sequential_steps(n, timed):
+        for i in range(n):
+            t0 = time.perf_counter()
+            if args.kernel_only:
+                batch.run(racon_amd.RH_POA_BATCHED)
+            else:
+                batch.upload_packed(arr, n_windows, keep)
+                batch.run(racon_amd.RH_POA_BATCHED)
+                batch.download_into(bufs)
+            if timed:
+                step_ms.append((time.perf_counter() - t0) * 1e3)
+                record(batch, i, step_ms[-1])
+
+    def pipelined_steps(n, timed):
+        t0 = time.perf_counter()
+        batches[0].upload_packed(arr, n_windows, keep)
+        batches[0].submit(racon_amd.RH_POA_BATCHED)
+        for k in range(n):
+            cur, nxt = batches[k % 2], batches[(k + 1) % 2]
+            if k + 1 < n:
+                nxt.upload_packed(arr, n_windows, keep)  # beside step k's kernels
+            cur.wait()
+            if k + 1 < n:
+                nxt.submit(racon_amd.RH_POA_BATCHED)
+            cur.download_into(all_bufs[k % 2])  # beside step k + 1's kernels
+            if timed:
+                record(cur, k, (time.perf_counter() - t0) * 1e3 / (k + 1))
+        if timed:
+            step_ms.extend([(time.perf_counter() - t0) * 1e3 / n] * n)
+        return all_bufs[(n - 1) % 2]
+
+    steps = sequential_steps if len(batches) == 1 else pipelined_steps
+    for i in range(args.warmup):
+        steps(len(batches), False)  # (every batch once)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    out_bufs = steps(args.steps, True)
+    if out_bufs is not None:
+        last_bufs = out_bufs
+    if len(batches) > 1:
+        # the same steps one after the other, for the record
+        for _ in range(2):
+            t0 = time.perf_counter()
+            batch.upload_packed(arr, n_windows, keep)
+            batch.run(racon_amd.RH_POA_BATCHED)
+            batch.download_into(bufs)
+            seq_ms.append((time.perf_counter() - t0) * 1e3)
+        log(f"sequential steps: {seq_ms}")
+        for b in batches[1:]:
+            b.close()
     st = batch.stats()
     phases = batch.phase_seconds()
     total_phase = sum(phases.values()) or 1.0
@@ -425,7 +483,8 @@ def main() -> int:
     sched = queue_report(windows, batch, st.slots, st.kernel_ms)
     if args.kernel_only:
         batch.download_into(bufs)
-    gpu_out, gpu_status = batch.split(bufs, n_windows)
+        last_bufs = bufs
+    gpu_out, gpu_status = batch.split(last_bufs, n_windows)
     cells = st.cells
     ms_per_step = sum(step_ms) / len(step_ms)
     kernel_avg = sum(p[2] for p in parts) / len(parts)
@@ -567,10 +626,15 @@ def main() -> int:
                    "gpu_windows": int(st.gpu_windows), "overlaps": len(pairs),
                    "window_length": 500, "scores": [m, x, g], "seed": CONFIGS[args.config]["seed"],
                    "step": ("rh_poa_run only (inputs resident)" if args.kernel_only else
-                            "polish phase: rh_poa_upload (pack + H2D) + rh_poa_run (kernels) + "
+                            ("polish phase, one step after the other: " if args.sequential
+                             else "polish phase, steps pipelined over two batches (step k+1's "
+                             "pack + H2D and step k's D2H beside the kernels; all inside the "
+                             "clock): ") +
+                            "rh_poa_upload (pack + H2D) + rh_poa_run (kernels) + "
                             "rh_poa_download (D2H)")},
         "step_ms_parts": {"pack": round(pack_ms, 3), "h2d": round(h2d_ms, 3),
                           "kernel": round(kernel_avg, 3), "d2h": round(d2h_ms, 3)},
+        "sequential_ms_per_step": round(min(seq_ms), 3) if seq_ms else None,
         "kernel_ms": round(kernel_avg, 3),
         "kernel_windows_per_s": round(kernel_wps, 2),
         "poa_gcups": round(poa_gcups, 3),
