@@ -176,13 +176,14 @@ int32_t rh_align_upload(rh_align_batch* b, uint32_t n, const rh_pair* pairs) {
   b->stats.pack_ms = rh::ms_since(t0);
 
   auto t1 = std::chrono::steady_clock::now();
-  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(b->arena.size(), 1)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_pairs.reserve(std::max<uint64_t>(n, 1) * sizeof(rh::AlignPair)),
+  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(b->arena.size(), 1), true), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_pairs.reserve(std::max<uint64_t>(n, 1) * sizeof(rh::AlignPair), true),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(rh::AlignResult)),
+  RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(rh::AlignResult), true),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_ops.reserve(std::max<uint64_t>(out, 1)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_order.reserve(std::max<uint64_t>(n, 1) * sizeof(uint32_t)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_ops.reserve(std::max<uint64_t>(out, 1), true), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_order.reserve(std::max<uint64_t>(n, 1) * sizeof(uint32_t), true),
+               RH_ERR_MEMORY);
   if (!b->arena.empty()) {
     RH_HIP_CHECK(hipMemcpyAsync(b->d_arena.ptr, b->arena.data(), b->arena.size(),
                                 hipMemcpyHostToDevice, b->stream),

@@ -203,7 +203,11 @@ inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyK
 struct DeviceBuffer {
   void* ptr = nullptr;
   uint64_t bytes = 0;
-  hipError_t reserve(uint64_t n) {
+  // Room for n bytes. A buffer that must grow is freed and allocated again,
+  // and hipFree waits for the whole device: a buffer refilled per chunk
+  // while another stream's kernels run takes a quarter of headroom
+  // (grow_headroom), so that chunks of about one size reuse it.
+  hipError_t reserve(uint64_t n, bool grow_headroom = false) {
     if (n <= bytes) {
       return hipSuccess;
     }
@@ -213,9 +217,10 @@ struct DeviceBuffer {
       bytes = 0;
       if (e != hipSuccess) return e;
     }
-    hipError_t e = hipMalloc(&ptr, n == 0 ? 1 : n);
+    const uint64_t want = grow_headroom ? n + n / 4 : n;
+    hipError_t e = hipMalloc(&ptr, want == 0 ? 1 : want);
     if (e == hipSuccess) {
-      bytes = n;
+      bytes = want;
     }
     return e;
   }

@@ -542,8 +542,11 @@ struct rh_poa_batch {
 
   rh::DeviceBuffer d_arena, d_windows, d_layers, d_out, d_results;
   // pinned staging of the input bytes, from the batch's second upload on
-  // (a batch object that is reused pays the pinning once)
+  // (a batch object that is reused pays the pinning once); it is allocated
+  // on a helper thread right after the first upload, so that the pinning
+  // runs beside the first run instead of inside the second upload
   rh::PinnedBuffer staging;
+  std::thread staging_maker;
   uint64_t uploads = 0;
   bool uploaded = false;
   bool ran = false;
@@ -631,6 +634,7 @@ void rh_poa_destroy(rh_poa_batch* b) {
   b->d_layers.release();
   b->d_out.release();
   b->d_results.release();
+  if (b->staging_maker.joinable()) b->staging_maker.join();
   b->staging.release();
   delete b;
 }
@@ -660,6 +664,7 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   auto t0 = std::chrono::steady_clock::now();
   b->stats = rh_poa_stats{};
   b->uploaded = false;
+  if (b->staging_maker.joinable()) b->staging_maker.join();
   const int32_t rc =
       rh::pack_windows(n, windows, b->trim, &b->packed, b->uploads > 0 ? &b->staging : nullptr);
   if (rc != RH_OK) {
@@ -679,14 +684,15 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
 
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t st = b->lane.stream;
-  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(P.arena_bytes, 1)), RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_windows.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaWindow)),
+  RH_HIP_CHECK(b->d_arena.reserve(std::max<uint64_t>(P.arena_bytes, 1), true), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_windows.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaWindow), true),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_layers.reserve(std::max<size_t>(P.layers.size(), 1) * sizeof(rh::PoaLayer)),
+  RH_HIP_CHECK(
+      b->d_layers.reserve(std::max<size_t>(P.layers.size(), 1) * sizeof(rh::PoaLayer), true),
+      RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaResult), true),
                RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_results.reserve(std::max<uint64_t>(n, 1) * sizeof(PoaResult)),
-               RH_ERR_MEMORY);
-  RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(P.out_bytes, 1)), RH_ERR_MEMORY);
+  RH_HIP_CHECK(b->d_out.reserve(std::max<uint64_t>(P.out_bytes, 1), true), RH_ERR_MEMORY);
   if (!P.staged_copies.empty()) {
     RH_HIP_CHECK(rh::copy_and_upload(b->staging.ptr, P.staged_copies, P.arena_bytes,
                                      b->d_arena.as<uint8_t>(), st),
@@ -711,6 +717,15 @@ int32_t rh_poa_upload(rh_poa_batch* b, uint32_t n, const rh_window* windows) {
   b->stats.h2d_ms = rh::ms_since(t1);
   b->uploaded = true;
   b->ran = false;
+  if (b->uploads == 1 && P.arena_bytes > 0) {
+    // (a failed pinning leaves the staging empty: the next upload tries
+    // again and falls back to pageable copies)
+    const uint64_t bytes = P.arena_bytes;
+    rh::PinnedBuffer* staging = &b->staging;
+    b->staging_maker = std::thread([staging, bytes] {
+      if (staging->reserve(bytes) != hipSuccess) (void)hipGetLastError();
+    });
+  }
   return RH_OK;
 }
 

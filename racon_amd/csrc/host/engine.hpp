@@ -139,8 +139,9 @@ class PolishEngine {
 
   // Generates the consensus of every window (set through
   // Window::set_consensus) and fills status[i] with kWindow* codes. ready(n)
-  // is called (from the calling thread, n increasing, the last call with
-  // windows.size()) once windows [0, n) and their status are final, so the
+  // is called (one call at a time, n increasing, the last call with
+  // windows.size(), all before generate_consensus returns; possibly from a
+  // helper thread) once windows [0, n) and their status are final, so the
   // caller can emit finished targets while later windows still run.
   virtual void generate_consensus(std::vector<std::shared_ptr<Window>>& windows,
                                   bool trim, std::vector<uint8_t>& status,

@@ -27,7 +27,8 @@ namespace {
 
 class HipEngine : public PolishEngine {
  public:
-  explicit HipEngine(const EngineConfig& config) : config_(config) {
+  explicit HipEngine(const EngineConfig& config)
+      : config_(config), chunk_log_(getenv("RACON_AMD_CHUNK_LOG") != nullptr) {
     if (rh_device_count() < 1) {
       fprintf(stderr, "[racon::HipEngine] error: no HIP device available!\n");
       exit(1);
@@ -230,6 +231,13 @@ class HipEngine : public PolishEngine {
       if (up_rc != RH_OK) die("rh_align_upload", up_rc);
       rh_align_stats st{};
       if (rh_align_get_stats(sl.batch, &st) == RH_OK) {
+        if (chunk_log_) {
+          fprintf(stderr,
+                  "[racon_amd::align] chunk %lu: %lu pairs, pack %.1f ms, H2D %.1f ms, kernels "
+                  "%.1f ms\n",
+                  static_cast<unsigned long>(c), static_cast<unsigned long>(st.pairs), st.pack_ms,
+                  st.h2d_ms, st.kernel_ms);
+        }
         total.pack_ms += st.pack_ms;
         total.h2d_ms += st.h2d_ms;
         total.kernel_ms += st.kernel_ms;
@@ -348,10 +356,16 @@ class HipEngine : public PolishEngine {
       return rh_poa_upload(sl.batch, static_cast<uint32_t>(sl.windows.size()),
                            sl.windows.data());
     };
-    std::vector<char> out;
-    std::vector<uint64_t> offsets;
-    std::vector<uint32_t> lengths;
-    std::vector<uint8_t> codes;
+    // results of two chunks: chunk c's are handed out (take_result,
+    // ready) on a helper thread while the main thread uploads chunk c + 2
+    struct Results {
+      std::vector<char> out;
+      std::vector<uint64_t> offsets;
+      std::vector<uint32_t> lengths;
+      std::vector<uint8_t> codes;
+    };
+    Results results[2];
+    std::thread taker;
     int32_t rc = chunks.empty() ? RH_OK : upload(0);
     if (rc != RH_OK) die("rh_poa_upload", rc);
     if (!chunks.empty()) {
@@ -362,7 +376,8 @@ class HipEngine : public PolishEngine {
       rh_poa_batch* batch = slots[c % n_batches].batch;
       // chunk c's kernels run while the next chunk is packed and copied
       // here; chunk c + 1's kernels are queued before chunk c's results are
-      // downloaded and handed out, so the device does not wait for the host
+      // downloaded, and those are handed out on a helper thread while the
+      // next upload runs here, so the device does not wait for the host
       if (c + 1 < chunks.size()) {
         rc = upload(c + 1);
         if (rc != RH_OK) die("rh_poa_upload", rc);
@@ -375,16 +390,25 @@ class HipEngine : public PolishEngine {
       }
       const uint64_t first = chunks[c].first, last = chunks[c].second;
       const uint64_t n = last - first;
-      out.resize(rh_poa_output_bound(batch));
-      offsets.resize(n);
-      lengths.resize(n);
-      codes.resize(n);
-      rc = rh_poa_download(batch, out.data(), out.size(), offsets.data(), lengths.data(),
-                           codes.data());
+      if (taker.joinable()) taker.join();  // chunk c - 1 handed out
+      Results& r = results[c % 2];
+      r.out.resize(rh_poa_output_bound(batch));
+      r.offsets.resize(n);
+      r.lengths.resize(n);
+      r.codes.resize(n);
+      rc = rh_poa_download(batch, r.out.data(), r.out.size(), r.offsets.data(),
+                           r.lengths.data(), r.codes.data());
       if (rc != RH_OK) die("rh_poa_download", rc);
       device_bytes = std::max<uint64_t>(device_bytes, rh_poa_device_bytes(batch));
       rh_poa_stats st{};
       if (rh_poa_get_stats(batch, &st) == RH_OK) {
+        if (chunk_log_) {
+          fprintf(stderr,
+                  "[racon_amd::consensus] chunk %lu: %lu windows, %.0f MB, pack %.1f ms, H2D "
+                  "%.1f ms, kernels %.1f ms, D2H %.1f ms\n",
+                  static_cast<unsigned long>(c), static_cast<unsigned long>(n),
+                  st.input_bytes / 1e6, st.pack_ms, st.h2d_ms, st.kernel_ms, st.d2h_ms);
+        }
         total.pack_ms += st.pack_ms;
         total.h2d_ms += st.h2d_ms;
         total.kernel_ms += st.kernel_ms;
@@ -392,12 +416,17 @@ class HipEngine : public PolishEngine {
         total.windows += st.windows;
         total.retried += st.retried;
       }
-      for (uint64_t i = first; i < last; ++i) {
-        const uint64_t k = i - first;
-        take_result(windows, i, out.data() + offsets[k], lengths[k], codes[k], status);
-      }
-      if (ready) ready(last);
+      const Results* rp = &r;
+      taker = std::thread([&windows, &status, &ready, rp, first, last] {
+        for (uint64_t i = first; i < last; ++i) {
+          const uint64_t k = i - first;
+          take_result(windows, i, rp->out.data() + rp->offsets[k], rp->lengths[k],
+                      rp->codes[k], status);
+        }
+        if (ready) ready(last);
+      });
     }
+    if (taker.joinable()) taker.join();
     if (ready && chunks.empty()) ready(0);
     for (uint32_t k = 0; k < n_batches; ++k) rh_poa_destroy(slots[k].batch);
     // where the consensus phase went (the next chunk's upload overlaps the
@@ -506,6 +535,9 @@ class HipEngine : public PolishEngine {
   }
 
   EngineConfig config_;
+  // RACON_AMD_CHUNK_LOG set: one stderr line per chunk (upload parts and
+  // kernel time), to see which step holds a pipeline up
+  bool chunk_log_ = false;
 };
 
 }  // namespace
