@@ -322,15 +322,13 @@ void Polisher::initialize() {
     for (uint32_t j = 0; j < sequences_[i]->data().size(); j += window_length_, ++k) {
       uint32_t length = std::min(j + window_length_,
                                  static_cast<uint32_t>(sequences_[i]->data().size())) - j;
+      const uint64_t w = windows_.size();
       windows_.emplace_back(createWindow(
           i, k, window_type, &(sequences_[i]->data()[j]), length,
           sequences_[i]->quality().empty() ? &(dummy_quality_[0])
                                            : &(sequences_[i]->quality()[j]),
-          length));
-      if (on_device) {
-        const uint64_t w = windows_.size() - 1;
-        windows_.back()->reserve_layers(static_cast<uint32_t>(layer_at[w + 1] - layer_at[w]));
-      }
+          length,
+          on_device ? static_cast<uint32_t>(layer_at[w + 1] - layer_at[w]) : 0));
     }
     id_to_first_window_id[i + 1] = id_to_first_window_id[i] + k;
   }

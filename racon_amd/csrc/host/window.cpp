@@ -8,7 +8,8 @@ namespace racon {
 
 std::shared_ptr<Window> createWindow(uint64_t id, uint32_t rank, WindowType type,
                                      const char* backbone, uint32_t backbone_length,
-                                     const char* quality, uint32_t quality_length) {
+                                     const char* quality, uint32_t quality_length,
+                                     uint32_t layers) {
   // window.cpp:15-27
   if (backbone_length == 0 || backbone_length != quality_length) {
     fprintf(stderr, "[racon::createWindow] error: "
@@ -16,13 +17,14 @@ std::shared_ptr<Window> createWindow(uint64_t id, uint32_t rank, WindowType type
     exit(1);
   }
   return std::make_shared<Window>(id, rank, type, backbone, backbone_length,
-                                  quality, quality_length);
+                                  quality, quality_length, layers);
 }
 
 Window::Window(uint64_t id, uint32_t rank, WindowType type, const char* backbone,
                uint32_t backbone_length, const char* quality,
-               uint32_t quality_length)
+               uint32_t quality_length, uint32_t layers)
     : id_(id), rank_(rank), type_(type) {
+  if (layers > 0) reserve_layers(layers + 1);
   sequences_.emplace_back(backbone, backbone_length);
   qualities_.emplace_back(quality, quality_length);
   positions_.emplace_back(0, 0);

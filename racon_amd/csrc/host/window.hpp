@@ -22,14 +22,17 @@ enum class WindowType {
 };
 
 class Window;
+// layers: room for that many add_layer calls (0: grow as they come).
 std::shared_ptr<Window> createWindow(uint64_t id, uint32_t rank, WindowType type,
                                      const char* backbone, uint32_t backbone_length,
-                                     const char* quality, uint32_t quality_length);
+                                     const char* quality, uint32_t quality_length,
+                                     uint32_t layers = 0);
 
 class Window {
  public:
   Window(uint64_t id, uint32_t rank, WindowType type, const char* backbone,
-         uint32_t backbone_length, const char* quality, uint32_t quality_length);
+         uint32_t backbone_length, const char* quality, uint32_t quality_length,
+         uint32_t layers = 0);
   Window(const Window&) = delete;
   Window& operator=(const Window&) = delete;
 
